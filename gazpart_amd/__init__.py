@@ -1,0 +1,251 @@
+"""gazpart_amd -- MI355X (gfx950) Lagrangian ellipsoid step of GAZPART.
+
+The compute path is the HIP library ``libgazpart_gpu.so`` (built in-tree by
+``gazpart_amd._build``); this module binds its C-ABI (include/gazpart_gpu.h)
+with ctypes and mirrors the reference's subroutine interface for this path:
+
+  ===============================  =====================================
+  reference (source/)              here
+  ===============================  =====================================
+  INTERPH  interpH.f90:26          :meth:`Context.interph`
+  ADV_PARTICLE_ELLP  adv_particle_new.f90:7
+                                   :meth:`Context.adv_particle_ellp`
+  COLLISION  collision.f90:7       :meth:`Context.collision`
+  COLLISION_DETECTION_CONNECTIVITY collision_detection_connectivity.f90:7
+                                   :meth:`Context.collision_detection_connectivity`
+  ellipsoid_contact_detection  ellipsoid_contact_detection.f90:1
+                                   :meth:`Context.ellipsoid_contact_detection`
+  ===============================  =====================================
+
+There is no CPU fallback: if the library is missing or no GPU is visible,
+constructing a :class:`Context` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgazpart_gpu.so")
+
+GP_OK = 0
+ERRORS = {
+    1: "GP_ERR_NAN_DRAG", 2: "GP_ERR_NAN_OMEGA", 3: "GP_ERR_KE_COLLISION",
+    4: "GP_ERR_KE_WALL", 5: "GP_ERR_OUT_OF_GRID", 6: "GP_ERR_NCLOSE_OVERFLOW",
+    7: "GP_ERR_INTERP_RANGE", 10: "GP_ERR_ARG", 11: "GP_ERR_STATE", 12: "GP_ERR_HIP",
+    13: "GP_ERR_NOMEM",
+}
+PREC_FP64, PREC_MIXED = 0, 1
+
+# every symbol include/gazpart_gpu.h declares
+EXPORTS = (
+    "gp_create", "gp_destroy", "gp_last_error_string", "gp_sync", "gp_bind_field",
+    "gp_set_species", "gp_set_particles", "gp_get_particles", "gp_get_fluid_at_particles",
+    "gp_interph", "gp_adv_particle_ellp", "gp_collision_detection", "gp_contact_batch",
+    "gp_collision", "gp_step", "gp_get_stats", "gp_collision_grid",
+)
+
+
+class GpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    """gp_params (include/gazpart_gpu.h)."""
+    _fields_ = [("n", C.c_int), ("interp_scheme", C.c_int), ("precision", C.c_int),
+                ("renorm_quat", C.c_int), ("L", C.c_double), ("visc", C.c_double),
+                ("rhof", C.c_double), ("dtime", C.c_double), ("ecw", C.c_double),
+                ("muw", C.c_double), ("dpmax", C.c_double), ("prefilter", C.c_int),
+                ("xcd_remap", C.c_int), ("reserved", C.c_int * 6)]
+
+
+class Stats(C.Structure):
+    """gp_stats (include/gazpart_gpu.h)."""
+    _fields_ = [("nclose", C.c_int64), ("nprefilter", C.c_int64), ("ncontact", C.c_int64),
+                ("ncollision", C.c_int64), ("lm_nfev", C.c_int64), ("nwall", C.c_int64),
+                ("greedy_rounds", C.c_int64), ("reserved", C.c_int64 * 9)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} missing: build it with `python -m gazpart_amd._build`")
+    L = C.CDLL(path)
+    vp, dp, ip = C.c_void_p, C.c_void_p, C.c_void_p
+    L.gp_create.argtypes = [C.POINTER(Params), C.POINTER(C.c_void_p)]
+    L.gp_destroy.argtypes = [vp]
+    L.gp_last_error_string.argtypes = [vp]
+    L.gp_last_error_string.restype = C.c_char_p
+    L.gp_sync.argtypes = [vp]
+    L.gp_bind_field.argtypes = [vp, dp]
+    L.gp_set_species.argtypes = [vp, C.c_int, dp]
+    L.gp_set_particles.argtypes = [vp, C.c_int, dp, ip]
+    L.gp_get_particles.argtypes = [vp, C.c_int, dp]
+    L.gp_get_fluid_at_particles.argtypes = [vp, C.c_int, dp]
+    L.gp_interph.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, C.c_int, dp]
+    L.gp_adv_particle_ellp.argtypes = [vp, C.c_int]
+    L.gp_collision_detection.argtypes = [vp, C.c_int64, ip, ip, C.POINTER(C.c_int64)]
+    L.gp_contact_batch.argtypes = [vp, C.c_int64, ip, ip, vp, dp, dp, dp, ip]
+    L.gp_collision.argtypes = [vp, C.c_int, C.c_double]
+    L.gp_step.argtypes = [vp, C.c_int]
+    L.gp_get_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.gp_collision_grid.argtypes = [vp, ip, dp]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+class Context:
+    """One device context: field, species table and particles resident on
+    the GPU, plus the reference-shaped entry points of the step."""
+
+    def __init__(self, n, L, visc, rhof, dtime, *, scheme=3, precision=PREC_FP64, ecw=1.0,
+                 muw=0.0, dpmax=0.0, prefilter=1, xcd_remap=1, renorm_quat=0,
+                 keep_fluid=False):
+        self.lib = load_library()
+        p = Params()
+        p.n, p.interp_scheme, p.precision, p.renorm_quat = n, scheme, precision, renorm_quat
+        p.L, p.visc, p.rhof, p.dtime, p.ecw, p.muw, p.dpmax = L, visc, rhof, dtime, ecw, muw, dpmax
+        p.prefilter, p.xcd_remap = prefilter, xcd_remap
+        p.reserved[0] = 1 if keep_fluid else 0
+        self.params = p
+        h = C.c_void_p()
+        rc = self.lib.gp_create(C.byref(p), C.byref(h))
+        if rc:
+            raise GpError(rc, "gp_create failed (no GPU visible?)")
+        self.h = h
+        self.np = 0
+
+    @classmethod
+    def from_workload(cls, wl, **kw):
+        ctx = cls(wl.n, wl.L, wl.visc, wl.rhof, wl.dtime, scheme=wl.scheme, ecw=wl.ecw,
+                  muw=wl.muw, dpmax=wl.dpmax, **kw)
+        ctx.bind_field(wl.field)
+        ctx.set_species(wl.species)
+        ctx.set_particles(wl.state, wl.species_id)
+        return ctx
+
+    def _chk(self, rc):
+        if rc:
+            msg = self.lib.gp_last_error_string(self.h)
+            raise GpError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- data ------------------------------------------------------------
+    def bind_field(self, field):
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        n = self.params.n
+        assert f.shape == (6, n + 4, n + 4, n), f.shape
+        self._chk(self.lib.gp_bind_field(self.h, _ptr(f)))
+
+    def set_species(self, rows):
+        r = np.ascontiguousarray(np.atleast_2d(rows), dtype=np.float64)
+        self._chk(self.lib.gp_set_species(self.h, r.shape[0], _ptr(r)))
+
+    def set_particles(self, state, species):
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        assert st.shape[0] == 13 and sp.shape == (st.shape[1],)
+        self._chk(self.lib.gp_set_particles(self.h, st.shape[1], _ptr(st), _ptr(sp)))
+        self.np = st.shape[1]
+
+    def get_particles(self):
+        out = np.empty((13, self.np))
+        self._chk(self.lib.gp_get_particles(self.h, self.np, _ptr(out)))
+        return out
+
+    def get_fluid_at_particles(self):
+        out = np.empty((6, self.np))
+        self._chk(self.lib.gp_get_fluid_at_particles(self.h, self.np, _ptr(out)))
+        return out
+
+    def sync(self):
+        self._chk(self.lib.gp_sync(self.h))
+
+    def stats(self):
+        s = Stats()
+        self._chk(self.lib.gp_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def collision_grid(self):
+        nb = np.zeros(4, dtype=np.int32)
+        bm = np.zeros(6)
+        self._chk(self.lib.gp_collision_grid(self.h, _ptr(nb), _ptr(bm)))
+        return nb, bm
+
+    # ---- reference-shaped entry points ----------------------------------
+    def interph(self, int_scheme, comp, xint, yint, zint):
+        """INTERPH (interpH.f90:26): field component comp (0..5 = u v w
+        wx wy wz) at the given positions; returns UINT."""
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (xint, yint, zint))
+        out = np.empty_like(x)
+        self._chk(self.lib.gp_interph(self.h, int_scheme, len(x), _ptr(x), _ptr(y), _ptr(z),
+                                      comp, _ptr(out)))
+        return out
+
+    def adv_particle_ellp(self, ncycle=1):
+        """ADV_PARTICLE_ELLP (adv_particle_new.f90:7), BOUNDARY_PARTICLE
+        included."""
+        self._chk(self.lib.gp_adv_particle_ellp(self.h, ncycle))
+
+    def collision(self, ncycle=1, time=0.0):
+        """COLLISION (collision.f90:7) at NPROC = 1."""
+        self._chk(self.lib.gp_collision(self.h, ncycle, time))
+
+    def step(self, nsteps=1):
+        """nsteps x (ADV_PARTICLE_ELLP + COLLISION), asynchronous."""
+        self._chk(self.lib.gp_step(self.h, nsteps))
+
+    def collision_detection_connectivity(self, nclosemax=None):
+        """COLLISION_DETECTION_CONNECTIVITY: 1-based (HOME, NEAR) in the
+        reference's order for the current positions."""
+        ncl = C.c_int64(0)
+        if nclosemax is None:
+            rc = self.lib.gp_collision_detection(self.h, 0, None, None, C.byref(ncl))
+            if rc not in (0, 6):
+                self._chk(rc)
+            nclosemax = ncl.value
+        home = np.zeros(max(nclosemax, 1), dtype=np.int32)
+        near = np.zeros(max(nclosemax, 1), dtype=np.int32)
+        self._chk(self.lib.gp_collision_detection(self.h, nclosemax, _ptr(home), _ptr(near),
+                                                  C.byref(ncl)))
+        return home[:ncl.value], near[:ncl.value]
+
+    def ellipsoid_contact_detection(self, home, near):
+        """ellipsoid_contact_detection on 1-based pairs of the resident
+        particles: (flag, EPT1, EPT2, cnormal, nfev)."""
+        h = np.ascontiguousarray(home, dtype=np.int32)
+        nr = np.ascontiguousarray(near, dtype=np.int32)
+        n = len(h)
+        flag = np.zeros(n, dtype=np.uint8)
+        e1, e2, cn = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        nfev = np.zeros(n, dtype=np.int32)
+        self._chk(self.lib.gp_contact_batch(self.h, n, _ptr(h), _ptr(nr), _ptr(flag), _ptr(e1),
+                                            _ptr(e2), _ptr(cn), _ptr(nfev)))
+        return flag, e1, e2, cn, nfev

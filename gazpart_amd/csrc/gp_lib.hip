@@ -1,0 +1,1398 @@
+// gp_lib.hip -- kernels and C-ABI of the MI355X Lagrangian ellipsoid step.
+//
+// Step structure (one HIP stream per context):
+//   k_advance      gather the 6 fields at x^n (lag3/lag4 stencil, LDS-free,
+//                  interleaved node layout), RK2 + drag/torque + quaternion
+//                  + x-walls + periodic wrap, then the particle's collision
+//                  cell and its slot in the cell (atomic count) -- fused.
+//   scan           cell counts -> cell offsets
+//   k_scatter/rank stable counting sort by (cell, particle index): the
+//                  reference bins particles in index order
+//                  (collision_detection_connectivity.f90:95-118)
+//   k_pair_count   candidate pairs per home particle, scan -> list offsets
+//                  (the list order of collision_detection_connectivity.f90:
+//                  145-207 is cell ascending, home in index order, self pairs,
+//                  then the 13 half-shell cells in CON order)
+//   k_prefilter    bounding-sphere test; survivors keep their list index
+//   k_contact      FP64 Levenberg-Marquardt contact solve + approach test
+//   k_greedy       one collision per particle in list order + impulses
+// Particles are stored cell-sorted: the next advance reads them through the
+// sort permutation, so the field gather of neighbouring threads touches
+// neighbouring stencils.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gazpart_gpu.h"
+#include "gp_contact.h"
+#include "gp_device.h"
+#include "gp_scan.h"
+
+namespace gp {
+
+constexpr int BLOCK = 256;
+
+// ---------------------------------------------------------------------------
+// device data structures
+// ---------------------------------------------------------------------------
+template <typename VT>
+struct PState {
+  double *x, *y, *z;
+  VT *u, *v, *w;
+  double *qa, *qb, *qc, *qd;
+  VT *ox, *oy, *oz;
+  int *sp;   // species row
+  int *id;   // reference index (0-based)
+};
+
+struct CellGrid {
+  int nx, ny, nz, nbox;
+  double xmin, ymin, zmin, dx, dy, dz;
+};
+
+struct ActiveRec {   // a contact that is approaching (flag and NVREL < ZERO)
+  long long k;       // index in the candidate list (its order)
+  int i, j;          // particle slots
+  double r1[3], r2[3], cn[3];
+};
+
+struct WorkRec {     // a candidate pair that passed the prefilter
+  long long k;
+  int i, j;
+};
+
+__device__ __forceinline__ int block_index(int xcd_remap) {
+  // AMD dispatches consecutive workgroups round-robin over the 8 XCDs;
+  // remapping gives each XCD a contiguous range of (cell-sorted) particles
+  // so that neighbouring stencils share that XCD's L2.
+  // (the launch pads the grid to a multiple of 8 blocks when remapping)
+  const int b = blockIdx.x;
+  if (!xcd_remap) return b;
+  const int per = gridDim.x >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
+__device__ __forceinline__ long long node_index(int n4, int i, int j, int k) {
+  return ((long long)(k + 1) * n4 + (j + 1)) * n4 + (i + 1);
+}
+
+// Lagrange weights of interp_lag3.f90:196-211 / interp_lag4.f90:148-166
+template <typename CT, int S>
+__device__ __forceinline__ void lweights(CT a, CT* f) {
+  if constexpr (S == 4) {
+    f[0] = -(CT(1.0 / 6.0) * (a - CT(2)) * (a - CT(1)) * a);
+    f[1] = CT(0.5) * (a - CT(2)) * (a - CT(1)) * (a + CT(1));
+    f[2] = -(CT(0.5) * (a - CT(2)) * a * (a + CT(1)));
+    f[3] = CT(1.0 / 6.0) * (a - CT(1)) * a * (a + CT(1));
+  } else {
+    f[0] = CT(1.0 / 24.0) * (a - CT(2)) * (a - CT(1)) * a * (a + CT(1));
+    f[1] = -(CT(1.0 / 6.0) * (a - CT(2)) * (a - CT(1)) * a * (a + CT(2)));
+    f[2] = CT(0.25) * (a - CT(2)) * (a - CT(1)) * (a + CT(1)) * (a + CT(2));
+    f[3] = -(CT(1.0 / 6.0) * (a - CT(2)) * a * (a + CT(1)) * (a + CT(2)));
+    f[4] = CT(1.0 / 24.0) * (a - CT(1)) * a * (a + CT(1)) * (a + CT(2));
+  }
+}
+
+// one x-row of S nodes x 6 components, contiguous in the interleaved layout
+template <typename FT, int S> struct Row;
+template <int S> struct Row<double, S> {
+  double v[S * 6];
+  __device__ __forceinline__ void load(const double* __restrict__ p) {
+    const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int i = 0; i < S * 3; ++i) {
+      double2 t = q[i];
+      v[2 * i] = t.x; v[2 * i + 1] = t.y;
+    }
+  }
+};
+template <int S> struct Row<float, S> {
+  float v[S * 6];
+  __device__ __forceinline__ void load(const float* __restrict__ p) {
+    const float2* q = reinterpret_cast<const float2*>(p);
+#pragma unroll
+    for (int i = 0; i < S * 3; ++i) {
+      float2 t = q[i];
+      v[2 * i] = t.x; v[2 * i + 1] = t.y;
+    }
+  }
+};
+
+// INTERP_LAG3 / INTERP_LAG4 of all six fields at once (interp_lag3.f90:
+// 119-290, interp_lag4.f90:84-312).  The field carries ghost planes in x as
+// well (index period n, the reference's x wrap of interp_lag3.f90:132-154),
+// so the stencil is branch-free.  For lag4 the reference's FYP2 row reads
+// J+1 (interp_lag4.f90:196,222,248,274,300): that row gets weight FYP1+FYP2.
+template <typename FT, typename CT, int S>
+__device__ __forceinline__ bool gather6(const FT* __restrict__ f, int n, double dx, double x,
+                                        double y, double z, CT out[6]) {
+  if (!(x == x && y == y && z == z)) return false;
+  const double tx = x / dx, ty = y / dx, tz = z / dx;
+  if (!(tx > -1.0 && ty > -1.0 && tz > -1.0 && tx < n && ty < n && tz < n)) return false;
+  const int i0 = (int)tx + 1, j0 = (int)ty + 1, k0 = (int)tz + 1;
+  if (i0 < 1 || i0 > n || j0 < 1 || j0 > n || k0 < 1 || k0 > n) return false;
+  CT fx[S], fy[S], fz[S];
+  lweights<CT, S>(CT((x - (double)(i0 - 1) * dx) / dx), fx);
+  lweights<CT, S>(CT((y - (double)(j0 - 1) * dx) / dx), fy);
+  lweights<CT, S>(CT((z - (double)(k0 - 1) * dx) / dx), fz);
+  constexpr int lo = (S == 4) ? -1 : -2;
+  constexpr int NY = (S == 4) ? 4 : 4;   // lag4: rows j0-2..j0+1 (J+2 never read)
+  CT wy[4];
+  if constexpr (S == 4) {
+    wy[0] = fy[0]; wy[1] = fy[1]; wy[2] = fy[2]; wy[3] = fy[3];
+  } else {
+    wy[0] = fy[0]; wy[1] = fy[1]; wy[2] = fy[2]; wy[3] = fy[3] + fy[4];
+  }
+  const int n4 = n + 4;
+  CT acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int kz = 0; kz < S; ++kz) {
+    CT sy[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int jy = 0; jy < NY; ++jy) {
+      Row<FT, S> r;
+      r.load(f + node_index(n4, i0 + lo, j0 + lo + jy, k0 + lo + kz) * 6);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        CT sx = 0;
+#pragma unroll
+        for (int ix = 0; ix < S; ++ix) sx += fx[ix] * CT(r.v[ix * 6 + c]);
+        sy[c] += wy[jy] * sx;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c] += fz[kz] * sy[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) out[c] = acc[c];
+  return true;
+}
+
+// collision cell of a position (collision_detection_connectivity.f90:74-78):
+// IEEE (x - xmin) / dx truncated, 1-based, as a 0-based box.  A position
+// outside the grid (the reference prints a warning and writes out of
+// bounds) raises *oob and is clamped to the nearest box, so that every
+// particle keeps a slot in the sort.
+__device__ __forceinline__ int axis_cell(double x, double x0, double d, int nmax, bool* oob) {
+  const double t = (x - x0) / d;
+  if (!(t > -1.0 && t < (double)nmax)) {   // also NaN
+    *oob = true;
+    return (t >= (double)nmax) ? nmax : 1;
+  }
+  const int i = (int)t + 1;
+  if (i < 1 || i > nmax) {
+    *oob = true;
+    return i < 1 ? 1 : nmax;
+  }
+  return i;
+}
+__device__ __forceinline__ int cell_of(const CellGrid& g, double x, double y, double z,
+                                       bool* oob) {
+  *oob = false;
+  const int i0 = axis_cell(x, g.xmin, g.dx, g.nx, oob);
+  const int j0 = axis_cell(y, g.ymin, g.dy, g.ny, oob);
+  const int k0 = axis_cell(z, g.zmin, g.dz, g.nz, oob);
+  return (i0 - 1) + (j0 - 1) * g.nx + (k0 - 1) * g.nx * g.ny;
+}
+
+// the 13-cell half shell of COLLISION_BOX_CONNECTIVITY
+// (collision_box_connectivity.f90:127-316) with the periodic wrap of a
+// single rank (all 13 present); 0-based boxes in CON order
+__device__ __forceinline__ void half_shell(const CellGrid& g, int c, int nb[13]) {
+  const int NX = g.nx, NY = g.ny, NZ = g.nz;
+  const int i0 = c % NX + 1, j0 = (c / NX) % NY + 1, k0 = c / (NX * NY) + 1;
+  int im1 = i0 - 1, ip1 = i0 + 1, jm1 = j0 - 1, jp1 = j0 + 1, km1 = k0 - 1, kp1 = k0 + 1;
+  if (im1 < 1) im1 = NX;
+  if (ip1 > NX) ip1 = 1;
+  if (jm1 < 1) jm1 = NY;
+  if (jp1 > NY) jp1 = 1;
+  if (km1 < 1) km1 = NZ;
+  if (kp1 > NZ) kp1 = 1;
+#define B(I, J, K) ((I) - 1 + ((J) - 1) * NX + ((K) - 1) * NX * NY)
+  nb[0] = B(ip1, jp1, km1);
+  nb[1] = B(ip1, jm1, k0);
+  nb[2] = B(ip1, j0, k0);
+  nb[3] = B(i0, jp1, k0);
+  nb[4] = B(ip1, jp1, k0);
+  nb[5] = B(i0, jm1, kp1);
+  nb[6] = B(ip1, jm1, kp1);
+  nb[7] = B(im1, j0, kp1);
+  nb[8] = B(i0, j0, kp1);
+  nb[9] = B(ip1, j0, kp1);
+  nb[10] = B(im1, jp1, kp1);
+  nb[11] = B(i0, jp1, kp1);
+  nb[12] = B(ip1, jp1, kp1);
+#undef B
+}
+
+template <typename VT>
+__device__ __forceinline__ void load_state(const PState<VT>& p, int s, double st[13]) {
+  st[0] = p.x[s]; st[1] = p.y[s]; st[2] = p.z[s];
+  st[3] = (double)p.u[s]; st[4] = (double)p.v[s]; st[5] = (double)p.w[s];
+  st[6] = p.qa[s]; st[7] = p.qb[s]; st[8] = p.qc[s]; st[9] = p.qd[s];
+  st[10] = (double)p.ox[s]; st[11] = (double)p.oy[s]; st[12] = (double)p.oz[s];
+}
+template <typename VT>
+__device__ __forceinline__ void store_state(const PState<VT>& p, int s, const double st[13]) {
+  p.x[s] = st[0]; p.y[s] = st[1]; p.z[s] = st[2];
+  p.u[s] = (VT)st[3]; p.v[s] = (VT)st[4]; p.w[s] = (VT)st[5];
+  p.qa[s] = st[6]; p.qb[s] = st[7]; p.qc[s] = st[8]; p.qd[s] = st[9];
+  p.ox[s] = (VT)st[10]; p.oy[s] = (VT)st[11]; p.oz[s] = (VT)st[12];
+}
+
+__device__ __forceinline__ Species load_species(const Species* __restrict__ t, int s) {
+  return t[s];
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T* p, T v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// reference layout UAIN(1:n,-1:n+2,-1:n+2) x 6 -> interleaved nodes with x
+// ghosts, [(k+1)(n+4)+(j+1)](n+4)+(i+1) nodes of 6 values
+template <typename FT>
+__global__ void k_field_prep(const double* __restrict__ ref, FT* __restrict__ out, int n) {
+  const int n4 = n + 4;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long total = (long long)n4 * n4 * n4;
+  if (t >= total) return;
+  int ii = (int)(t % n4);               // stored x index = i + 1, i in [-1, n+2]
+  int jj = (int)((t / n4) % n4);
+  int kk = (int)(t / ((long long)n4 * n4));
+  int i = ii - 1;
+  int is = (i < 1) ? i + n : (i > n ? i - n : i);   // x ghosts: index period n
+  const long long comp = (long long)n * n4 * n4;
+  const long long src = (long long)(is - 1) + (long long)jj * n + (long long)kk * n * n4;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) out[t * 6 + c] = (FT)ref[c * comp + src];
+}
+
+template <typename FT, typename CT, int S>
+__global__ __launch_bounds__(BLOCK) void k_interph(const FT* __restrict__ f, int n, double dx,
+                                                   const double* __restrict__ x,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ z, int comp,
+                                                   double* __restrict__ out, int nint,
+                                                   unsigned* err) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nint) return;
+  CT v[6];
+  if (!gather6<FT, CT, S>(f, n, dx, x[p], y[p], z[p], v)) {
+    atomicOr(err, EB_INTERP_RANGE);
+    out[p] = 0.0;
+    return;
+  }
+  out[p] = (double)v[comp];
+}
+
+// fused gather + advance + wrap + cell binning; dst[i] = advance(src[perm[i]])
+template <typename FT, typename VT, typename CT, int S>
+__global__ __launch_bounds__(BLOCK) void k_advance(
+    StepParams P, const FT* __restrict__ f, const Species* __restrict__ spec, PState<VT> src,
+    PState<VT> dst, const int* __restrict__ perm, int np, CellGrid g, int* __restrict__ cell,
+    int* __restrict__ slot, int* __restrict__ cnt, unsigned* err, unsigned long long* nwall,
+    double* __restrict__ flout, int xcd) {
+  const int i = block_index(xcd) * BLOCK + threadIdx.x;
+  if (i >= np) return;
+  const int s = perm ? perm[i] : i;
+  double st[13];
+  load_state(src, s, st);
+  const int sid = src.sp[s];
+  const int pid = src.id[s];
+  CT fl[6];
+  unsigned e = 0;
+  if (!gather6<FT, CT, S>(f, P.n, P.dx, st[0], st[1], st[2], fl)) {
+    e |= EB_INTERP_RANGE;
+    for (int c = 0; c < 6; ++c) fl[c] = CT(0);
+  }
+  if (flout)
+    for (int c = 0; c < 6; ++c) flout[(long long)c * np + pid] = (double)fl[c];
+  const Species sp = load_species(spec, sid);
+  int wall = 0;
+  e |= advance_particle<CT>(P, sp, st, fl, &wall);
+  store_state(dst, i, st);
+  dst.sp[i] = sid;
+  dst.id[i] = pid;
+  if (cell) {
+    bool oob;
+    int c = cell_of(g, st[0], st[1], st[2], &oob);
+    if (oob) e |= EB_OUT_OF_GRID;
+    cell[i] = c;
+    slot[i] = (c >= 0) ? atomicAdd(&cnt[c], 1) : 0;
+  }
+  if (wall) atomicAdd(nwall, 1ull);
+  if (e) atomicOr(err, e);
+}
+
+// cell binning of the current positions (when no advance preceded)
+template <typename VT>
+__global__ __launch_bounds__(BLOCK) void k_cells(PState<VT> p, int np, CellGrid g,
+                                                 int* __restrict__ cell, int* __restrict__ slot,
+                                                 int* __restrict__ cnt, unsigned* err) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  bool oob;
+  int c = cell_of(g, p.x[i], p.y[i], p.z[i], &oob);
+  if (oob) atomicOr(err, EB_OUT_OF_GRID);
+  cell[i] = c;
+  slot[i] = (c >= 0) ? atomicAdd(&cnt[c], 1) : 0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scatter(const int* __restrict__ cell,
+                                                   const int* __restrict__ slot,
+                                                   const int* __restrict__ off,
+                                                   int* __restrict__ tmp, int np) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  int c = cell[i];
+  if (c >= 0) tmp[off[c] + slot[i]] = i;
+}
+
+// stable order inside each cell: rank by reference index
+__global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
+                                                const int* __restrict__ off,
+                                                const int* __restrict__ cnt,
+                                                const int* __restrict__ tmp,
+                                                const int* __restrict__ id,
+                                                int* __restrict__ sorted, int np) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  int c = cell[i];
+  if (c < 0) return;
+  const int o = off[c], m = cnt[c], me = id[i];
+  int r = 0;
+  for (int t = 0; t < m; ++t) r += (id[tmp[o + t]] < me) ? 1 : 0;
+  sorted[o + r] = i;
+}
+
+// candidate pairs emitted for the home particle at sorted position p
+__global__ __launch_bounds__(BLOCK) void k_pair_count(CellGrid g, const int* __restrict__ sorted,
+                                                      const int* __restrict__ cell,
+                                                      const int* __restrict__ off,
+                                                      const int* __restrict__ cnt,
+                                                      int* __restrict__ pc, int nsorted) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nsorted) return;
+  const int c = cell[sorted[p]];
+  int nb[13];
+  half_shell(g, c, nb);
+  int k = cnt[c] - 1 - (p - off[c]);
+#pragma unroll
+  for (int v = 0; v < 13; ++v) k += cnt[nb[v]];
+  pc[p] = k;
+}
+
+// materialised HOME/NEAR (1-based reference indices) for the API
+__global__ __launch_bounds__(BLOCK) void k_pair_emit(CellGrid g, const int* __restrict__ sorted,
+                                                     const int* __restrict__ cell,
+                                                     const int* __restrict__ off,
+                                                     const int* __restrict__ cnt,
+                                                     const long long* __restrict__ poff,
+                                                     const int* __restrict__ id,
+                                                     int* __restrict__ home, int* __restrict__ nr,
+                                                     int nsorted) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nsorted) return;
+  const int hi = sorted[p];
+  const int c = cell[hi];
+  const int h = id[hi] + 1;
+  long long k = poff[p];
+  const int o = off[c], m = cnt[c];
+  for (int t = p + 1; t < o + m; ++t) { home[k] = h; nr[k] = id[sorted[t]] + 1; ++k; }
+  int nb[13];
+  half_shell(g, c, nb);
+  for (int v = 0; v < 13; ++v) {
+    const int o2 = off[nb[v]], m2 = cnt[nb[v]];
+    for (int t = 0; t < m2; ++t) { home[k] = h; nr[k] = id[sorted[o2 + t]] + 1; ++k; }
+  }
+}
+
+template <typename VT>
+__device__ __forceinline__ void min_image(const PState<VT>& p, int i, int j, double L,
+                                          double bi[3], double bj[3]) {
+  bi[0] = p.x[i]; bi[1] = p.y[i]; bi[2] = p.z[i];
+  double vk[3] = {p.x[j] - bi[0], p.y[j] - bi[1], p.z[j] - bi[2]};
+  for (int c = 0; c < 3; ++c) {
+    if (fabs(vk[c]) > L / 2.0) vk[c] = vk[c] - copysign(L, vk[c]);
+    bj[c] = bi[c] + vk[c];
+  }
+}
+
+// bounding-sphere prefilter over each home's candidates (list order kept in
+// the index k); survivors go to the LM work list.  Exact: a contact needs
+// the support point of one ellipsoid inside the other, which is impossible
+// when the centres are farther apart than the sum of the largest semi-axes.
+template <typename VT>
+__global__ __launch_bounds__(BLOCK) void k_prefilter(
+    CellGrid g, PState<VT> p, const Species* __restrict__ spec, const int* __restrict__ sorted,
+    const int* __restrict__ cell, const int* __restrict__ off, const int* __restrict__ cnt,
+    const long long* __restrict__ poff, double L, int use_filter, WorkRec* __restrict__ work,
+    unsigned long long* nwork, unsigned long long cap, unsigned* err, int nsorted) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nsorted) return;
+  const int hi = sorted[q];
+  const int c = cell[hi];
+  long long k = poff[q];
+  const double xi = p.x[hi], yi = p.y[hi], zi = p.z[hi];
+  const double ri = spec[p.sp[hi]].rmax;
+  auto test = [&](int j) {
+    double dx = p.x[j] - xi, dy = p.y[j] - yi, dz = p.z[j] - zi;
+    if (fabs(dx) > L / 2.0) dx -= copysign(L, dx);
+    if (fabs(dy) > L / 2.0) dy -= copysign(L, dy);
+    if (fabs(dz) > L / 2.0) dz -= copysign(L, dz);
+    const double rs = ri + spec[p.sp[j]].rmax;
+    if (!use_filter || dx * dx + dy * dy + dz * dz < rs * rs * (1.0 + 1.0e-9)) {
+      unsigned long long w = atomicAdd(nwork, 1ull);
+      if (w < cap) {
+        work[w].k = k; work[w].i = hi; work[w].j = j;
+      } else {
+        atomicOr(err, EB_ACTIVE_OVERFLOW);
+      }
+    }
+    ++k;
+  };
+  const int o = off[c], m = cnt[c];
+  for (int t = q + 1; t < o + m; ++t) test(sorted[t]);
+  int nb[13];
+  half_shell(g, c, nb);
+  for (int v = 0; v < 13; ++v) {
+    const int o2 = off[nb[v]], m2 = cnt[nb[v]];
+    for (int t = 0; t < m2; ++t) test(sorted[o2 + t]);
+  }
+}
+
+// LM contact solve on the work list + NVREL test
+// (collision_interaction.f90:221-348); approaching contacts are appended.
+template <typename VT>
+__global__ __launch_bounds__(BLOCK) void k_contact(
+    PState<VT> p, const Species* __restrict__ spec, const WorkRec* __restrict__ work,
+    const unsigned long long* nwork, unsigned long long cap, double L,
+    ActiveRec* __restrict__ act, unsigned long long* nact, unsigned long long acap,
+    unsigned long long* nfev_sum, unsigned long long* ncontact, unsigned* err) {
+  const unsigned long long nw = min(*nwork, cap);
+  for (unsigned long long w = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
+       w += (unsigned long long)gridDim.x * blockDim.x) {
+    const WorkRec r = work[w];
+    double bi[3], bj[3];
+    min_image(p, r.i, r.j, L, bi, bj);
+    const double qi[4] = {p.qa[r.i], p.qb[r.i], p.qc[r.i], p.qd[r.i]};
+    const double qj[4] = {p.qa[r.j], p.qb[r.j], p.qc[r.j], p.qd[r.j]};
+    const Species si = spec[p.sp[r.i]], sj = spec[p.sp[r.j]];
+    ContactOut o = contact_detect(bi, qi, si, bj, qj, sj, nullptr);
+    atomicAdd(nfev_sum, (unsigned long long)o.nfev);
+    if (!o.flag) continue;
+    // velocities of the deepest points (collision_interaction.f90:269-324)
+    double r1[3] = {o.e1[0] - bi[0], o.e1[1] - bi[1], o.e1[2] - bi[2]};
+    double r2[3] = {o.e2[0] - bj[0], o.e2[1] - bj[1], o.e2[2] - bj[2]};
+    Rot<double> R1(qi[0], qi[1], qi[2], qi[3]), R2(qj[0], qj[1], qj[2], qj[3]);
+    double w1[3], w2[3];
+    R1.mul((double)p.ox[r.i], (double)p.oy[r.i], (double)p.oz[r.i], w1[0], w1[1], w1[2]);
+    R2.mul((double)p.ox[r.j], (double)p.oy[r.j], (double)p.oz[r.j], w2[0], w2[1], w2[2]);
+    double v1[3] = {(double)p.u[r.i] + (w1[1] * r1[2] - w1[2] * r1[1]),
+                    (double)p.v[r.i] + (w1[2] * r1[0] - w1[0] * r1[2]),
+                    (double)p.w[r.i] + (w1[0] * r1[1] - w1[1] * r1[0])};
+    double v2[3] = {(double)p.u[r.j] + (w2[1] * r2[2] - w2[2] * r2[1]),
+                    (double)p.v[r.j] + (w2[2] * r2[0] - w2[0] * r2[2]),
+                    (double)p.w[r.j] + (w2[0] * r2[1] - w2[1] * r2[0])};
+    double nvrel = (v2[0] - v1[0]) * o.cn[0] + (v2[1] - v1[1]) * o.cn[1] +
+                   (v2[2] - v1[2]) * o.cn[2];
+    if (nvrel < ZERO_R) {
+      atomicAdd(ncontact, 1ull);
+      unsigned long long a = atomicAdd(nact, 1ull);
+      if (a < acap) {
+        ActiveRec& ar = act[a];
+        ar.k = r.k; ar.i = r.i; ar.j = r.j;
+        for (int c = 0; c < 3; ++c) { ar.r1[c] = r1[c]; ar.r2[c] = r2[c]; ar.cn[c] = o.cn[c]; }
+      } else {
+        atomicOr(err, EB_ACTIVE_OVERFLOW);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ long long ld_acq(long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_acq(int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr int GREEDY_BLOCK = 1024;
+
+// One collision per particle in list order (collision_interaction.f90:190-
+// 207, 397-398).  Contact flags depend only on positions and orientations,
+// and a particle's velocity changes only once it is used, so the reference
+// loop equals a greedy matching of the approaching contacts in list order.
+// Rounds: every undecided contact that holds the lowest list index at both
+// of its particles is accepted; contacts touching an accepted particle are
+// dropped.  Then the accepted (disjoint) pairs exchange impulses.
+// mark[] (per particle, LLONG_MAX when idle) and used[] are restored on exit.
+template <typename VT>
+__global__ __launch_bounds__(GREEDY_BLOCK) void k_greedy(
+    PState<VT> p, const Species* __restrict__ spec, ActiveRec* __restrict__ act,
+    const unsigned long long* nact, unsigned long long acap, long long* mark, int* used,
+    unsigned char* __restrict__ status, unsigned long long* ncoll,
+    unsigned long long* rounds_out, unsigned* err) {
+  const long long na = (long long)min(*nact, acap);
+  __shared__ int s_left;
+  __shared__ int s_ncoll;
+  if (threadIdx.x == 0) s_ncoll = 0;
+  for (long long a = threadIdx.x; a < na; a += blockDim.x) status[a] = 0;
+  __syncthreads();
+  int rounds = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_left = 0;
+    for (long long a = threadIdx.x; a < na; a += blockDim.x) {
+      if (status[a] != 0) continue;
+      atomicMin(&mark[act[a].i], act[a].k);
+      atomicMin(&mark[act[a].j], act[a].k);
+    }
+    __syncthreads();
+    for (long long a = threadIdx.x; a < na; a += blockDim.x) {
+      if (status[a] != 0) continue;
+      const long long k = act[a].k;
+      if (ld_acq(&mark[act[a].i]) == k && ld_acq(&mark[act[a].j]) == k) {
+        status[a] = 1;
+        atomicExch(&used[act[a].i], 1);
+        atomicExch(&used[act[a].j], 1);
+      }
+    }
+    __syncthreads();
+    for (long long a = threadIdx.x; a < na; a += blockDim.x) {
+      if (status[a] != 0) {
+        continue;
+      }
+      if (ld_acq(&used[act[a].i]) || ld_acq(&used[act[a].j])) {
+        status[a] = 2;
+      } else {
+        atomicAdd(&s_left, 1);
+      }
+      atomicExch((unsigned long long*)&mark[act[a].i], (unsigned long long)LLONG_MAX);
+      atomicExch((unsigned long long*)&mark[act[a].j], (unsigned long long)LLONG_MAX);
+    }
+    __syncthreads();
+    ++rounds;
+    const int left = s_left;
+    __syncthreads();
+    if (left == 0) break;
+  }
+  // impulses on the accepted, disjoint pairs
+  for (long long a = threadIdx.x; a < na; a += blockDim.x) {
+    const ActiveRec r = act[a];
+    if (status[a] == 1) {
+      atomicAdd(&s_ncoll, 1);
+      double u1[3] = {(double)p.u[r.i], (double)p.v[r.i], (double)p.w[r.i]};
+      double o1[3] = {(double)p.ox[r.i], (double)p.oy[r.i], (double)p.oz[r.i]};
+      double u2[3] = {(double)p.u[r.j], (double)p.v[r.j], (double)p.w[r.j]};
+      double o2[3] = {(double)p.ox[r.j], (double)p.oy[r.j], (double)p.oz[r.j]};
+      const double q1[4] = {p.qa[r.i], p.qb[r.i], p.qc[r.i], p.qd[r.i]};
+      const double q2[4] = {p.qa[r.j], p.qb[r.j], p.qc[r.j], p.qd[r.j]};
+      const Species s1 = spec[p.sp[r.i]];
+      if (!collision_response(s1, q1, q2, u1, o1, u2, o2, r.r1, r.r2, r.cn))
+        atomicOr(err, EB_KE_COLLISION);
+      p.u[r.i] = (VT)u1[0]; p.v[r.i] = (VT)u1[1]; p.w[r.i] = (VT)u1[2];
+      p.ox[r.i] = (VT)o1[0]; p.oy[r.i] = (VT)o1[1]; p.oz[r.i] = (VT)o1[2];
+      p.u[r.j] = (VT)u2[0]; p.v[r.j] = (VT)u2[1]; p.w[r.j] = (VT)u2[2];
+      p.ox[r.j] = (VT)o2[0]; p.oy[r.j] = (VT)o2[1]; p.oz[r.j] = (VT)o2[2];
+    }
+  }
+  __syncthreads();
+  for (long long a = threadIdx.x; a < na; a += blockDim.x) {
+    used[act[a].i] = 0;
+    used[act[a].j] = 0;
+    mark[act[a].i] = LLONG_MAX;
+    mark[act[a].j] = LLONG_MAX;
+  }
+  if (threadIdx.x == 0) {
+    *ncoll += (unsigned long long)s_ncoll;
+    *rounds_out += (unsigned long long)rounds;
+  }
+}
+
+// contact detection on caller-given pairs (API / tests)
+template <typename VT>
+__global__ __launch_bounds__(BLOCK) void k_contact_batch(
+    PState<VT> p, const Species* __restrict__ spec, const int* __restrict__ home,
+    const int* __restrict__ nr, const int* __restrict__ pos_of_id, long long npairs, double L,
+    int np, unsigned char* flag, double* e1, double* e2, double* cn, int* nfev, unsigned* err) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs) return;
+  const int hi = home[t] - 1, ni = nr[t] - 1;
+  if (hi < 0 || hi >= np || ni < 0 || ni >= np) {
+    atomicOr(err, EB_OUT_OF_GRID);
+    return;
+  }
+  const int i = pos_of_id[hi], j = pos_of_id[ni];
+  double bi[3], bj[3];
+  min_image(p, i, j, L, bi, bj);
+  const double qi[4] = {p.qa[i], p.qb[i], p.qc[i], p.qd[i]};
+  const double qj[4] = {p.qa[j], p.qb[j], p.qc[j], p.qd[j]};
+  ContactOut o = contact_detect(bi, qi, spec[p.sp[i]], bj, qj, spec[p.sp[j]], nullptr);
+  flag[t] = (unsigned char)o.flag;
+  for (int c = 0; c < 3; ++c) {
+    e1[t * 3 + c] = o.e1[c];
+    e2[t * 3 + c] = o.e2[c];
+    cn[t * 3 + c] = o.cn[c];
+  }
+  nfev[t] = o.nfev;
+}
+
+template <typename VT>
+__global__ void k_pos_of_id(PState<VT> p, int* pos, int np) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) pos[p.id[i]] = i;
+}
+
+template <typename VT>
+__global__ void k_gather_out(PState<VT> p, double* out, int np) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double st[13];
+  load_state(p, i, st);
+  const int id = p.id[i];
+  for (int c = 0; c < 13; ++c) out[(long long)c * np + id] = st[c];
+}
+
+template <typename VT>
+__global__ void k_scatter_in(PState<VT> p, const double* in, const int* sp, int np) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double st[13];
+  for (int c = 0; c < 13; ++c) st[c] = in[(long long)c * np + i];
+  store_state(p, i, st);
+  p.sp[i] = sp[i];
+  p.id[i] = i;
+}
+
+}  // namespace gp
+
+// ===========================================================================
+// host side
+// ===========================================================================
+using namespace gp;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+};
+
+struct gp_ctx {
+  gp_params prm{};
+  StepParams sp{};
+  hipStream_t st = nullptr;
+  int device = 0;
+  std::string err;
+  bool mixed = false;
+  // field
+  void* field = nullptr;
+  bool have_field = false;
+  // species
+  Species* spec = nullptr;
+  int nspec = 0;
+  double max_emaj = 0.0;
+  // particles
+  int np = 0;
+  void* pbuf[2] = {nullptr, nullptr};
+  int cur = 0;
+  int* perm = nullptr;          // order in which the next advance reads cur
+  bool perm_identity = true;
+  // collision grid
+  CellGrid g{};
+  int* cell = nullptr;
+  int* slot = nullptr;
+  int* cnt = nullptr;
+  int* off = nullptr;
+  int* tmp = nullptr;
+  int* sorted = nullptr;
+  int* pcount = nullptr;
+  long long* poff = nullptr;
+  void* scan_scr = nullptr;
+  bool cells_valid = false;
+  long long* mark = nullptr;
+  int* used = nullptr;
+  int* pos_of_id = nullptr;
+  // work lists
+  WorkRec* work = nullptr;
+  unsigned long long work_cap = 0;
+  ActiveRec* act = nullptr;
+  unsigned char* act_status = nullptr;
+  unsigned long long act_cap = 0;
+  // counters (device)
+  unsigned* d_err = nullptr;
+  unsigned long long* d_cnt = nullptr;   // 0 nwall 1 nwork 2 nact 3 nfev 4 ncontact 5 ncoll 6 rounds
+  unsigned long long* h_cnt = nullptr;   // pinned mirror
+  long long* h_total = nullptr;
+  double* flout = nullptr;
+  gp_stats stats{};
+  long long last_nclose = 0;
+};
+
+#define HIPCHK(x)                                                               \
+  do {                                                                          \
+    hipError_t e_ = (x);                                                        \
+    if (e_ != hipSuccess) {                                                     \
+      ctx->err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " +  \
+                 __FILE__ + ":" + std::to_string(__LINE__);                    \
+      return GP_ERR_HIP;                                                        \
+    }                                                                           \
+  } while (0)
+
+static inline unsigned nblk(long long n, int b = BLOCK) {
+  return (unsigned)((n + b - 1) / b);
+}
+
+template <typename VT>
+static PState<VT> pstate(gp_ctx* ctx, int which) {
+  PState<VT> s;
+  char* b = (char*)ctx->pbuf[which];
+  const size_t n = (size_t)ctx->np;
+  auto dd = [&](int k) { return (double*)(b); };
+  (void)dd;
+  double* d = (double*)b;
+  s.x = d; s.y = d + n; s.z = d + 2 * n;
+  s.qa = d + 3 * n; s.qb = d + 4 * n; s.qc = d + 5 * n; s.qd = d + 6 * n;
+  VT* v = (VT*)(d + 7 * n);
+  s.u = v; s.v = v + n; s.w = v + 2 * n; s.ox = v + 3 * n; s.oy = v + 4 * n; s.oz = v + 5 * n;
+  int* ip = (int*)(v + 6 * n);
+  s.sp = ip; s.id = ip + n;
+  return s;
+}
+
+static size_t pstate_bytes(int np, bool mixed) {
+  const size_t n = (size_t)np;
+  return n * 7 * 8 + n * 6 * (mixed ? 4 : 8) + n * 2 * 4 + 256;
+}
+
+template <typename T>
+static int dalloc(gp_ctx* ctx, T** p, size_t count) {
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    ctx->err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+    return GP_ERR_NOMEM;
+  }
+  return GP_OK;
+}
+
+static int map_err(gp_ctx* ctx, unsigned e) {
+  if (!e) return GP_OK;
+  int code = GP_ERR_STATE;
+  if (e & EB_NAN_DRAG) code = GP_ERR_NAN_DRAG;
+  else if (e & EB_NAN_OMEGA) code = GP_ERR_NAN_OMEGA;
+  else if (e & EB_KE_COLLISION) code = GP_ERR_KE_COLLISION;
+  else if (e & EB_KE_WALL) code = GP_ERR_KE_WALL;
+  else if (e & EB_OUT_OF_GRID) code = GP_ERR_OUT_OF_GRID;
+  else if (e & (EB_OVERFLOW | EB_ACTIVE_OVERFLOW)) code = GP_ERR_NCLOSE_OVERFLOW;
+  else if (e & EB_INTERP_RANGE) code = GP_ERR_INTERP_RANGE;
+  char buf[128];
+  snprintf(buf, sizeof buf, "kernel error bits 0x%x (status %d)", e, code);
+  ctx->err = buf;
+  return code;
+}
+
+// reads and clears the device error word (synchronises the stream)
+static int check_err(gp_ctx* ctx) {
+  unsigned e = 0;
+  HIPCHK(hipMemcpyAsync(&e, ctx->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if (e) {
+    unsigned z = 0;
+    HIPCHK(hipMemcpy(ctx->d_err, &z, sizeof z, hipMemcpyHostToDevice));
+  }
+  return map_err(ctx, e);
+}
+
+static void grid_init(gp_ctx* ctx) {
+  // COLLISION_INITIATION (collision_initiation.f90:50-102), INIT_GRID = 2,
+  // NPROC = 1: DELTA_COLL = ZERO; the y/z range spans L + DY.
+  const double L = ctx->prm.L, dy = L / (double)(ctx->prm.n - 1);
+  const double dpmax = ctx->prm.dpmax > 0.0 ? ctx->prm.dpmax : 2.0 * ctx->max_emaj;
+  double dxbox = 2.0 * dpmax;
+  int nx = (int)(L / dxbox);
+  if (nx < 1) nx = 1;
+  dxbox = L / (double)nx;
+  const double delta = ZERO_R;
+  const double yend = dy * (double)(ctx->prm.n - 1);
+  const double ymin = 0.0 - delta, ymax = yend + dy + delta;
+  CellGrid& g = ctx->g;
+  g.nx = nx;
+  g.xmin = 0.0;
+  g.ymin = ymin;
+  g.zmin = ymin;
+  g.ny = (int)((ymax - ymin) / dxbox);
+  g.nz = (int)((ymax - ymin) / dxbox);
+  if (g.ny < 1) g.ny = 1;
+  if (g.nz < 1) g.nz = 1;
+  g.dx = dxbox;
+  g.dy = (ymax - ymin) / (double)g.ny;
+  g.dz = (ymax - ymin) / (double)g.nz;
+  g.nbox = g.nx * g.ny * g.nz;
+}
+
+static int ensure_grid_buffers(gp_ctx* ctx) {
+  const CellGrid& g = ctx->g;
+  int rc;
+  if ((rc = dalloc(ctx, &ctx->cnt, (size_t)g.nbox))) return rc;
+  if ((rc = dalloc(ctx, &ctx->off, (size_t)g.nbox + 1))) return rc;
+  const int64_t sc = std::max(scan_scratch(g.nbox), scan_scratch((int64_t)ctx->np + 1)) + 16;
+  if (ctx->scan_scr) hipFree(ctx->scan_scr);
+  ctx->scan_scr = nullptr;
+  HIPCHK(hipMalloc(&ctx->scan_scr, (size_t)sc * sizeof(long long)));
+  return GP_OK;
+}
+
+extern "C" {
+
+int gp_create(const gp_params* p, gp_ctx** out) {
+  if (!p || !out) return GP_ERR_ARG;
+  *out = nullptr;
+  if (p->n < 5 || (p->interp_scheme != 3 && p->interp_scheme != 4) ||
+      (p->precision != GP_PREC_FP64 && p->precision != GP_PREC_MIXED) || !(p->L > 0.0) ||
+      !(p->dtime > 0.0) || !(p->visc > 0.0))
+    return GP_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return GP_ERR_HIP;
+  gp_ctx* ctx = new gp_ctx();
+  ctx->prm = *p;
+  ctx->mixed = p->precision == GP_PREC_MIXED;
+  hipGetDevice(&ctx->device);
+  StepParams& s = ctx->sp;
+  s.n = p->n;
+  s.scheme = p->interp_scheme;
+  s.renorm = p->renorm_quat;
+  s.prefilter = p->prefilter;
+  s.L = p->L;
+  s.dx = p->L / (double)(p->n - 1);   // meshing.f90:43
+  s.visc = p->visc;
+  s.rhof = p->rhof;
+  s.dtime = p->dtime;
+  s.ecw = p->ecw;
+  s.muw = p->muw;
+  if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return GP_ERR_HIP;
+  }
+  if (hipMalloc((void**)&ctx->d_err, sizeof(unsigned)) != hipSuccess ||
+      hipMalloc((void**)&ctx->d_cnt, 16 * sizeof(unsigned long long)) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_cnt, 16 * sizeof(unsigned long long)) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_total, 4 * sizeof(long long)) != hipSuccess) {
+    delete ctx;
+    return GP_ERR_NOMEM;
+  }
+  hipMemset(ctx->d_err, 0, sizeof(unsigned));
+  hipMemset(ctx->d_cnt, 0, 16 * sizeof(unsigned long long));
+  hipDeviceSynchronize();
+  *out = ctx;
+  return GP_OK;
+}
+
+int gp_destroy(gp_ctx* ctx) {
+  if (!ctx) return GP_ERR_ARG;
+  hipStreamSynchronize(ctx->st);
+  void* bufs[] = {ctx->field, ctx->spec, ctx->pbuf[0], ctx->pbuf[1], ctx->perm, ctx->cell,
+                  ctx->slot, ctx->cnt, ctx->off, ctx->tmp, ctx->sorted, ctx->pcount,
+                  ctx->poff, ctx->scan_scr, ctx->mark, ctx->used, ctx->pos_of_id, ctx->work,
+                  ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (ctx->h_cnt) hipHostFree(ctx->h_cnt);
+  if (ctx->h_total) hipHostFree(ctx->h_total);
+  hipStreamDestroy(ctx->st);
+  delete ctx;
+  return GP_OK;
+}
+
+const char* gp_last_error_string(const gp_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+int gp_sync(gp_ctx* ctx) {
+  if (!ctx) return GP_ERR_ARG;
+  return check_err(ctx);
+}
+
+int gp_bind_field(gp_ctx* ctx, const double* field) {
+  if (!ctx || !field) return GP_ERR_ARG;
+  const int n = ctx->prm.n, n4 = n + 4;
+  const size_t nref = (size_t)6 * n * n4 * n4;
+  const size_t nodes = (size_t)n4 * n4 * n4;
+  double* dref = nullptr;
+  HIPCHK(hipMalloc((void**)&dref, nref * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(dref, field, nref * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+  if (ctx->field) hipFree(ctx->field);
+  ctx->field = nullptr;
+  const size_t esz = ctx->mixed ? sizeof(float) : sizeof(double);
+  HIPCHK(hipMalloc(&ctx->field, nodes * 6 * esz + 256));
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_field_prep<float>, dim3(nblk((long long)nodes)), dim3(BLOCK), 0, ctx->st,
+                       dref, (float*)ctx->field, n);
+  else
+    hipLaunchKernelGGL(k_field_prep<double>, dim3(nblk((long long)nodes)), dim3(BLOCK), 0,
+                       ctx->st, dref, (double*)ctx->field, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  hipFree(dref);
+  ctx->have_field = true;
+  return GP_OK;
+}
+
+int gp_set_species(gp_ctx* ctx, int nspec, const double* rows) {
+  if (!ctx || nspec < 1 || !rows) return GP_ERR_ARG;
+  std::vector<Species> h((size_t)nspec);
+  double emax = 0.0;
+  for (int s = 0; s < nspec; ++s) {
+    const double* r = rows + (size_t)s * GP_SP_STRIDE;
+    Species& d = h[(size_t)s];
+    d.a = r[GP_SP_A]; d.b = r[GP_SP_B]; d.c = r[GP_SP_C];
+    d.pmass = r[GP_SP_PMASS]; d.deq = r[GP_SP_DEQ];
+    d.ixx = r[GP_SP_IXX]; d.iyy = r[GP_SP_IYY]; d.izz = r[GP_SP_IZZ];
+    d.grav = r[GP_SP_GRAV]; d.ecp = r[GP_SP_ECP]; d.mup = r[GP_SP_MUP];
+    d.rmax = std::max(d.a, std::max(d.b, d.c));
+    if (!(d.a > 0 && d.b > 0 && d.c > 0 && d.pmass > 0 && d.deq > 0)) {
+      ctx->err = "species row " + std::to_string(s) + " has a non-positive size or mass";
+      return GP_ERR_ARG;
+    }
+    emax = std::max(emax, r[GP_SP_EMAJ] > 0 ? r[GP_SP_EMAJ] : d.a);
+  }
+  int rc;
+  if ((rc = dalloc(ctx, &ctx->spec, (size_t)nspec))) return rc;
+  HIPCHK(hipMemcpy(ctx->spec, h.data(), h.size() * sizeof(Species), hipMemcpyHostToDevice));
+  ctx->nspec = nspec;
+  ctx->max_emaj = emax;
+  grid_init(ctx);
+  if (ctx->np > 0) {
+    if ((rc = ensure_grid_buffers(ctx))) return rc;
+  }
+  ctx->cells_valid = false;
+  return GP_OK;
+}
+
+int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* species) {
+  if (!ctx || np < 1 || !state || !species) return GP_ERR_ARG;
+  if (!ctx->spec) {
+    ctx->err = "gp_set_species must precede gp_set_particles";
+    return GP_ERR_STATE;
+  }
+  for (int i = 0; i < np; ++i)
+    if (species[i] < 0 || species[i] >= ctx->nspec) {
+      ctx->err = "species index out of range at particle " + std::to_string(i);
+      return GP_ERR_ARG;
+    }
+  ctx->np = np;
+  const size_t pb = pstate_bytes(np, ctx->mixed);
+  for (int k = 0; k < 2; ++k) {
+    if (ctx->pbuf[k]) hipFree(ctx->pbuf[k]);
+    ctx->pbuf[k] = nullptr;
+    HIPCHK(hipMalloc(&ctx->pbuf[k], pb));
+  }
+  int rc;
+  if ((rc = dalloc(ctx, &ctx->perm, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->cell, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->slot, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->tmp, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->sorted, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->pcount, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->poff, (size_t)np + 1))) return rc;
+  if ((rc = dalloc(ctx, &ctx->mark, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->used, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->pos_of_id, (size_t)np))) return rc;
+  if (ctx->prm.reserved[0]) {
+    if ((rc = dalloc(ctx, &ctx->flout, (size_t)np * 6))) return rc;
+  }
+  if ((rc = ensure_grid_buffers(ctx))) return rc;
+  hipLaunchKernelGGL(k_fill<long long>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->mark,
+                     (long long)LLONG_MAX, (long long)np);
+  hipLaunchKernelGGL(k_fill<int>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->used, 0,
+                     (long long)np);
+  double* din = nullptr;
+  int* dsp = nullptr;
+  HIPCHK(hipMalloc((void**)&din, (size_t)np * 13 * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&dsp, (size_t)np * sizeof(int)));
+  HIPCHK(hipMemcpyAsync(din, state, (size_t)np * 13 * sizeof(double), hipMemcpyHostToDevice,
+                        ctx->st));
+  HIPCHK(hipMemcpyAsync(dsp, species, (size_t)np * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+  ctx->cur = 0;
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_scatter_in<float>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st,
+                       pstate<float>(ctx, 0), din, dsp, np);
+  else
+    hipLaunchKernelGGL(k_scatter_in<double>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st,
+                       pstate<double>(ctx, 0), din, dsp, np);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  hipFree(din);
+  hipFree(dsp);
+  ctx->perm_identity = true;
+  ctx->cells_valid = false;
+  return GP_OK;
+}
+
+int gp_get_particles(gp_ctx* ctx, int np, double* state) {
+  if (!ctx || !state || np != ctx->np) return GP_ERR_ARG;
+  double* dout = nullptr;
+  HIPCHK(hipMalloc((void**)&dout, (size_t)np * 13 * sizeof(double)));
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_gather_out<float>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st,
+                       pstate<float>(ctx, ctx->cur), dout, np);
+  else
+    hipLaunchKernelGGL(k_gather_out<double>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st,
+                       pstate<double>(ctx, ctx->cur), dout, np);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(state, dout, (size_t)np * 13 * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  hipFree(dout);
+  return check_err(ctx);
+}
+
+int gp_get_fluid_at_particles(gp_ctx* ctx, int np, double* fl) {
+  if (!ctx || !fl || np != ctx->np || !ctx->flout) return GP_ERR_ARG;
+  HIPCHK(hipMemcpyAsync(fl, ctx->flout, (size_t)np * 6 * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return GP_OK;
+}
+
+int gp_interph(gp_ctx* ctx, int scheme, int nint, const double* x, const double* y,
+               const double* z, int comp, double* out) {
+  if (!ctx || nint < 0 || comp < 0 || comp > 5 || (scheme != 3 && scheme != 4)) return GP_ERR_ARG;
+  if (!ctx->have_field) {
+    ctx->err = "no field bound";
+    return GP_ERR_STATE;
+  }
+  if (nint == 0) return GP_OK;
+  double* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, (size_t)nint * 4 * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(d, x, (size_t)nint * 8, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(d + nint, y, (size_t)nint * 8, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(d + 2 * (size_t)nint, z, (size_t)nint * 8, hipMemcpyHostToDevice, ctx->st));
+  const int n = ctx->prm.n;
+  const double dx = ctx->sp.dx;
+  double* o = d + 3 * (size_t)nint;
+  if (ctx->mixed) {
+    if (scheme == 3)
+      hipLaunchKernelGGL((k_interph<float, float, 4>), dim3(nblk(nint)), dim3(BLOCK), 0, ctx->st,
+                         (const float*)ctx->field, n, dx, d, d + nint, d + 2 * (size_t)nint, comp,
+                         o, nint, ctx->d_err);
+    else
+      hipLaunchKernelGGL((k_interph<float, float, 5>), dim3(nblk(nint)), dim3(BLOCK), 0, ctx->st,
+                         (const float*)ctx->field, n, dx, d, d + nint, d + 2 * (size_t)nint, comp,
+                         o, nint, ctx->d_err);
+  } else {
+    if (scheme == 3)
+      hipLaunchKernelGGL((k_interph<double, double, 4>), dim3(nblk(nint)), dim3(BLOCK), 0,
+                         ctx->st, (const double*)ctx->field, n, dx, d, d + nint,
+                         d + 2 * (size_t)nint, comp, o, nint, ctx->d_err);
+    else
+      hipLaunchKernelGGL((k_interph<double, double, 5>), dim3(nblk(nint)), dim3(BLOCK), 0,
+                         ctx->st, (const double*)ctx->field, n, dx, d, d + nint,
+                         d + 2 * (size_t)nint, comp, o, nint, ctx->d_err);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, o, (size_t)nint * 8, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  hipFree(d);
+  return check_err(ctx);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// step pieces (asynchronous on ctx->st)
+// ---------------------------------------------------------------------------
+template <typename VT>
+static int launch_advance(gp_ctx* ctx, bool bin) {
+  const int np = ctx->np;
+  const int src = ctx->cur, dst = 1 - ctx->cur;
+  if (bin)
+    hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->g.nbox)), dim3(BLOCK), 0, ctx->st, ctx->cnt, 0,
+                       (long long)ctx->g.nbox);
+  const int* perm = ctx->perm_identity ? nullptr : ctx->perm;
+  const int xcd = ctx->prm.xcd_remap;
+  auto ps = pstate<VT>(ctx, src), pd = pstate<VT>(ctx, dst);
+  int* cell = bin ? ctx->cell : nullptr;
+  unsigned nb = nblk(np);
+  if (xcd) nb = (nb + 7) / 8 * 8;
+  const dim3 gr(nb), bl(BLOCK);
+  if (ctx->mixed) {
+    if (ctx->prm.interp_scheme == 3)
+      hipLaunchKernelGGL((k_advance<float, VT, float, 4>), gr, bl, 0, ctx->st, ctx->sp,
+                         (const float*)ctx->field, ctx->spec, ps, pd, perm, np, ctx->g, cell,
+                         ctx->slot, ctx->cnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd);
+    else
+      hipLaunchKernelGGL((k_advance<float, VT, float, 5>), gr, bl, 0, ctx->st, ctx->sp,
+                         (const float*)ctx->field, ctx->spec, ps, pd, perm, np, ctx->g, cell,
+                         ctx->slot, ctx->cnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd);
+  } else {
+    if (ctx->prm.interp_scheme == 3)
+      hipLaunchKernelGGL((k_advance<double, VT, double, 4>), gr, bl, 0, ctx->st, ctx->sp,
+                         (const double*)ctx->field, ctx->spec, ps, pd, perm, np, ctx->g, cell,
+                         ctx->slot, ctx->cnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd);
+    else
+      hipLaunchKernelGGL((k_advance<double, VT, double, 5>), gr, bl, 0, ctx->st, ctx->sp,
+                         (const double*)ctx->field, ctx->spec, ps, pd, perm, np, ctx->g, cell,
+                         ctx->slot, ctx->cnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd);
+  }
+  HIPCHK(hipGetLastError());
+  ctx->cur = dst;
+  ctx->perm_identity = true;
+  ctx->cells_valid = bin;
+  return GP_OK;
+}
+
+template <typename VT>
+static int launch_bin(gp_ctx* ctx) {
+  const int np = ctx->np;
+  hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->g.nbox)), dim3(BLOCK), 0, ctx->st, ctx->cnt, 0,
+                     (long long)ctx->g.nbox);
+  hipLaunchKernelGGL(k_cells<VT>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st,
+                     pstate<VT>(ctx, ctx->cur), np, ctx->g, ctx->cell, ctx->slot, ctx->cnt,
+                     ctx->d_err);
+  HIPCHK(hipGetLastError());
+  ctx->cells_valid = true;
+  return GP_OK;
+}
+
+// counting sort + pair counts; leaves the total candidate count in
+// ctx->h_total[0] (synchronises the stream once)
+template <typename VT>
+static int launch_sort_count(gp_ctx* ctx) {
+  const int np = ctx->np;
+  const CellGrid& g = ctx->g;
+  int rc;
+  if (!ctx->cells_valid)
+    if ((rc = launch_bin<VT>(ctx))) return rc;
+  scan_exclusive<int, int>(ctx->cnt, ctx->off, g.nbox, (int*)ctx->scan_scr, ctx->st);
+  auto ps = pstate<VT>(ctx, ctx->cur);
+  hipLaunchKernelGGL(k_scatter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->slot,
+                     ctx->off, ctx->tmp, np);
+  hipLaunchKernelGGL(k_rank, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->off,
+                     ctx->cnt, ctx->tmp, ps.id, ctx->sorted, np);
+  hipLaunchKernelGGL(k_pair_count, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, g, ctx->sorted,
+                     ctx->cell, ctx->off, ctx->cnt, ctx->pcount, np);
+  scan_exclusive<int, long long>(ctx->pcount, ctx->poff, np, (long long*)ctx->scan_scr, ctx->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(ctx->h_total, ctx->poff + np, sizeof(long long), hipMemcpyDeviceToHost,
+                        ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->last_nclose = ctx->h_total[0];
+  return GP_OK;
+}
+
+template <typename VT>
+static int launch_collide(gp_ctx* ctx) {
+  const int np = ctx->np;
+  int rc;
+  if ((rc = launch_sort_count<VT>(ctx))) return rc;
+  const long long nclose = ctx->last_nclose;
+  // work list sized for every candidate (prefilter survivors <= nclose)
+  const unsigned long long need = (unsigned long long)std::max<long long>(nclose, 1);
+  if (need > ctx->work_cap) {
+    const unsigned long long cap = need + need / 4 + 1024;
+    if ((rc = dalloc(ctx, &ctx->work, (size_t)cap))) return rc;
+    ctx->work_cap = cap;
+  }
+  if (ctx->act_cap < ctx->work_cap) {
+    const unsigned long long cap = std::max<unsigned long long>(ctx->work_cap / 4, 4096);
+    if (cap > ctx->act_cap) {
+      if ((rc = dalloc(ctx, &ctx->act, (size_t)cap))) return rc;
+      if ((rc = dalloc(ctx, &ctx->act_status, (size_t)cap))) return rc;
+      ctx->act_cap = cap;
+    }
+  }
+  unsigned long long* c = ctx->d_cnt;
+  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(8), 0, ctx->st, c + 1, 0ull, 6ll);
+  auto ps = pstate<VT>(ctx, ctx->cur);
+  hipLaunchKernelGGL(k_prefilter<VT>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->g, ps,
+                     ctx->spec, ctx->sorted, ctx->cell, ctx->off, ctx->cnt, ctx->poff, ctx->prm.L,
+                     ctx->prm.prefilter, ctx->work, c + 1, ctx->work_cap, ctx->d_err, np);
+  const unsigned grid = 2048;
+  hipLaunchKernelGGL(k_contact<VT>, dim3(grid), dim3(BLOCK), 0, ctx->st, ps, ctx->spec, ctx->work,
+                     c + 1, ctx->work_cap, ctx->prm.L, ctx->act, c + 2, ctx->act_cap, c + 3,
+                     c + 4, ctx->d_err);
+  hipLaunchKernelGGL(k_greedy<VT>, dim3(1), dim3(GREEDY_BLOCK), 0, ctx->st, ps, ctx->spec,
+                     ctx->act, c + 2, ctx->act_cap, ctx->mark, ctx->used, ctx->act_status, c + 5,
+                     c + 6, ctx->d_err);
+  HIPCHK(hipGetLastError());
+  // the next advance reads the particles in cell order
+  std::swap(ctx->perm, ctx->sorted);
+  ctx->perm_identity = false;
+  ctx->cells_valid = false;
+  return GP_OK;
+}
+
+static void accumulate_stats(gp_ctx* ctx) {
+  // called after a stream sync
+  unsigned long long* h = ctx->h_cnt;
+  ctx->stats.nwall = (int64_t)h[0];
+  ctx->stats.nprefilter = (int64_t)h[1];
+  ctx->stats.ncontact = (int64_t)h[4];
+  ctx->stats.ncollision = (int64_t)h[5];
+  ctx->stats.lm_nfev = (int64_t)h[3];
+  ctx->stats.greedy_rounds = (int64_t)h[6];
+  ctx->stats.nclose = ctx->last_nclose;
+}
+
+extern "C" {
+
+static int ready(gp_ctx* ctx) {
+  if (!ctx) return GP_ERR_ARG;
+  if (!ctx->have_field || !ctx->spec || ctx->np < 1) {
+    ctx->err = "field, species and particles must be set first";
+    return GP_ERR_STATE;
+  }
+  return GP_OK;
+}
+
+int gp_adv_particle_ellp(gp_ctx* ctx, int ncycle) {
+  (void)ncycle;
+  int rc = ready(ctx);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, ctx->d_cnt, 0ull,
+                     1ll);
+  rc = ctx->mixed ? launch_advance<float>(ctx, true) : launch_advance<double>(ctx, true);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 8 * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, ctx->st));
+  rc = check_err(ctx);
+  ctx->stats.nwall = (int64_t)ctx->h_cnt[0];
+  return rc;
+}
+
+int gp_collision_detection(gp_ctx* ctx, int64_t nclosemax, int* home, int* near_,
+                           int64_t* nclose) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (!nclose || nclosemax < 0 || (nclosemax > 0 && (!home || !near_))) return GP_ERR_ARG;
+  rc = ctx->mixed ? launch_sort_count<float>(ctx) : launch_sort_count<double>(ctx);
+  if (rc) return rc;
+  const long long total = ctx->last_nclose;
+  *nclose = total;
+  if ((rc = check_err(ctx))) return rc;
+  if (total > nclosemax) {
+    ctx->err = "NCLOSE exceeds NCLOSEMAX";
+    return GP_ERR_NCLOSE_OVERFLOW;
+  }
+  if (total == 0) return GP_OK;
+  int *dh = nullptr, *dn = nullptr;
+  HIPCHK(hipMalloc((void**)&dh, (size_t)total * sizeof(int)));
+  HIPCHK(hipMalloc((void**)&dn, (size_t)total * sizeof(int)));
+  const int* id = ctx->mixed ? pstate<float>(ctx, ctx->cur).id : pstate<double>(ctx, ctx->cur).id;
+  hipLaunchKernelGGL(k_pair_emit, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st, ctx->g,
+                     ctx->sorted, ctx->cell, ctx->off, ctx->cnt, ctx->poff, id, dh, dn, ctx->np);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(home, dh, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipMemcpyAsync(near_, dn, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  hipFree(dh);
+  hipFree(dn);
+  return GP_OK;
+}
+
+int gp_contact_batch(gp_ctx* ctx, int64_t npairs, const int* home, const int* near_,
+                     unsigned char* flag, double* ept1, double* ept2, double* cnormal,
+                     int* nfev) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (npairs < 0 || (npairs > 0 && (!home || !near_))) return GP_ERR_ARG;
+  if (npairs == 0) return GP_OK;
+  const size_t n = (size_t)npairs;
+  char* d = nullptr;
+  const size_t bytes = n * (2 * sizeof(int) + 1 + 9 * sizeof(double) + sizeof(int)) + 1024;
+  HIPCHK(hipMalloc((void**)&d, bytes));
+  double* de1 = (double*)d;
+  double* de2 = de1 + 3 * n;
+  double* dcn = de2 + 3 * n;
+  int* dh = (int*)(dcn + 3 * n);
+  int* dn = dh + n;
+  int* dnf = dn + n;
+  unsigned char* df = (unsigned char*)(dnf + n);
+  HIPCHK(hipMemcpyAsync(dh, home, n * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(dn, near_, n * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+  if (ctx->mixed) {
+    auto ps = pstate<float>(ctx, ctx->cur);
+    hipLaunchKernelGGL(k_pos_of_id<float>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st, ps,
+                       ctx->pos_of_id, ctx->np);
+    hipLaunchKernelGGL(k_contact_batch<float>, dim3(nblk(npairs)), dim3(BLOCK), 0, ctx->st, ps,
+                       ctx->spec, dh, dn, ctx->pos_of_id, (long long)npairs, ctx->prm.L, ctx->np,
+                       df, de1, de2, dcn, dnf, ctx->d_err);
+  } else {
+    auto ps = pstate<double>(ctx, ctx->cur);
+    hipLaunchKernelGGL(k_pos_of_id<double>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st, ps,
+                       ctx->pos_of_id, ctx->np);
+    hipLaunchKernelGGL(k_contact_batch<double>, dim3(nblk(npairs)), dim3(BLOCK), 0, ctx->st, ps,
+                       ctx->spec, dh, dn, ctx->pos_of_id, (long long)npairs, ctx->prm.L, ctx->np,
+                       df, de1, de2, dcn, dnf, ctx->d_err);
+  }
+  HIPCHK(hipGetLastError());
+  if (flag) HIPCHK(hipMemcpyAsync(flag, df, n, hipMemcpyDeviceToHost, ctx->st));
+  if (ept1) HIPCHK(hipMemcpyAsync(ept1, de1, 3 * n * 8, hipMemcpyDeviceToHost, ctx->st));
+  if (ept2) HIPCHK(hipMemcpyAsync(ept2, de2, 3 * n * 8, hipMemcpyDeviceToHost, ctx->st));
+  if (cnormal) HIPCHK(hipMemcpyAsync(cnormal, dcn, 3 * n * 8, hipMemcpyDeviceToHost, ctx->st));
+  if (nfev) HIPCHK(hipMemcpyAsync(nfev, dnf, n * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  hipFree(d);
+  return check_err(ctx);
+}
+
+int gp_collision(gp_ctx* ctx, int ncycle, double time) {
+  (void)ncycle;
+  (void)time;
+  int rc = ready(ctx);
+  if (rc) return rc;
+  rc = ctx->mixed ? launch_collide<float>(ctx) : launch_collide<double>(ctx);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 8 * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, ctx->st));
+  rc = check_err(ctx);
+  accumulate_stats(ctx);
+  return rc;
+}
+
+int gp_step(gp_ctx* ctx, int nsteps) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  for (int s = 0; s < nsteps; ++s) {
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, ctx->d_cnt, 0ull,
+                       1ll);
+    rc = ctx->mixed ? launch_advance<float>(ctx, true) : launch_advance<double>(ctx, true);
+    if (rc) return rc;
+    rc = ctx->mixed ? launch_collide<float>(ctx) : launch_collide<double>(ctx);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 8 * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, ctx->st));
+  return GP_OK;
+}
+
+int gp_get_stats(gp_ctx* ctx, gp_stats* out) {
+  if (!ctx || !out) return GP_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  accumulate_stats(ctx);
+  *out = ctx->stats;
+  return GP_OK;
+}
+
+int gp_collision_grid(gp_ctx* ctx, int* nbox_xyz, double* box_min_size) {
+  if (!ctx || !nbox_xyz || !box_min_size) return GP_ERR_ARG;
+  nbox_xyz[0] = ctx->g.nx; nbox_xyz[1] = ctx->g.ny; nbox_xyz[2] = ctx->g.nz;
+  nbox_xyz[3] = ctx->g.nbox;
+  box_min_size[0] = ctx->g.xmin; box_min_size[1] = ctx->g.ymin; box_min_size[2] = ctx->g.zmin;
+  box_min_size[3] = ctx->g.dx; box_min_size[4] = ctx->g.dy; box_min_size[5] = ctx->g.dz;
+  return GP_OK;
+}
+
+}  // extern "C"

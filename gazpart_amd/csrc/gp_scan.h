@@ -1,0 +1,103 @@
+// gp_scan.h -- device-wide exclusive prefix sum for the counting sort and
+// the pair offsets: 64-lane wavefront scans (__shfl_up) inside 256-thread
+// tiles of 2048 elements, tile totals scanned recursively, then added back.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gp {
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// exclusive scan of in[0..n) (of type Tin, widened to T) into out[0..n);
+// tile totals go to sums[blockIdx.x]
+template <typename Tin, typename T>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const Tin* __restrict__ in,
+                                                           T* __restrict__ out,
+                                                           T* __restrict__ sums, int64_t n) {
+  __shared__ T wsum[SCAN_BLOCK / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  T v[SCAN_ITEMS];
+  T tot = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    int64_t idx = base + k;
+    v[k] = (idx < n) ? (T)in[idx] : (T)0;
+    tot += v[k];
+  }
+  T incl = wave_incl_scan(tot);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  T woff = 0;
+  for (int w = 0; w < wid; ++w) woff += wsum[w];
+  T run = woff + incl - tot;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    int64_t idx = base + k;
+    if (idx < n) out[idx] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == SCAN_BLOCK - 1) sums[blockIdx.x] = woff + incl;
+}
+
+template <typename T>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_add(T* __restrict__ out,
+                                                         const T* __restrict__ sums, int64_t n) {
+  const T add = sums[blockIdx.x];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  for (int k = threadIdx.x; k < SCAN_TILE; k += SCAN_BLOCK) {
+    int64_t idx = base + k;
+    if (idx < n) out[idx] += add;
+  }
+}
+
+template <typename T>
+__global__ void k_copy1(T* dst, const T* src, int zero) {
+  dst[0] = zero ? (T)0 : src[0];
+}
+
+// scratch elements needed by scan_exclusive for n inputs
+inline int64_t scan_scratch(int64_t n) {
+  const int64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nt <= 1) return 1;
+  return 2 * nt + 1 + scan_scratch(nt);
+}
+
+// out[i] = sum_{k<i} in[k] for i < n, and out[n] = total (out has n+1 slots)
+template <typename Tin, typename T>
+void scan_exclusive(const Tin* in, T* out, int64_t n, T* scratch, hipStream_t st) {
+  const int64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (n <= 0) {
+    hipLaunchKernelGGL((k_copy1<T>), dim3(1), dim3(1), 0, st, out, (const T*)out, 1);
+    return;
+  }
+  T* sums = scratch;
+  hipLaunchKernelGGL((k_scan_tiles<Tin, T>), dim3((unsigned)nt), dim3(SCAN_BLOCK), 0, st, in,
+                     out, sums, n);
+  if (nt > 1) {
+    T* sums_ex = scratch + nt;   // nt + 1 slots, its total lands at sums_ex[nt]
+    scan_exclusive<T, T>(sums, sums_ex, nt, scratch + 2 * nt + 1, st);
+    hipLaunchKernelGGL((k_scan_add<T>), dim3((unsigned)nt), dim3(SCAN_BLOCK), 0, st, out,
+                       sums_ex, n);
+    hipLaunchKernelGGL((k_copy1<T>), dim3(1), dim3(1), 0, st, out + n, (const T*)(sums_ex + nt), 0);
+  } else {
+    hipLaunchKernelGGL((k_copy1<T>), dim3(1), dim3(1), 0, st, out + n, (const T*)sums, 0);
+  }
+}
+
+}  // namespace gp

@@ -1,0 +1,142 @@
+/*
+ * gazpart_gpu.h -- C-ABI of the MI355X (gfx950) Lagrangian ellipsoid step.
+ *
+ * The reference (GAZPART, Fortran/MPI) has no plugin API: its hot path is a
+ * set of external subroutines with implicit interfaces plus module-global
+ * state.  Each entry point below replaces one of them; a Fortran caller
+ * binds them with an iso_c_binding interface block (see
+ * gazpart_amd/fortran/gazpart_gpu_mod.f90).  Citations are to the
+ * reference's source/ directory.
+ *
+ * Conventions
+ *  - Every call returns an int status (GP_OK = 0); nothing calls exit/stop.
+ *  - Host arrays are owned by the caller and never retained after return.
+ *  - Field and particles stay resident in device memory between calls.
+ *  - One context per GPU; a context is not re-entrant.
+ *  - Particle indices at the boundary are the reference's: 1-based, in the
+ *    order the particles were given to gp_set_particles.
+ *  - state layout: double[13][np], components x y z u v w qa qb qc qd
+ *    ox oy oz (omega in the body frame, as PART%OMEGAX..).
+ *  - field layout: double[6][n+4][n+4][n] = UAIN(1:n,-1:n+2,-1:n+2) for
+ *    u v w wx wy wz (allocate_arrays.f90:108-118).
+ */
+#ifndef GAZPART_GPU_H
+#define GAZPART_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gp_ctx gp_ctx;
+
+/* status codes; 1-6 mirror the reference's `stop` statements */
+enum {
+  GP_OK = 0,
+  GP_ERR_NAN_DRAG = 1,        /* fluid_force_ellipsoid.f90:150-158 */
+  GP_ERR_NAN_OMEGA = 2,       /* fluid_torque_ellipsoid.f90:178-188 */
+  GP_ERR_KE_COLLISION = 3,    /* collision_response.f90:408-436 */
+  GP_ERR_KE_WALL = 4,         /* wall_ellipsoid_rebound.f90:372-384 */
+  GP_ERR_OUT_OF_GRID = 5,     /* collision_detection_connectivity.f90:81-89 */
+  GP_ERR_NCLOSE_OVERFLOW = 6, /* NCLOSE > NCLOSEMAX (collision.f90:35-37) */
+  GP_ERR_INTERP_RANGE = 7,    /* stencil outside UAIN (interp_lag3.f90:124) */
+  GP_ERR_ARG = 10,
+  GP_ERR_STATE = 11,
+  GP_ERR_HIP = 12,
+  GP_ERR_NOMEM = 13
+};
+
+enum { GP_PREC_FP64 = 0, GP_PREC_MIXED = 1 };
+
+/* species (particle class x size) row layout, GP_SP_STRIDE doubles */
+enum {
+  GP_SP_A = 0, GP_SP_B, GP_SP_C,   /* semi-axes (body x, y, z) */
+  GP_SP_PMASS, GP_SP_DEQ,          /* PMASS, dequiv (fluid_force_ellipsoid.f90:59,70) */
+  GP_SP_IXX, GP_SP_IYY, GP_SP_IZZ, /* inertia / mass */
+  GP_SP_GRAV, GP_SP_ECP, GP_SP_MUP,
+  GP_SP_RHOP, GP_SP_EMAJ, GP_SP_APR,
+  GP_SP_STRIDE
+};
+
+typedef struct {
+  int n;               /* grid points per axis, NX = NY = NZ */
+  int interp_scheme;   /* 3 = INTERP_LAG3, 4 = INTERP_LAG4 */
+  int precision;       /* GP_PREC_FP64 or GP_PREC_MIXED */
+  int renorm_quat;     /* 0 = as the reference (no renormalisation) */
+  double L;            /* LXMAX = LYMAX = LZMAX */
+  double visc, rhof;   /* VISC, RHOF */
+  double dtime;        /* DTIME */
+  double ecw, muw;     /* wall restitution / friction (ECW, MUW) */
+  double dpmax;        /* collision_initiation.f90:50 DPMAX; <= 0: 2*max(EMAJ) */
+  int prefilter;       /* 1: skip the LM for pairs farther apart than the sum
+                          of their largest semi-axes (flag is then false in
+                          the reference too); 0: LM on every pair */
+  int xcd_remap;       /* 1: XCD-aware block order for the gather */
+  int reserved[6];
+} gp_params;
+
+typedef struct {
+  int64_t nclose;       /* candidate pairs of the last collision pass */
+  int64_t nprefilter;   /* pairs that reached the LM */
+  int64_t ncontact;     /* pairs with flag = .true. and NVREL < ZERO */
+  int64_t ncollision;   /* pairs resolved (one collision per particle) */
+  int64_t lm_nfev;      /* function evaluations summed over LM solves */
+  int64_t nwall;        /* wall bounces in the last advance */
+  int64_t greedy_rounds;
+  int64_t reserved[9];
+} gp_stats;
+
+/* lifetime ----------------------------------------------------------- */
+int gp_create(const gp_params *p, gp_ctx **out);
+int gp_destroy(gp_ctx *ctx);
+const char *gp_last_error_string(const gp_ctx *ctx);
+int gp_sync(gp_ctx *ctx);
+
+/* data in / out ------------------------------------------------------- */
+/* UFLU..VORTICITY_Z after FLUIDCOMM (main.f90:255-259); copied once.   */
+int gp_bind_field(gp_ctx *ctx, const double *field);
+/* per-species constants (PARAM_PHYS EMAJ_PART, APR_PART, RHOP, IPXX..)  */
+int gp_set_species(gp_ctx *ctx, int nspec, const double *rows);
+/* PART(1:np) -> device; species[i] in [0, nspec)                       */
+int gp_set_particles(gp_ctx *ctx, int np, const double *state, const int *species);
+/* device -> PART(1:np) in the order given to gp_set_particles           */
+int gp_get_particles(gp_ctx *ctx, int np, double *state);
+/* fluid velocity and vorticity at x^n of the last advance (UFAP..VORTZAP),
+ * double[6][np]; requires gp_params.reserved[0] = 1 at create time      */
+int gp_get_fluid_at_particles(gp_ctx *ctx, int np, double *fl);
+
+/* hot-path entry points ------------------------------------------------ */
+/* INTERPH (interpH.f90:26-145) of field component comp (0..5) at nint
+ * host positions; scheme 3 or 4.                                       */
+int gp_interph(gp_ctx *ctx, int scheme, int nint, const double *x, const double *y,
+               const double *z, int comp, double *out);
+/* ADV_PARTICLE_ELLP (adv_particle_new.f90:7-704, called at main.f90:466)
+ * on the device-resident particles, including BOUNDARY_PARTICLE.       */
+int gp_adv_particle_ellp(gp_ctx *ctx, int ncycle);
+/* COLLISION_DETECTION_CONNECTIVITY (collision_detection_connectivity.f90:
+ * 7-232) on the device-resident positions: 1-based HOME/NEAR in the
+ * reference order.  Returns GP_ERR_NCLOSE_OVERFLOW (with *nclose set) if
+ * the list does not fit.                                                */
+int gp_collision_detection(gp_ctx *ctx, int64_t nclosemax, int *home, int *near_,
+                           int64_t *nclose);
+/* ellipsoid_contact_detection (ellipsoid_contact_detection.f90:1-155) on
+ * npairs (1-based) pairs of device-resident particles, with the minimum
+ * image of collision_interaction.f90:221-237.  Any output may be NULL.  */
+int gp_contact_batch(gp_ctx *ctx, int64_t npairs, const int *home, const int *near_,
+                     unsigned char *flag, double *ept1, double *ept2, double *cnormal,
+                     int *nfev);
+/* COLLISION (collision.f90:7-201, called at main.f90:483) at NPROC = 1. */
+int gp_collision(gp_ctx *ctx, int ncycle, double time);
+/* nsteps x (ADV_PARTICLE_ELLP + COLLISION), asynchronous on the context's
+ * stream; errors surface at the next gp_sync / gp_get_particles.        */
+int gp_step(gp_ctx *ctx, int nsteps);
+int gp_get_stats(gp_ctx *ctx, gp_stats *out);
+
+/* introspection for benchmarks: collision grid of the context           */
+int gp_collision_grid(gp_ctx *ctx, int *nbox_xyz, double *box_min_size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

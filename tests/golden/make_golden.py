@@ -1,0 +1,212 @@
+"""Write the golden fixtures of tests/golden/ from the reference's own Fortran.
+
+Runs only where the reference sources exist (they are compiled unmodified
+by oracle/ref/Makefile into oracle/_ref/libgazpart_ref.so, together with a
+C-callable driver).  Each fixture holds inputs and the reference's outputs:
+
+  golden_leaf.npz        interp_lag3/4 (6 fields), force, torque, quaternion
+                         map, wall test + rebound, contact detection,
+                         collision response
+  golden_cells.npz       COLLISION_INITIATION grid, connectivity, HOME/NEAR
+  golden_trajectory.npz  full ADV_PARTICLE_ELLP + COLLISION steps of a dense
+                         small ensemble (collisions and wall bounces happen)
+
+Usage:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+from gazpart_amd import workload as W  # noqa: E402
+
+L = W.TWOPI
+VISC, RHOF = 0.01, 1.0
+
+
+def small_field(n=12, k0=2.0, seed=7):
+    fld, umax = W.random_fourier_field(n, k0, seed)
+    return fld, umax
+
+
+def leaf():
+    n = 12
+    fld, umax = small_field(n)
+    dx = L / (n - 1)
+    dtime = 0.1 * dx / umax
+    # three classes with different aspect ratios, two sizes each
+    apr = np.array([2.0, 0.5, 5.0])
+    emaj = np.array([[0.2, 0.3], [0.15, 0.25], [0.4, 0.3]]) * dx
+    rhop = np.array([[50.0, 80.0], [30.0, 60.0], [90.0, 40.0]])
+    R = O.Ref()
+    R.setup(n, L, VISC, RHOF, dtime, 1.0, 0.0, 3, emaj, apr, rhop)
+    R.set_field(fld)
+    g = W.rng(99)
+    out = dict(n=n, L=L, visc=VISC, rhof=RHOF, dtime=dtime, field=fld, apr=apr, emaj=emaj,
+               rhop=rhop)
+    # interpolation: random points plus nodes, domain edges and x = L
+    m = 400
+    pts = g.uniform(0.0, L, (3, m))
+    pts[:, :8] = [[0.0, L, dx, L - 1e-12, 0.5 * dx, 3 * dx, L, 0.0],
+                  [0.0, 0.0, L, 2 * dx, L - 1e-9, 0.25, L, L],
+                  [L, 0.0, 0.0, dx, 1e-14, L, L, L]]
+    out["interp_pts"] = pts
+    out["interp3"] = np.array([[R.interph(3, c, *pts) for c in range(6)]])[0]
+    out["interp4"] = np.array([[R.interph(4, c, *pts) for c in range(6)]])[0]
+    # force / torque / quaternion
+    k = 300
+    cls = g.integers(1, 4, k)
+    idp = g.integers(1, 3, k)
+    q = g.standard_normal((k, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    up = g.standard_normal((k, 3)) * 0.5
+    fl6 = g.standard_normal((k, 6))
+    om = g.standard_normal((k, 3)) * 3.0
+    fd, flf, tp, tr, qn = (np.zeros((k, 3)), np.zeros((k, 3)), np.zeros((k, 3)),
+                           np.zeros((k, 3)), np.zeros((k, 4)))
+    for t in range(k):
+        fd[t], flf[t] = R.force(cls[t], idp[t], *up[t], *fl6[t, :3], q[t])
+        st13 = np.zeros(13)
+        st13[3:6] = up[t]
+        tp[t], tr[t] = R.torque(cls[t], idp[t], st13, fl6[t], *om[t], q[t])
+        qn[t] = R.quat(q[t], *om[t], dtime * (1 + t % 7))
+    out.update(ft_cls=cls, ft_idp=idp, ft_q=q, ft_up=up, ft_fl6=fl6, ft_om=om, f_drag=fd,
+               f_lift=flf, t_pitch=tp, t_rot=tr, q_dt=dtime * (1 + np.arange(k) % 7), q_new=qn)
+    # wall test + rebound
+    k = 300
+    wcls = g.integers(1, 4, k)
+    widp = g.integers(1, 3, k)
+    wq = g.standard_normal((k, 4))
+    wq /= np.linalg.norm(wq, axis=1)[:, None]
+    a = emaj[wcls - 1, widp - 1]
+    side = g.integers(0, 2, k)
+    off = g.uniform(-0.3, 1.4, k) * a * np.maximum(1.0, 1.0 / apr[wcls - 1])
+    wx = np.where(side == 0, off, L - off)
+    wyz = g.uniform(0, L, (k, 2))
+    wn, wa, wd, wh = np.zeros((k, 3)), np.zeros((k, 3)), np.zeros(k), np.zeros(k, dtype=np.int32)
+    wu, wo = g.standard_normal((k, 3)), g.standard_normal((k, 3)) * 4.0
+    ru, ro = np.zeros((k, 3)), np.zeros((k, 3))
+    for t in range(k):
+        wn[t], wa[t], wd[t], wh[t] = R.wall(wcls[t], widp[t], wx[t], *wyz[t], wq[t])
+        if wh[t]:
+            ru[t], ro[t] = R.rebound(wcls[t], widp[t], wu[t], wo[t], wq[t], wa[t], wn[t])
+    out.update(w_cls=wcls, w_idp=widp, w_q=wq, w_x=wx, w_yz=wyz, w_normal=wn, w_arm=wa,
+               w_dx=wd, w_hit=wh, w_u=wu, w_o=wo, w_u_after=ru, w_o_after=ro)
+    # contact detection: pairs of one class at distances around contact
+    k = 600
+    ccls = g.integers(1, 4, k)
+    ci1, ci2 = g.integers(1, 3, k), g.integers(1, 3, k)
+    q1 = g.standard_normal((k, 4))
+    q1 /= np.linalg.norm(q1, axis=1)[:, None]
+    q2 = g.standard_normal((k, 4))
+    q2 /= np.linalg.norm(q2, axis=1)[:, None]
+    b1 = g.uniform(1.0, 5.0, (k, 3))
+    d = g.standard_normal((k, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rmax = np.maximum(emaj, emaj / apr[:, None])
+    reach = rmax[ccls - 1, ci1 - 1] + rmax[ccls - 1, ci2 - 1]
+    b2 = b1 + d * (g.uniform(0.1, 1.1, k) * reach)[:, None]
+    cf, ce1, ce2, ccn = np.zeros(k, dtype=np.int32), np.zeros((k, 3)), np.zeros((k, 3)), \
+        np.zeros((k, 3))
+    for t in range(k):
+        cf[t], ce1[t], ce2[t], ccn[t] = R.contact(ccls[t], ci1[t], ci2[t], b1[t], q1[t], b2[t],
+                                                  q2[t])
+    out.update(c_cls=ccls, c_idp1=ci1, c_idp2=ci2, c_b1=b1, c_q1=q1, c_b2=b2, c_q2=q2,
+               c_flag=cf, c_e1=ce1, c_e2=ce2, c_cn=ccn)
+    # collision response (ECP = 1, MUP = 0)
+    k = 200
+    rcls, ridp = g.integers(1, 4, k), g.integers(1, 3, k)
+    s1, s2 = np.zeros((k, 13)), np.zeros((k, 13))
+    for s in (s1, s2):
+        s[:, 3:6] = g.standard_normal((k, 3))
+        qq = g.standard_normal((k, 4))
+        s[:, 6:10] = qq / np.linalg.norm(qq, axis=1)[:, None]
+        s[:, 10:13] = g.standard_normal((k, 3)) * 5.0
+    ra = emaj[rcls - 1, ridp - 1][:, None]
+    r1, r2 = g.standard_normal((k, 3)) * ra, g.standard_normal((k, 3)) * ra
+    nr = g.standard_normal((k, 3))
+    nr /= np.linalg.norm(nr, axis=1)[:, None]
+    o1, o2 = np.zeros((k, 13)), np.zeros((k, 13))
+    for t in range(k):
+        o1[t], o2[t] = R.response(rcls[t], ridp[t], s1[t], s2[t], r1[t], r2[t], nr[t])
+    out.update(r_cls=rcls, r_idp=ridp, r_s1=s1, r_s2=s2, r_r1=r1, r_r2=r2, r_n=nr, r_s1_after=o1,
+               r_s2_after=o2)
+    return out
+
+
+def dense_setup(n=12, np_=400, seed=11, apr=2.0):
+    fld, umax = small_field(n, seed=seed)
+    dx = L / (n - 1)
+    dtime = 0.1 * dx / umax
+    # semi-major axis from a 5 % volume fraction
+    phi = 0.05
+    req = (phi * L ** 3 * 3.0 / (4.0 * math.pi * np_)) ** (1.0 / 3.0)
+    emaj = req * apr ** (2.0 / 3.0)
+    deq = 2.0 * emaj / apr ** (2.0 / 3.0)
+    rhop = W.rhop_for_stokes(1.0, deq, 0.5, RHOF, VISC)
+    st = W.random_particles(np_, L, emaj, seed + 1)
+    # give particles an initial velocity so that contacts approach
+    g = W.rng(seed + 2)
+    st[3:6] = g.standard_normal((3, np_)) * 0.5
+    st[10:13] = g.standard_normal((3, np_)) * 2.0
+    return fld, dtime, emaj, apr, rhop, st
+
+
+def cells():
+    n = 12
+    fld, dtime, emaj, apr, rhop, st = dense_setup(n=n, np_=2000, seed=21)
+    R = O.Ref()
+    R.setup(n, L, VISC, RHOF, dtime, 1.0, 0.0, 3, [[emaj]], [apr], [[rhop]])
+    R.set_field(fld)
+    R.set_particles(1, st, np.ones(st.shape[1], dtype=np.int32))
+    inum, rnum = R.collision_init()
+    con, ncon = R.connectivity(int(inum[3]))
+    home, near = R.detect(1, 200 * st.shape[1])
+    return dict(n=n, L=L, emaj=emaj, apr=apr, state=st, grid_i=inum, grid_r=rnum, con=con,
+                ncon=ncon, home=home, near=near)
+
+
+def trajectory(nsteps=30, scheme=3):
+    n = 12
+    fld, dtime, emaj, apr, rhop, st = dense_setup(n=n, np_=400, seed=31)
+    R = O.Ref()
+    R.setup(n, L, VISC, RHOF, dtime, 1.0, 0.0, scheme, [[emaj]], [apr], [[rhop]])
+    R.set_field(fld)
+    np_ = st.shape[1]
+    R.set_particles(1, st, np.ones(np_, dtype=np.int32))
+    R.collision_init()
+    states, colls, ncloses = [], [], []
+    for s in range(1, nsteps + 1):
+        R.adv(s)
+        h, nr = R.collision(1, s, 0.0, 200 * np_)
+        sn, fl, col = R.get_particles(1, np_)
+        states.append(sn)
+        colls.append((col < 0).astype(np.int8))
+        ncloses.append(len(h))
+    keep = [s for s in (1, 2, 3, 5, 10, 20, 30) if s <= nsteps]
+    return dict(n=n, L=L, visc=VISC, rhof=RHOF, dtime=dtime, emaj=emaj, apr=apr, rhop=rhop,
+                scheme=scheme, field=fld, state0=st, keep_steps=np.array(keep),
+                states=np.array([states[s - 1] for s in keep]),
+                collided=np.array(colls), nclose=np.array(ncloses))
+
+
+def main():
+    np.savez_compressed(os.path.join(HERE, "golden_leaf.npz"), **leaf())
+    np.savez_compressed(os.path.join(HERE, "golden_cells.npz"), **cells())
+    np.savez_compressed(os.path.join(HERE, "golden_trajectory.npz"), **trajectory(30, 3))
+    np.savez_compressed(os.path.join(HERE, "golden_trajectory_lag4.npz"), **trajectory(10, 4))
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

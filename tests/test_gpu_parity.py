@@ -1,0 +1,273 @@
+"""HIP path (through the C-ABI) against the reference's own outputs
+(tests/golden/) and the CPU oracle, on the same seeded inputs.
+
+Tolerances
+  FP64 form   : interpolation 1e-13 of the field scale; one step 1e-12 of
+                each quantity's scale; trajectories 1e-9 of scale after
+                excluding particles whose collision history differs from the
+                reference (they are listed in the assertion message).
+  mixed form  : FP32 field, gather and drag; relative error per quantity
+                (max |gpu - oracle| / max |oracle|) <= 1e-5.
+  integer     : cell grid, candidate pairs (HOME/NEAR) bit-exact.
+"""
+import numpy as np
+import pytest
+
+import gazpart_amd as G
+import oracle as O
+from gazpart_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+GOLD = "tests/golden/"
+QUANT = {"x": slice(0, 3), "u": slice(3, 6), "q": slice(6, 10), "omega": slice(10, 13)}
+
+
+def relerr(a, b, sl):
+    return float(np.abs(a[sl] - b[sl]).max() / max(np.abs(b[sl]).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return W.c1(np_=4000)
+
+
+@pytest.fixture(scope="module")
+def leaf():
+    return np.load(GOLD + "golden_leaf.npz")
+
+
+def golden_workload(d):
+    np_ = d["state0"].shape[1]
+    return W.Workload(name="golden", n=int(d["n"]), L=float(d["L"]), k0=2.0, seed=0, np_=np_,
+                      scheme=int(d["scheme"]), steps=0, visc=float(d["visc"]),
+                      rhof=float(d["rhof"]), dtime=float(d["dtime"]), field=d["field"], umax=0.0,
+                      species=W.species_spheroid(float(d["emaj"]), float(d["apr"]),
+                                                 float(d["rhop"]))[None, :],
+                      species_id=np.zeros(np_, dtype=np.int32), state=d["state0"],
+                      dpmax=2.0 * float(d["emaj"]))
+
+
+@pytest.mark.parametrize("prec,tol", [(G.PREC_FP64, 1e-13), (G.PREC_MIXED, 2e-6)])
+def test_interph_against_reference_fixture(leaf, prec, tol):
+    n, L = int(leaf["n"]), float(leaf["L"])
+    ctx = G.Context(n, L, 0.01, 1.0, 1e-3, precision=prec)
+    ctx.bind_field(leaf["field"])
+    pts = leaf["interp_pts"]
+    for scheme, key in ((3, "interp3"), (4, "interp4")):
+        for c in range(6):
+            out = ctx.interph(scheme, c, *pts)
+            ref = leaf[key][c]
+            assert np.abs(out - ref).max() <= tol * np.abs(ref).max(), (scheme, c)
+
+
+@pytest.mark.parametrize("prec,tol", [(G.PREC_FP64, 1e-13), (G.PREC_MIXED, 2e-6)])
+def test_interph_against_oracle(c1, prec, tol):
+    ctx = G.Context.from_workload(c1, precision=prec)
+    g = W.rng(3)
+    x, y, z = g.uniform(0, c1.L, (3, 20000))
+    for scheme in (3, 4):
+        for c in range(6):
+            out = ctx.interph(scheme, c, x, y, z)
+            ref, err = O.interph(scheme, c1.n, c1.dx, c1.field[c], x, y, z)
+            assert np.abs(out - ref).max() <= tol * np.abs(ref).max(), (scheme, c)
+
+
+def test_cell_list_bitexact_against_reference_fixture():
+    d = np.load(GOLD + "golden_cells.npz")
+    st = d["state"]
+    sp = W.species_spheroid(float(d["emaj"]), float(d["apr"]), 10.0)
+    ctx = G.Context(int(d["n"]), float(d["L"]), 0.01, 1.0, 1e-3, dpmax=2.0 * float(d["emaj"]))
+    ctx.bind_field(np.zeros((6, int(d["n"]) + 4, int(d["n"]) + 4, int(d["n"]))))
+    ctx.set_species(sp)
+    ctx.set_particles(st, np.zeros(st.shape[1], dtype=np.int32))
+    nb, bm = ctx.collision_grid()
+    assert list(nb) == list(d["grid_i"])
+    assert list(bm) == list(d["grid_r"])
+    home, near = ctx.collision_detection_connectivity()
+    np.testing.assert_array_equal(home, d["home"])
+    np.testing.assert_array_equal(near, d["near"])
+
+
+def test_cell_list_bitexact_edge_positions(c1):
+    """Particles on box faces, at 0, just below L and in the L+DY strip of
+    y/z (x = L exactly lands outside the grid in the reference, which then
+    aliases into the next row; both sides report it as an error instead)."""
+    p = O.params(c1)
+    g = O.collision_grid(p, c1.dpmax)
+    st = c1.state.copy()
+    m = 400
+    gen = W.rng(17)
+    for k in range(m):
+        i = k
+        ax = k % 3
+        face = gen.integers(0, g.nxbox)
+        d = (g.dxbox, g.dybox, g.dzbox)[ax]
+        st[ax, i] = [0.0, 1e-300, face * d, face * d * (1 + 2e-16), c1.L - 1e-15][k % 5]
+    ctx = G.Context.from_workload(c1)
+    ctx.set_particles(st, c1.species_id)
+    home, near = ctx.collision_detection_connectivity()
+    con, ncon = O.connectivity(g)
+    ho, no, nc, err = O.detect(g, con, ncon, st[0], st[1], st[2], 10 ** 7)
+    assert err == 0
+    np.testing.assert_array_equal(home, ho)
+    np.testing.assert_array_equal(near, no)
+
+
+def test_contact_detection_against_reference_fixture(leaf):
+    """LM contact solve on the fixture's 600 pairs (three aspect ratios,
+    oblate included): flags identical, contact points and normals within the
+    solver's own tolerance of the reference."""
+    k = len(leaf["c_cls"])
+    apr, emaj, rhop = leaf["apr"], leaf["emaj"], leaf["rhop"]
+    rows = np.array([W.species_spheroid(emaj[c, i], apr[c], rhop[c, i]) for c in range(3)
+                     for i in range(2)])
+    st = np.zeros((13, 2 * k))
+    st[0:3, 0::2] = leaf["c_b1"].T
+    st[0:3, 1::2] = leaf["c_b2"].T
+    st[6:10, 0::2] = leaf["c_q1"].T
+    st[6:10, 1::2] = leaf["c_q2"].T
+    sid = np.zeros(2 * k, dtype=np.int32)
+    sid[0::2] = (leaf["c_cls"] - 1) * 2 + leaf["c_idp1"] - 1
+    sid[1::2] = (leaf["c_cls"] - 1) * 2 + leaf["c_idp2"] - 1
+    n = int(leaf["n"])
+    ctx = G.Context(n, float(leaf["L"]), 0.01, 1.0, 1e-3)
+    ctx.bind_field(leaf["field"])
+    ctx.set_species(rows)
+    ctx.set_particles(st, sid)
+    home = np.arange(1, 2 * k, 2, dtype=np.int32)
+    flag, e1, e2, cn, nfev = ctx.ellipsoid_contact_detection(home, home + 1)
+    np.testing.assert_array_equal(flag, leaf["c_flag"])
+    f = flag.astype(bool)
+    scale = emaj.max()
+    assert np.abs(e1[f] - leaf["c_e1"][f]).max() < 1e-5 * scale
+    assert np.abs(e2[f] - leaf["c_e2"][f]).max() < 1e-5 * scale
+    assert np.abs(cn[f] - leaf["c_cn"][f]).max() < 1e-4
+    assert (nfev > 0).all() and (nfev <= 300).all()
+
+
+@pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
+def test_one_advance_against_oracle(c1, prec):
+    ctx = G.Context.from_workload(c1, precision=prec, keep_fluid=True)
+    ctx.adv_particle_ellp(1)
+    st, fl = ctx.get_particles(), ctx.get_fluid_at_particles()
+    so, flo, err = O.adv(O.params(c1), c1.field, c1.species, c1.state, c1.species_id,
+                         want_fluid=True)
+    assert err == 0
+    tol = 1e-12 if prec == G.PREC_FP64 else 1e-5
+    assert np.abs(fl - flo).max() <= (1e-13 if prec == G.PREC_FP64 else 2e-6) * np.abs(flo).max()
+    for name, sl in QUANT.items():
+        assert relerr(st, so, sl) <= tol, name
+
+
+def _history_divergence(coll_gpu, coll_ref):
+    """particles whose set of collision steps differs"""
+    return np.nonzero((coll_gpu != coll_ref).any(axis=0))[0]
+
+
+@pytest.mark.parametrize("name", ["golden_trajectory.npz", "golden_trajectory_lag4.npz"])
+def test_trajectory_against_reference_fixture(name):
+    """Full ADV_PARTICLE_ELLP + COLLISION steps of a dense ensemble (5 %
+    volume fraction, wall bounces and ~20 collisions per step) against the
+    reference Fortran's own trajectory."""
+    d = np.load(GOLD + name)
+    wl = golden_workload(d)
+    ctx = G.Context.from_workload(wl)
+    np_ = wl.np_
+    keep = list(d["keep_steps"])
+    coll = np.zeros_like(d["collided"])
+    prev = wl.state.copy()
+    for s in range(1, int(max(keep)) + 1):
+        ctx.adv_particle_ellp(s)
+        mid = ctx.get_particles()
+        ctx.collision(s)
+        cur = ctx.get_particles()
+        # a particle took part in a collision iff its velocity changed there
+        coll[s - 1] = np.any(cur[3:6] != mid[3:6], axis=0) | np.any(cur[10:13] != mid[10:13], axis=0)
+        prev = cur
+        if s in keep:
+            ref = d["states"][keep.index(s)]
+            bad = _history_divergence(coll[:s], d["collided"][:s])
+            ok = np.setdiff1d(np.arange(np_), bad)
+            assert len(bad) <= 0.05 * np_, f"step {s}: collision history differs for {bad}"
+            for q, sl in QUANT.items():
+                e = np.abs(cur[sl][:, ok] - ref[sl][:, ok]).max() / np.abs(ref[sl]).max()
+                assert e <= 1e-9, (s, q, e, "excluded:", bad)
+    assert coll.sum() > 100
+
+
+@pytest.mark.parametrize("prec,tol", [(G.PREC_FP64, 1e-9), (G.PREC_MIXED, 1e-5)])
+def test_trajectory_against_oracle_c1(c1, prec, tol):
+    """20 steps of the C1 configuration (64^3, k0 = 4, APR 2, St 1) with
+    collisions on: per-quantity relative error vs the FP64 oracle."""
+    ctx = G.Context.from_workload(c1, precision=prec)
+    stp = O.Stepper(c1)
+    so = c1.state.copy()
+    ncoll_o = ncoll_g = 0
+    for s in range(20):
+        ctx.step(1)
+        so, err, nc, ncoll = stp.step(so)
+        assert err == 0
+        ncoll_o += ncoll
+        ncoll_g += ctx.stats()["ncollision"]
+    sg = ctx.get_particles()
+    assert ncoll_o == ncoll_g
+    for name, sl in QUANT.items():
+        assert relerr(sg, so, sl) <= tol, name
+
+
+def test_step_equals_adv_plus_collision_and_is_deterministic(c1):
+    a = G.Context.from_workload(c1)
+    b = G.Context.from_workload(c1)
+    c = G.Context.from_workload(c1)
+    for s in range(5):
+        a.adv_particle_ellp(s + 1)
+        a.collision(s + 1)
+    b.step(5)
+    c.step(5)
+    sa, sb, sc = a.get_particles(), b.get_particles(), c.get_particles()
+    np.testing.assert_array_equal(sa, sb)
+    np.testing.assert_array_equal(sb, sc)
+
+
+def test_error_nan_drag(c1):
+    """u_p = u_f exactly gives Re_p = 0 and C_D = NaN: the reference stops
+    (fluid_force_ellipsoid.f90:150-158); the library returns the code."""
+    ctx = G.Context.from_workload(c1)
+    st = c1.state.copy()
+    for c in range(3):
+        st[3 + c] = ctx.interph(c1.scheme, c, st[0], st[1], st[2])
+    st[10:13] = 0.0
+    ctx.set_particles(st, c1.species_id)
+    with pytest.raises(G.GpError) as e:
+        ctx.adv_particle_ellp(1)
+    assert e.value.code == 1
+
+
+def test_error_ke_collision():
+    """Restitution below 1 breaks the kinetic-energy balance the reference
+    enforces (collision_response.f90:408-436)."""
+    d = np.load(GOLD + "golden_trajectory.npz")
+    wl = golden_workload(d)
+    wl.species = W.species_spheroid(float(d["emaj"]), float(d["apr"]), float(d["rhop"]),
+                                    ecp=0.5)[None, :]
+    ctx = G.Context.from_workload(wl)
+    ctx.adv_particle_ellp(1)
+    with pytest.raises(G.GpError) as e:
+        ctx.collision(1)
+    assert e.value.code == 3
+
+
+def test_error_out_of_grid_and_interp_range(c1):
+    ctx = G.Context.from_workload(c1)
+    st = c1.state.copy()
+    st[0, 7] = c1.L + 0.5
+    ctx.set_particles(st, c1.species_id)
+    with pytest.raises(G.GpError) as e:
+        ctx.collision_detection_connectivity()
+    assert e.value.code == 5
+    with pytest.raises(G.GpError) as e:
+        ctx.interph(3, 0, np.array([-1.0]), np.array([1.0]), np.array([1.0]))
+    assert e.value.code == 7
+    with pytest.raises(G.GpError) as e:
+        ctx.interph(5, 0, np.array([1.0]), np.array([1.0]), np.array([1.0]))
+    assert e.value.code == 10
