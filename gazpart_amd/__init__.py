@@ -43,8 +43,8 @@ PREC_FP64, PREC_MIXED = 0, 1
 EXPORTS = (
     "gp_create", "gp_destroy", "gp_last_error_string", "gp_sync", "gp_bind_field",
     "gp_set_species", "gp_set_particles", "gp_get_particles", "gp_get_fluid_at_particles",
-    "gp_interph", "gp_adv_particle_ellp", "gp_collision_detection", "gp_contact_batch",
-    "gp_collision", "gp_step", "gp_get_stats", "gp_collision_grid",
+    "gp_interph", "gp_interph_field", "gp_adv_particle_ellp", "gp_collision_detection", "gp_contact_batch",
+    "gp_collision", "gp_step", "gp_get_stats", "gp_reset_timers", "gp_collision_grid",
 )
 
 
@@ -67,7 +67,9 @@ class Stats(C.Structure):
     """gp_stats (include/gazpart_gpu.h)."""
     _fields_ = [("nclose", C.c_int64), ("nprefilter", C.c_int64), ("ncontact", C.c_int64),
                 ("ncollision", C.c_int64), ("lm_nfev", C.c_int64), ("nwall", C.c_int64),
-                ("greedy_rounds", C.c_int64), ("reserved", C.c_int64 * 9)]
+                ("greedy_rounds", C.c_int64), ("t_advance_ms", C.c_double),
+                ("t_sort_ms", C.c_double), ("t_contact_ms", C.c_double),
+                ("t_greedy_ms", C.c_double), ("ntimed", C.c_int64), ("reserved", C.c_int64 * 4)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -96,6 +98,7 @@ def load_library(path: str = LIB_PATH):
     L.gp_get_particles.argtypes = [vp, C.c_int, dp]
     L.gp_get_fluid_at_particles.argtypes = [vp, C.c_int, dp]
     L.gp_interph.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, C.c_int, dp]
+    L.gp_interph_field.argtypes = [vp, C.c_int, dp, C.c_int, dp, dp, dp, dp]
     L.gp_adv_particle_ellp.argtypes = [vp, C.c_int]
     L.gp_collision_detection.argtypes = [vp, C.c_int64, ip, ip, C.POINTER(C.c_int64)]
     L.gp_contact_batch.argtypes = [vp, C.c_int64, ip, ip, vp, dp, dp, dp, ip]
@@ -103,6 +106,7 @@ def load_library(path: str = LIB_PATH):
     L.gp_step.argtypes = [vp, C.c_int]
     L.gp_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.gp_collision_grid.argtypes = [vp, ip, dp]
+    L.gp_reset_timers.argtypes = [vp]
     _lib = L
     return L
 
@@ -117,13 +121,14 @@ class Context:
 
     def __init__(self, n, L, visc, rhof, dtime, *, scheme=3, precision=PREC_FP64, ecw=1.0,
                  muw=0.0, dpmax=0.0, prefilter=1, xcd_remap=1, renorm_quat=0,
-                 keep_fluid=False):
+                 keep_fluid=False, timing=False):
         self.lib = load_library()
         p = Params()
         p.n, p.interp_scheme, p.precision, p.renorm_quat = n, scheme, precision, renorm_quat
         p.L, p.visc, p.rhof, p.dtime, p.ecw, p.muw, p.dpmax = L, visc, rhof, dtime, ecw, muw, dpmax
         p.prefilter, p.xcd_remap = prefilter, xcd_remap
         p.reserved[0] = 1 if keep_fluid else 0
+        p.reserved[1] = 1 if timing else 0
         self.params = p
         h = C.c_void_p()
         rc = self.lib.gp_create(C.byref(p), C.byref(h))
@@ -193,6 +198,9 @@ class Context:
         self._chk(self.lib.gp_get_stats(self.h, C.byref(s)))
         return s.as_dict()
 
+    def reset_timers(self):
+        self._chk(self.lib.gp_reset_timers(self.h))
+
     def collision_grid(self):
         nb = np.zeros(4, dtype=np.int32)
         bm = np.zeros(6)
@@ -207,6 +215,18 @@ class Context:
         out = np.empty_like(x)
         self._chk(self.lib.gp_interph(self.h, int_scheme, len(x), _ptr(x), _ptr(y), _ptr(z),
                                       comp, _ptr(out)))
+        return out
+
+    def interph_field(self, int_scheme, uain, xint, yint, zint):
+        """INTERPH with the field component passed in (UAIN of shape
+        (n+4, n+4, n) in numpy order = UAIN(1:n,-1:n+2,-1:n+2))."""
+        n = self.params.n
+        u = np.ascontiguousarray(uain, dtype=np.float64)
+        assert u.shape == (n + 4, n + 4, n), u.shape
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (xint, yint, zint))
+        out = np.empty_like(x)
+        self._chk(self.lib.gp_interph_field(self.h, int_scheme, _ptr(u), len(x), _ptr(x), _ptr(y),
+                                            _ptr(z), _ptr(out)))
         return out
 
     def adv_particle_ellp(self, ncycle=1):

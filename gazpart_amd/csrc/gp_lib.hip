@@ -99,9 +99,11 @@ __device__ __forceinline__ void lweights(CT a, CT* f) {
   }
 }
 
-// one x-row of S nodes x 6 components, contiguous in the interleaved layout
-template <typename FT, int S> struct Row;
-template <int S> struct Row<double, S> {
+// one x-row of S nodes x NC components, contiguous in the interleaved
+// layout; NC = 6 rows start 16-byte (FP64) / 8-byte (FP32) aligned and are
+// read as double2 / float2, NC = 1 rows element by element
+template <typename FT, int S, int NC> struct Row;
+template <int S> struct Row<double, S, 6> {
   double v[S * 6];
   __device__ __forceinline__ void load(const double* __restrict__ p) {
     const double2* q = reinterpret_cast<const double2*>(p);
@@ -112,7 +114,7 @@ template <int S> struct Row<double, S> {
     }
   }
 };
-template <int S> struct Row<float, S> {
+template <int S> struct Row<float, S, 6> {
   float v[S * 6];
   __device__ __forceinline__ void load(const float* __restrict__ p) {
     const float2* q = reinterpret_cast<const float2*>(p);
@@ -123,15 +125,22 @@ template <int S> struct Row<float, S> {
     }
   }
 };
+template <typename FT, int S> struct Row<FT, S, 1> {
+  FT v[S];
+  __device__ __forceinline__ void load(const FT* __restrict__ p) {
+#pragma unroll
+    for (int i = 0; i < S; ++i) v[i] = p[i];
+  }
+};
 
 // INTERP_LAG3 / INTERP_LAG4 of all six fields at once (interp_lag3.f90:
 // 119-290, interp_lag4.f90:84-312).  The field carries ghost planes in x as
 // well (index period n, the reference's x wrap of interp_lag3.f90:132-154),
 // so the stencil is branch-free.  For lag4 the reference's FYP2 row reads
 // J+1 (interp_lag4.f90:196,222,248,274,300): that row gets weight FYP1+FYP2.
-template <typename FT, typename CT, int S>
+template <typename FT, typename CT, int S, int NC = 6>
 __device__ __forceinline__ bool gather6(const FT* __restrict__ f, int n, double dx, double x,
-                                        double y, double z, CT out[6]) {
+                                        double y, double z, CT out[NC]) {
   if (!(x == x && y == y && z == z)) return false;
   const double tx = x / dx, ty = y / dx, tz = z / dx;
   if (!(tx > -1.0 && ty > -1.0 && tz > -1.0 && tx < n && ty < n && tz < n)) return false;
@@ -150,27 +159,31 @@ __device__ __forceinline__ bool gather6(const FT* __restrict__ f, int n, double 
     wy[0] = fy[0]; wy[1] = fy[1]; wy[2] = fy[2]; wy[3] = fy[3] + fy[4];
   }
   const int n4 = n + 4;
-  CT acc[6] = {0, 0, 0, 0, 0, 0};
+  CT acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = CT(0);
 #pragma unroll
   for (int kz = 0; kz < S; ++kz) {
-    CT sy[6] = {0, 0, 0, 0, 0, 0};
+    CT sy[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) sy[c] = CT(0);
 #pragma unroll
     for (int jy = 0; jy < NY; ++jy) {
-      Row<FT, S> r;
-      r.load(f + node_index(n4, i0 + lo, j0 + lo + jy, k0 + lo + kz) * 6);
+      Row<FT, S, NC> r;
+      r.load(f + node_index(n4, i0 + lo, j0 + lo + jy, k0 + lo + kz) * NC);
 #pragma unroll
-      for (int c = 0; c < 6; ++c) {
+      for (int c = 0; c < NC; ++c) {
         CT sx = 0;
 #pragma unroll
-        for (int ix = 0; ix < S; ++ix) sx += fx[ix] * CT(r.v[ix * 6 + c]);
+        for (int ix = 0; ix < S; ++ix) sx += fx[ix] * CT(r.v[ix * NC + c]);
         sy[c] += wy[jy] * sx;
       }
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) acc[c] += fz[kz] * sy[c];
+    for (int c = 0; c < NC; ++c) acc[c] += fz[kz] * sy[c];
   }
 #pragma unroll
-  for (int c = 0; c < 6; ++c) out[c] = acc[c];
+  for (int c = 0; c < NC; ++c) out[c] = acc[c];
   return true;
 }
 
@@ -261,7 +274,7 @@ __global__ void k_fill(T* p, T v, long long n) {
 
 // reference layout UAIN(1:n,-1:n+2,-1:n+2) x 6 -> interleaved nodes with x
 // ghosts, [(k+1)(n+4)+(j+1)](n+4)+(i+1) nodes of 6 values
-template <typename FT>
+template <typename FT, int NC = 6>
 __global__ void k_field_prep(const double* __restrict__ ref, FT* __restrict__ out, int n) {
   const int n4 = n + 4;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -275,7 +288,7 @@ __global__ void k_field_prep(const double* __restrict__ ref, FT* __restrict__ ou
   const long long comp = (long long)n * n4 * n4;
   const long long src = (long long)(is - 1) + (long long)jj * n + (long long)kk * n * n4;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) out[t * 6 + c] = (FT)ref[c * comp + src];
+  for (int c = 0; c < NC; ++c) out[t * NC + c] = (FT)ref[c * comp + src];
 }
 
 template <typename FT, typename CT, int S>
@@ -294,6 +307,24 @@ __global__ __launch_bounds__(BLOCK) void k_interph(const FT* __restrict__ f, int
     return;
   }
   out[p] = (double)v[comp];
+}
+
+template <typename FT, typename CT, int S>
+__global__ __launch_bounds__(BLOCK) void k_interph1(const FT* __restrict__ f, int n, double dx,
+                                                    const double* __restrict__ x,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ z,
+                                                    double* __restrict__ out, int nint,
+                                                    unsigned* err) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nint) return;
+  CT v[1];
+  if (!gather6<FT, CT, S, 1>(f, n, dx, x[p], y[p], z[p], v)) {
+    atomicOr(err, EB_INTERP_RANGE);
+    out[p] = 0.0;
+    return;
+  }
+  out[p] = (double)v[0];
 }
 
 // fused gather + advance + wrap + cell binning; dst[i] = advance(src[perm[i]])
@@ -736,6 +767,13 @@ struct gp_ctx {
   double* flout = nullptr;
   gp_stats stats{};
   long long last_nclose = 0;
+  // stage timing: ev[0] before the advance, ev[1] after it, ev[2] after the
+  // pair-count scan, ev[3] after the LM, ev[4] after the greedy pass
+  bool timing = false;
+  hipEvent_t ev[5] = {};
+  bool ev_pending = false;
+  double t_stage[4] = {0, 0, 0, 0};
+  long long ntimed = 0;
 };
 
 #define HIPCHK(x)                                                               \
@@ -894,6 +932,13 @@ int gp_create(const gp_params* p, gp_ctx** out) {
   }
   hipMemset(ctx->d_err, 0, sizeof(unsigned));
   hipMemset(ctx->d_cnt, 0, 16 * sizeof(unsigned long long));
+  ctx->timing = p->reserved[1] != 0;
+  if (ctx->timing)
+    for (auto& e : ctx->ev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        delete ctx;
+        return GP_ERR_HIP;
+      }
   hipDeviceSynchronize();
   *out = ctx;
   return GP_OK;
@@ -908,6 +953,8 @@ int gp_destroy(gp_ctx* ctx) {
                   ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout};
   for (void* b : bufs)
     if (b) hipFree(b);
+  for (auto& e : ctx->ev)
+    if (e) hipEventDestroy(e);
   if (ctx->h_cnt) hipHostFree(ctx->h_cnt);
   if (ctx->h_total) hipHostFree(ctx->h_total);
   hipStreamDestroy(ctx->st);
@@ -1108,11 +1155,60 @@ int gp_interph(gp_ctx* ctx, int scheme, int nint, const double* x, const double*
   return check_err(ctx);
 }
 
+int gp_interph_field(gp_ctx* ctx, int scheme, const double* uain, int nint, const double* x,
+                     const double* y, const double* z, double* out) {
+  if (!ctx || !uain || nint < 0 || (scheme != 3 && scheme != 4)) return GP_ERR_ARG;
+  if (nint == 0) return GP_OK;
+  const int n = ctx->prm.n, n4 = n + 4;
+  const size_t nref = (size_t)n * n4 * n4, nodes = (size_t)n4 * n4 * n4;
+  char* d = nullptr;
+  const size_t bytes = (nref + nodes + 4 * (size_t)nint) * sizeof(double) + 256;
+  HIPCHK(hipMalloc((void**)&d, bytes));
+  double* dref = (double*)d;
+  double* dfld = dref + nref;
+  double* dx_ = dfld + nodes;
+  double* dy_ = dx_ + nint;
+  double* dz_ = dy_ + nint;
+  double* dout = dz_ + nint;
+  HIPCHK(hipMemcpyAsync(dref, uain, nref * 8, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(dx_, x, (size_t)nint * 8, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(dy_, y, (size_t)nint * 8, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(dz_, z, (size_t)nint * 8, hipMemcpyHostToDevice, ctx->st));
+  hipLaunchKernelGGL((k_field_prep<double, 1>), dim3(nblk((long long)nodes)), dim3(BLOCK), 0,
+                     ctx->st, dref, dfld, n);
+  if (scheme == 3)
+    hipLaunchKernelGGL((k_interph1<double, double, 4>), dim3(nblk(nint)), dim3(BLOCK), 0, ctx->st,
+                       dfld, n, ctx->sp.dx, dx_, dy_, dz_, dout, nint, ctx->d_err);
+  else
+    hipLaunchKernelGGL((k_interph1<double, double, 5>), dim3(nblk(nint)), dim3(BLOCK), 0, ctx->st,
+                       dfld, n, ctx->sp.dx, dx_, dy_, dz_, dout, nint, ctx->d_err);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, (size_t)nint * 8, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  hipFree(d);
+  return check_err(ctx);
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
 // step pieces (asynchronous on ctx->st)
 // ---------------------------------------------------------------------------
+// adds the stage times of the last timed step (waits for its last event)
+static void collect_timing(gp_ctx* ctx) {
+  if (!ctx->timing || !ctx->ev_pending) return;
+  if (hipEventSynchronize(ctx->ev[4]) != hipSuccess) return;
+  for (int k = 0; k < 4; ++k) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]) == hipSuccess)
+      ctx->t_stage[k] += ms;
+  }
+  ctx->ntimed++;
+  ctx->ev_pending = false;
+}
+static void mark(gp_ctx* ctx, int k) {
+  if (ctx->timing) hipEventRecord(ctx->ev[k], ctx->st);
+}
 template <typename VT>
 static int launch_advance(gp_ctx* ctx, bool bin) {
   const int np = ctx->np;
@@ -1124,6 +1220,8 @@ static int launch_advance(gp_ctx* ctx, bool bin) {
   const int xcd = ctx->prm.xcd_remap;
   auto ps = pstate<VT>(ctx, src), pd = pstate<VT>(ctx, dst);
   int* cell = bin ? ctx->cell : nullptr;
+  collect_timing(ctx);
+  mark(ctx, 0);
   unsigned nb = nblk(np);
   if (xcd) nb = (nb + 7) / 8 * 8;
   const dim3 gr(nb), bl(BLOCK);
@@ -1147,6 +1245,7 @@ static int launch_advance(gp_ctx* ctx, bool bin) {
                          ctx->slot, ctx->cnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd);
   }
   HIPCHK(hipGetLastError());
+  mark(ctx, 1);
   ctx->cur = dst;
   ctx->perm_identity = true;
   ctx->cells_valid = bin;
@@ -1185,6 +1284,7 @@ static int launch_sort_count(gp_ctx* ctx) {
                      ctx->cell, ctx->off, ctx->cnt, ctx->pcount, np);
   scan_exclusive<int, long long>(ctx->pcount, ctx->poff, np, (long long*)ctx->scan_scr, ctx->st);
   HIPCHK(hipGetLastError());
+  mark(ctx, 2);
   HIPCHK(hipMemcpyAsync(ctx->h_total, ctx->poff + np, sizeof(long long), hipMemcpyDeviceToHost,
                         ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
@@ -1223,10 +1323,13 @@ static int launch_collide(gp_ctx* ctx) {
   hipLaunchKernelGGL(k_contact<VT>, dim3(grid), dim3(BLOCK), 0, ctx->st, ps, ctx->spec, ctx->work,
                      c + 1, ctx->work_cap, ctx->prm.L, ctx->act, c + 2, ctx->act_cap, c + 3,
                      c + 4, ctx->d_err);
+  mark(ctx, 3);
   hipLaunchKernelGGL(k_greedy<VT>, dim3(1), dim3(GREEDY_BLOCK), 0, ctx->st, ps, ctx->spec,
                      ctx->act, c + 2, ctx->act_cap, ctx->mark, ctx->used, ctx->act_status, c + 5,
                      c + 6, ctx->d_err);
   HIPCHK(hipGetLastError());
+  mark(ctx, 4);
+  ctx->ev_pending = ctx->timing;
   // the next advance reads the particles in cell order
   std::swap(ctx->perm, ctx->sorted);
   ctx->perm_identity = false;
@@ -1381,8 +1484,23 @@ int gp_step(gp_ctx* ctx, int nsteps) {
 int gp_get_stats(gp_ctx* ctx, gp_stats* out) {
   if (!ctx || !out) return GP_ERR_ARG;
   HIPCHK(hipStreamSynchronize(ctx->st));
+  collect_timing(ctx);
   accumulate_stats(ctx);
+  ctx->stats.t_advance_ms = ctx->t_stage[0];
+  ctx->stats.t_sort_ms = ctx->t_stage[1];
+  ctx->stats.t_contact_ms = ctx->t_stage[2];
+  ctx->stats.t_greedy_ms = ctx->t_stage[3];
+  ctx->stats.ntimed = ctx->ntimed;
   *out = ctx->stats;
+  return GP_OK;
+}
+
+int gp_reset_timers(gp_ctx* ctx) {
+  if (!ctx) return GP_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  collect_timing(ctx);
+  for (double& t : ctx->t_stage) t = 0.0;
+  ctx->ntimed = 0;
   return GP_OK;
 }
 

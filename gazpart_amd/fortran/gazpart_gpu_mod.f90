@@ -1,0 +1,162 @@
+!! gazpart_gpu_mod.f90 -- iso_c_binding interface to libgazpart_gpu.so
+!! (include/gazpart_gpu.h).  Standalone: needs no module of the reference.
+module gazpart_gpu
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: GP_OK = 0
+  integer(c_int), parameter :: GP_PREC_FP64 = 0, GP_PREC_MIXED = 1
+  integer(c_int), parameter :: GP_SP_STRIDE = 14
+
+  type, bind(C) :: gp_params
+    integer(c_int) :: n, interp_scheme, precision, renorm_quat
+    real(c_double) :: L, visc, rhof, dtime, ecw, muw, dpmax
+    integer(c_int) :: prefilter, xcd_remap
+    integer(c_int) :: reserved(6)
+  end type gp_params
+
+  type, bind(C) :: gp_stats
+    integer(c_int64_t) :: nclose, nprefilter, ncontact, ncollision, lm_nfev, nwall
+    integer(c_int64_t) :: greedy_rounds
+    real(c_double) :: t_advance_ms, t_sort_ms, t_contact_ms, t_greedy_ms
+    integer(c_int64_t) :: ntimed
+    integer(c_int64_t) :: reserved(4)
+  end type gp_stats
+
+  interface
+    integer(c_int) function gp_create(p, ctx) bind(C, name='gp_create')
+      import :: c_int, c_ptr, gp_params
+      type(gp_params), intent(in) :: p
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function gp_destroy(ctx) bind(C, name='gp_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    type(c_ptr) function gp_last_error_string(ctx) bind(C, name='gp_last_error_string')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function gp_sync(ctx) bind(C, name='gp_sync')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function gp_bind_field(ctx, field) bind(C, name='gp_bind_field')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: field(*)
+    end function
+    integer(c_int) function gp_set_species(ctx, nspec, rows) bind(C, name='gp_set_species')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nspec
+      real(c_double), intent(in) :: rows(*)
+    end function
+    integer(c_int) function gp_set_particles(ctx, np, state, species) &
+        bind(C, name='gp_set_particles')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: np
+      real(c_double), intent(in) :: state(*)
+      integer(c_int), intent(in) :: species(*)
+    end function
+    integer(c_int) function gp_get_particles(ctx, np, state) bind(C, name='gp_get_particles')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: np
+      real(c_double), intent(out) :: state(*)
+    end function
+    integer(c_int) function gp_get_fluid_at_particles(ctx, np, fl) &
+        bind(C, name='gp_get_fluid_at_particles')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: np
+      real(c_double), intent(out) :: fl(*)
+    end function
+    integer(c_int) function gp_interph(ctx, scheme, nint, x, y, z, comp, out) &
+        bind(C, name='gp_interph')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: scheme, nint, comp
+      real(c_double), intent(in) :: x(*), y(*), z(*)
+      real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function gp_interph_field(ctx, scheme, uain, nint, x, y, z, out) &
+        bind(C, name='gp_interph_field')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: scheme, nint
+      real(c_double), intent(in) :: uain(*), x(*), y(*), z(*)
+      real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function gp_adv_particle_ellp(ctx, ncycle) bind(C, name='gp_adv_particle_ellp')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ncycle
+    end function
+    integer(c_int) function gp_collision_detection(ctx, nclosemax, home, near, nclose) &
+        bind(C, name='gp_collision_detection')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nclosemax
+      integer(c_int), intent(out) :: home(*), near(*)
+      integer(c_int64_t), intent(out) :: nclose
+    end function
+    integer(c_int) function gp_contact_batch(ctx, npairs, home, near, flag, ept1, ept2, &
+                                             cnormal, nfev) bind(C, name='gp_contact_batch')
+      import :: c_int, c_ptr, c_int64_t, c_double, c_signed_char
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: npairs
+      integer(c_int), intent(in) :: home(*), near(*)
+      integer(c_signed_char), intent(out) :: flag(*)
+      real(c_double), intent(out) :: ept1(*), ept2(*), cnormal(*)
+      integer(c_int), intent(out) :: nfev(*)
+    end function
+    integer(c_int) function gp_collision(ctx, ncycle, time) bind(C, name='gp_collision')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ncycle
+      real(c_double), value :: time
+    end function
+    integer(c_int) function gp_step(ctx, nsteps) bind(C, name='gp_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function gp_get_stats(ctx, st) bind(C, name='gp_get_stats')
+      import :: c_int, c_ptr, gp_stats
+      type(c_ptr), value :: ctx
+      type(gp_stats), intent(out) :: st
+    end function
+    integer(c_int) function gp_reset_timers(ctx) bind(C, name='gp_reset_timers')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function gp_collision_grid(ctx, nbox, boxr) bind(C, name='gp_collision_grid')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: nbox(4)
+      real(c_double), intent(out) :: boxr(6)
+    end function
+  end interface
+
+contains
+
+  !! message of the last error of a context, as a Fortran string
+  function gp_error_message(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=256) :: msg
+    character(kind=c_char), pointer :: s(:)
+    type(c_ptr) :: p
+    integer :: i
+    msg = ' '
+    p = gp_last_error_string(ctx)
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, s, [256])
+    do i = 1, 256
+      if (s(i) == c_null_char) exit
+      msg(i:i) = s(i)
+    end do
+  end function gp_error_message
+
+end module gazpart_gpu

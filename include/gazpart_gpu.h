@@ -84,7 +84,13 @@ typedef struct {
   int64_t lm_nfev;      /* function evaluations summed over LM solves */
   int64_t nwall;        /* wall bounces in the last advance */
   int64_t greedy_rounds;
-  int64_t reserved[9];
+  /* stage times accumulated with HIP events on the context's stream when
+   * gp_params.reserved[1] = 1 (0 otherwise): advance (gather + RK2 + walls
+   * + wrap + binning), sort (scan, stable scatter, pair counts), contact
+   * (prefilter + LM), greedy (one-collision pass + impulses) */
+  double t_advance_ms, t_sort_ms, t_contact_ms, t_greedy_ms;
+  int64_t ntimed;       /* steps included in the stage times */
+  int64_t reserved[4];
 } gp_stats;
 
 /* lifetime ----------------------------------------------------------- */
@@ -111,6 +117,12 @@ int gp_get_fluid_at_particles(gp_ctx *ctx, int np, double *fl);
  * host positions; scheme 3 or 4.                                       */
 int gp_interph(gp_ctx *ctx, int scheme, int nint, const double *x, const double *y,
                const double *z, int comp, double *out);
+/* INTERPH with the field passed in, as the reference passes UAIN: one
+ * component UAIN(1:n,-1:n+2,-1:n+2) of the context's grid, FP64 gather.
+ * (Copies the component to the device on every call; the step itself
+ * uses the field bound by gp_bind_field.)                              */
+int gp_interph_field(gp_ctx *ctx, int scheme, const double *uain, int nint, const double *x,
+                     const double *y, const double *z, double *out);
 /* ADV_PARTICLE_ELLP (adv_particle_new.f90:7-704, called at main.f90:466)
  * on the device-resident particles, including BOUNDARY_PARTICLE.       */
 int gp_adv_particle_ellp(gp_ctx *ctx, int ncycle);
@@ -133,6 +145,8 @@ int gp_collision(gp_ctx *ctx, int ncycle, double time);
 int gp_step(gp_ctx *ctx, int nsteps);
 int gp_get_stats(gp_ctx *ctx, gp_stats *out);
 
+/* zero the accumulated stage times                                       */
+int gp_reset_timers(gp_ctx *ctx);
 /* introspection for benchmarks: collision grid of the context           */
 int gp_collision_grid(gp_ctx *ctx, int *nbox_xyz, double *box_min_size);
 

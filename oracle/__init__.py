@@ -24,7 +24,7 @@ REF_PATH = os.path.join(HERE, "_ref", "libgazpart_ref.so")
 REF_SRC = "/root/reference/source"
 
 NSTATE = 13
-SP_STRIDE = 15
+SP_STRIDE = 14
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")

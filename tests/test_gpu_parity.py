@@ -60,6 +60,19 @@ def test_interph_against_reference_fixture(leaf, prec, tol):
             assert np.abs(out - ref).max() <= tol * np.abs(ref).max(), (scheme, c)
 
 
+def test_interph_field_argument_against_reference_fixture(leaf):
+    """INTERPH with UAIN passed in (gp_interph_field), one component at a
+    time, no bound field."""
+    n, L = int(leaf["n"]), float(leaf["L"])
+    ctx = G.Context(n, L, 0.01, 1.0, 1e-3)
+    pts = leaf["interp_pts"]
+    for scheme, key in ((3, "interp3"), (4, "interp4")):
+        for c in range(6):
+            out = ctx.interph_field(scheme, leaf["field"][c], *pts)
+            ref = leaf[key][c]
+            assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max(), (scheme, c)
+
+
 @pytest.mark.parametrize("prec,tol", [(G.PREC_FP64, 1e-13), (G.PREC_MIXED, 2e-6)])
 def test_interph_against_oracle(c1, prec, tol):
     ctx = G.Context.from_workload(c1, precision=prec)
