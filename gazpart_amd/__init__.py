@@ -60,7 +60,8 @@ class Params(C.Structure):
                 ("renorm_quat", C.c_int), ("L", C.c_double), ("visc", C.c_double),
                 ("rhof", C.c_double), ("dtime", C.c_double), ("ecw", C.c_double),
                 ("muw", C.c_double), ("dpmax", C.c_double), ("prefilter", C.c_int),
-                ("xcd_remap", C.c_int), ("reserved", C.c_int * 6)]
+                ("xcd_remap", C.c_int), ("keep_fluid", C.c_int), ("timing", C.c_int),
+                ("graph", C.c_int), ("reserved", C.c_int * 3)]
 
 
 class Stats(C.Structure):
@@ -121,14 +122,15 @@ class Context:
 
     def __init__(self, n, L, visc, rhof, dtime, *, scheme=3, precision=PREC_FP64, ecw=1.0,
                  muw=0.0, dpmax=0.0, prefilter=1, xcd_remap=1, renorm_quat=0,
-                 keep_fluid=False, timing=False):
+                 keep_fluid=False, timing=False, graph=False):
         self.lib = load_library()
         p = Params()
         p.n, p.interp_scheme, p.precision, p.renorm_quat = n, scheme, precision, renorm_quat
         p.L, p.visc, p.rhof, p.dtime, p.ecw, p.muw, p.dpmax = L, visc, rhof, dtime, ecw, muw, dpmax
         p.prefilter, p.xcd_remap = prefilter, xcd_remap
-        p.reserved[0] = 1 if keep_fluid else 0
-        p.reserved[1] = 1 if timing else 0
+        p.keep_fluid = 1 if keep_fluid else 0
+        p.timing = 1 if timing else 0
+        p.graph = 1 if graph else 0
         self.params = p
         h = C.c_void_p()
         rc = self.lib.gp_create(C.byref(p), C.byref(h))

@@ -774,7 +774,18 @@ struct gp_ctx {
   bool ev_pending = false;
   double t_stage[4] = {0, 0, 0, 0};
   long long ntimed = 0;
+  // captured two-step graph (gp_params.graph) and the buffer configuration
+  // it was captured in
+  hipGraphExec_t gexec = nullptr;
+  int g_cur = -1;
+  int* g_perm = nullptr;
+  bool g_perm_identity = false;
 };
+
+static void drop_graph(gp_ctx* ctx) {
+  if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
+  ctx->gexec = nullptr;
+}
 
 #define HIPCHK(x)                                                               \
   do {                                                                          \
@@ -932,7 +943,7 @@ int gp_create(const gp_params* p, gp_ctx** out) {
   }
   hipMemset(ctx->d_err, 0, sizeof(unsigned));
   hipMemset(ctx->d_cnt, 0, 16 * sizeof(unsigned long long));
-  ctx->timing = p->reserved[1] != 0;
+  ctx->timing = p->timing != 0 && p->graph == 0;
   if (ctx->timing)
     for (auto& e : ctx->ev)
       if (hipEventCreate(&e) != hipSuccess) {
@@ -955,6 +966,7 @@ int gp_destroy(gp_ctx* ctx) {
     if (b) hipFree(b);
   for (auto& e : ctx->ev)
     if (e) hipEventDestroy(e);
+  drop_graph(ctx);
   if (ctx->h_cnt) hipHostFree(ctx->h_cnt);
   if (ctx->h_total) hipHostFree(ctx->h_total);
   hipStreamDestroy(ctx->st);
@@ -973,6 +985,8 @@ int gp_sync(gp_ctx* ctx) {
 
 int gp_bind_field(gp_ctx* ctx, const double* field) {
   if (!ctx || !field) return GP_ERR_ARG;
+  hipStreamSynchronize(ctx->st);
+  drop_graph(ctx);
   const int n = ctx->prm.n, n4 = n + 4;
   const size_t nref = (size_t)6 * n * n4 * n4;
   const size_t nodes = (size_t)n4 * n4 * n4;
@@ -998,6 +1012,8 @@ int gp_bind_field(gp_ctx* ctx, const double* field) {
 
 int gp_set_species(gp_ctx* ctx, int nspec, const double* rows) {
   if (!ctx || nspec < 1 || !rows) return GP_ERR_ARG;
+  hipStreamSynchronize(ctx->st);
+  drop_graph(ctx);
   std::vector<Species> h((size_t)nspec);
   double emax = 0.0;
   for (int s = 0; s < nspec; ++s) {
@@ -1029,6 +1045,8 @@ int gp_set_species(gp_ctx* ctx, int nspec, const double* rows) {
 
 int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* species) {
   if (!ctx || np < 1 || !state || !species) return GP_ERR_ARG;
+  hipStreamSynchronize(ctx->st);
+  drop_graph(ctx);
   if (!ctx->spec) {
     ctx->err = "gp_set_species must precede gp_set_particles";
     return GP_ERR_STATE;
@@ -1056,7 +1074,7 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   if ((rc = dalloc(ctx, &ctx->mark, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->used, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->pos_of_id, (size_t)np))) return rc;
-  if (ctx->prm.reserved[0]) {
+  if (ctx->prm.keep_fluid) {
     if ((rc = dalloc(ctx, &ctx->flout, (size_t)np * 6))) return rc;
   }
   if ((rc = ensure_grid_buffers(ctx))) return rc;
@@ -1265,10 +1283,9 @@ static int launch_bin(gp_ctx* ctx) {
   return GP_OK;
 }
 
-// counting sort + pair counts; leaves the total candidate count in
-// ctx->h_total[0] (synchronises the stream once)
+// counting sort + pair counts (asynchronous)
 template <typename VT>
-static int launch_sort_count(gp_ctx* ctx) {
+static int launch_sort(gp_ctx* ctx) {
   const int np = ctx->np;
   const CellGrid& g = ctx->g;
   int rc;
@@ -1285,34 +1302,48 @@ static int launch_sort_count(gp_ctx* ctx) {
   scan_exclusive<int, long long>(ctx->pcount, ctx->poff, np, (long long*)ctx->scan_scr, ctx->st);
   HIPCHK(hipGetLastError());
   mark(ctx, 2);
-  HIPCHK(hipMemcpyAsync(ctx->h_total, ctx->poff + np, sizeof(long long), hipMemcpyDeviceToHost,
-                        ctx->st));
+  return GP_OK;
+}
+
+// the total candidate count of the last sort (synchronises the stream)
+static int read_nclose(gp_ctx* ctx) {
+  HIPCHK(hipMemcpyAsync(ctx->h_total, ctx->poff + ctx->np, sizeof(long long),
+                        hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->last_nclose = ctx->h_total[0];
   return GP_OK;
 }
 
 template <typename VT>
-static int launch_collide(gp_ctx* ctx) {
-  const int np = ctx->np;
+static int launch_sort_count(gp_ctx* ctx) {
   int rc;
-  if ((rc = launch_sort_count<VT>(ctx))) return rc;
-  const long long nclose = ctx->last_nclose;
-  // work list sized for every candidate (prefilter survivors <= nclose)
-  const unsigned long long need = (unsigned long long)std::max<long long>(nclose, 1);
+  if ((rc = launch_sort<VT>(ctx))) return rc;
+  return read_nclose(ctx);
+}
+
+// work / active lists able to hold `need` prefilter survivors
+static int ensure_lists(gp_ctx* ctx, unsigned long long need) {
+  int rc;
   if (need > ctx->work_cap) {
+    drop_graph(ctx);
     const unsigned long long cap = need + need / 4 + 1024;
     if ((rc = dalloc(ctx, &ctx->work, (size_t)cap))) return rc;
     ctx->work_cap = cap;
   }
-  if (ctx->act_cap < ctx->work_cap) {
-    const unsigned long long cap = std::max<unsigned long long>(ctx->work_cap / 4, 4096);
-    if (cap > ctx->act_cap) {
-      if ((rc = dalloc(ctx, &ctx->act, (size_t)cap))) return rc;
-      if ((rc = dalloc(ctx, &ctx->act_status, (size_t)cap))) return rc;
-      ctx->act_cap = cap;
-    }
+  const unsigned long long acap = std::max<unsigned long long>(ctx->work_cap / 4, 4096);
+  if (acap > ctx->act_cap) {
+    drop_graph(ctx);
+    if ((rc = dalloc(ctx, &ctx->act, (size_t)acap))) return rc;
+    if ((rc = dalloc(ctx, &ctx->act_status, (size_t)acap))) return rc;
+    ctx->act_cap = acap;
   }
+  return GP_OK;
+}
+
+// prefilter + LM + greedy pass on the sorted lists (asynchronous)
+template <typename VT>
+static int launch_contacts(gp_ctx* ctx) {
+  const int np = ctx->np;
   unsigned long long* c = ctx->d_cnt;
   hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(8), 0, ctx->st, c + 1, 0ull, 6ll);
   auto ps = pstate<VT>(ctx, ctx->cur);
@@ -1334,6 +1365,81 @@ static int launch_collide(gp_ctx* ctx) {
   std::swap(ctx->perm, ctx->sorted);
   ctx->perm_identity = false;
   ctx->cells_valid = false;
+  return GP_OK;
+}
+
+// COLLISION with the list sizes read back from the device
+template <typename VT>
+static int launch_collide(gp_ctx* ctx) {
+  int rc;
+  if ((rc = launch_sort_count<VT>(ctx))) return rc;
+  // prefilter survivors <= candidates
+  if ((rc = ensure_lists(ctx, (unsigned long long)std::max<long long>(ctx->last_nclose, 1))))
+    return rc;
+  return launch_contacts<VT>(ctx);
+}
+
+// one step without any host synchronisation (lists already sized)
+template <typename VT>
+static int launch_step_async(gp_ctx* ctx) {
+  int rc;
+  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, ctx->d_cnt, 0ull,
+                     1ll);
+  if ((rc = launch_advance<VT>(ctx, true))) return rc;
+  if ((rc = launch_sort<VT>(ctx))) return rc;
+  return launch_contacts<VT>(ctx);
+}
+
+// captures two steps (the buffer flips of two steps cancel) into ctx->gexec
+template <typename VT>
+static int capture_graph(gp_ctx* ctx) {
+  drop_graph(ctx);
+  ctx->g_cur = ctx->cur;
+  ctx->g_perm = ctx->perm;
+  ctx->g_perm_identity = ctx->perm_identity;
+  HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+  int rc = launch_step_async<VT>(ctx);
+  if (!rc) rc = launch_step_async<VT>(ctx);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(ctx->st, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    ctx->err = std::string("stream capture failed: ") + hipGetErrorString(e);
+    return GP_ERR_HIP;
+  }
+  e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    ctx->gexec = nullptr;
+    ctx->err = std::string("graph instantiation failed: ") + hipGetErrorString(e);
+    return GP_ERR_HIP;
+  }
+  return GP_OK;
+}
+
+template <typename VT>
+static int graph_steps(gp_ctx* ctx, int nsteps) {
+  int rc;
+  const bool same = ctx->gexec && ctx->g_cur == ctx->cur && ctx->g_perm == ctx->perm &&
+                    ctx->g_perm_identity == ctx->perm_identity;
+  if (!same) {
+    // one eager step sizes the lists; headroom for the candidate count to grow
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, ctx->d_cnt, 0ull,
+                       1ll);
+    if ((rc = launch_advance<VT>(ctx, true))) return rc;
+    if ((rc = launch_collide<VT>(ctx))) return rc;
+    --nsteps;
+    const unsigned long long need = 2ull * (unsigned long long)ctx->last_nclose + 4096ull;
+    if ((rc = ensure_lists(ctx, need))) return rc;
+    if ((rc = capture_graph<VT>(ctx))) return rc;
+  }
+  for (; nsteps >= 2; nsteps -= 2) HIPCHK(hipGraphLaunch(ctx->gexec, ctx->st));
+  if (nsteps == 1) {
+    if ((rc = launch_step_async<VT>(ctx))) return rc;
+  }
   return GP_OK;
 }
 
@@ -1468,6 +1574,14 @@ int gp_collision(gp_ctx* ctx, int ncycle, double time) {
 int gp_step(gp_ctx* ctx, int nsteps) {
   int rc = ready(ctx);
   if (rc) return rc;
+  if (nsteps <= 0) return GP_OK;
+  if (ctx->prm.graph) {
+    rc = ctx->mixed ? graph_steps<float>(ctx, nsteps) : graph_steps<double>(ctx, nsteps);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 8 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, ctx->st));
+    return GP_OK;
+  }
   for (int s = 0; s < nsteps; ++s) {
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, ctx->d_cnt, 0ull,
                        1ll);

@@ -1,0 +1,231 @@
+!! gp_dropin.f90 -- drop-in GPU versions of the reference's hot-path
+!! subroutines, with their argument interfaces unchanged:
+!!
+!!   call INTERPH(...)            -> call GPU_INTERPH(...)          (same args)
+!!   call ADV_PARTICLE_ELLP(NCYCLE) -> call GPU_ADV_PARTICLE_ELLP(NCYCLE)
+!!   call COLLISION(NCYCLE, TIME)   -> call GPU_COLLISION(NCYCLE, TIME)
+!!
+!! Compiled together with the reference's own modules (DNS_DIM, PARAM_PHYS,
+!! PARTICLE_PARALLEL, FLUID_VARIABLE, GEOMETRIC_VARIABLE, COLLISION_VARIABLE)
+!! and linked with libgazpart_gpu.so.  GPU_INIT is called once after
+!! INITIATION_FROZEN_FLOW + FLUIDCOMM, the particle initialisation and
+!! COLLISION_INITIATION (main.f90:251-289); GPU_FINALIZE at the end.
+!!
+!! One GPU context per particle class (the reference collides particles of
+!! one class only, collision.f90:76-171).  With GPU_MIRROR = .true. (default)
+!! PART is copied to the device before and back after every call, so every
+!! other routine of the reference sees the usual PART; with .false. the
+!! device copy is authoritative and GPU_SYNC_PART refreshes PART on demand
+!! (before statistics or output).  A non-zero status stops the run with
+!! the library's message, as the reference's own `stop`s do.
+module gp_dropin
+  use iso_c_binding
+  use gazpart_gpu
+  use DNS_DIM
+  use PARAM_PHYS
+  use PARTICLE_PARALLEL
+  use FLUID_VARIABLE
+  use GEOMETRIC_VARIABLE
+  use COLLISION_VARIABLE
+  implicit none
+
+  type(c_ptr), allocatable, save :: GPU_CTX(:)
+  logical, save :: GPU_MIRROR = .true.
+  integer(c_int), save :: GPU_PRECISION = GP_PREC_FP64
+  logical, save :: GPU_DEVICE_AHEAD = .false.
+
+contains
+
+  subroutine GPU_CHECK(rc, ctx, what)
+    integer(c_int), intent(in) :: rc
+    type(c_ptr), intent(in) :: ctx
+    character(len=*), intent(in) :: what
+    if (rc /= GP_OK) then
+      write(*,*) 'gazpart_gpu: ', what, ' failed with status ', rc
+      if (c_associated(ctx)) write(*,*) trim(gp_error_message(ctx))
+      stop 'gazpart_gpu error'
+    end if
+  end subroutine GPU_CHECK
+
+  !! one context per class: grid, physics, species of the class, field, PART
+  subroutine GPU_INIT
+    type(gp_params) :: p
+    integer :: J, IDP, ns
+    integer(c_int) :: rc
+    real(c_double), allocatable :: rows(:,:), fld(:,:,:,:)
+
+    if (allocated(GPU_CTX)) call GPU_FINALIZE
+    allocate(GPU_CTX(NIG))
+    allocate(fld(NX, -1:NY+2, -1:NZ+2, 6))
+    fld(:,:,:,1) = UFLU; fld(:,:,:,2) = VFLU; fld(:,:,:,3) = WFLU
+    fld(:,:,:,4) = VORTICITY_X; fld(:,:,:,5) = VORTICITY_Y; fld(:,:,:,6) = VORTICITY_Z
+    do J = 1, NIG
+      p%n = NX
+      p%interp_scheme = INTERP_SCHEME
+      p%precision = GPU_PRECISION
+      p%renorm_quat = 0
+      p%L = LXMAX
+      p%visc = VISC
+      p%rhof = RHOF
+      p%dtime = DTIME
+      p%ecw = ECW
+      p%muw = MUW
+      p%dpmax = 2.0*maxval(EMAJ_PART(:,:))        ! collision_initiation.f90:50
+      p%prefilter = 1
+      p%xcd_remap = 1
+      p%keep_fluid = 0
+      p%timing = 0
+      p%graph = 0
+      p%reserved = 0
+      GPU_CTX(J) = c_null_ptr
+      rc = gp_create(p, GPU_CTX(J))
+      call GPU_CHECK(rc, GPU_CTX(J), 'gp_create')
+      rc = gp_bind_field(GPU_CTX(J), fld)
+      call GPU_CHECK(rc, GPU_CTX(J), 'gp_bind_field')
+      ns = POLYDISP(J)
+      allocate(rows(GP_SP_STRIDE, ns))
+      rows = 0.0
+      do IDP = 1, ns
+        ! the reference's own expressions (fluid_force_ellipsoid.f90:59,70,
+        ! calculate_ellipsoid_matrix.f90:33-35, initiation_particle_position.f90:620-622)
+        rows(1, IDP) = EMAJ_PART(J,IDP)
+        rows(2, IDP) = EMAJ_PART(J,IDP)/APR_PART(J)
+        rows(3, IDP) = rows(2, IDP)
+        rows(4, IDP) = RHOP(J, IDP)*(4.0/3.0)*PPI*(EMAJ_PART(J,IDP)**3.0)/((APR_PART(J)**(2.0)))
+        rows(5, IDP) = (2.0*EMAJ_PART(J,IDP))/(APR_PART(J)**(2.0/3.0))
+        rows(6, IDP) = IPXX(J,IDP)
+        rows(7, IDP) = IPYY(J,IDP)
+        rows(8, IDP) = IPZZ(J,IDP)
+        rows(9, IDP) = GRAVITY(J)
+        rows(10, IDP) = ECP(J)
+        rows(11, IDP) = MUP(J)
+        rows(12, IDP) = RHOP(J, IDP)
+        rows(13, IDP) = EMAJ_PART(J,IDP)
+        rows(14, IDP) = APR_PART(J)
+      end do
+      rc = gp_set_species(GPU_CTX(J), ns, rows)
+      call GPU_CHECK(rc, GPU_CTX(J), 'gp_set_species')
+      deallocate(rows)
+      call GPU_PUT_PART(J)
+    end do
+    deallocate(fld)
+    GPU_DEVICE_AHEAD = .false.
+  end subroutine GPU_INIT
+
+  subroutine GPU_FINALIZE
+    integer :: J
+    integer(c_int) :: rc
+    if (.not. allocated(GPU_CTX)) return
+    do J = 1, size(GPU_CTX)
+      if (c_associated(GPU_CTX(J))) rc = gp_destroy(GPU_CTX(J))
+    end do
+    deallocate(GPU_CTX)
+  end subroutine GPU_FINALIZE
+
+  !! PART(1:NPART_LOC(J), J) -> device
+  subroutine GPU_PUT_PART(J)
+    integer, intent(in) :: J
+    integer :: I, np
+    integer(c_int) :: rc
+    real(c_double), allocatable :: st(:,:)
+    integer(c_int), allocatable :: sp(:)
+    np = NPART_LOC(J)
+    if (np < 1) return
+    allocate(st(np, 13), sp(np))
+    do I = 1, np
+      st(I, 1) = PART(I,J)%XP; st(I, 2) = PART(I,J)%YP; st(I, 3) = PART(I,J)%ZP
+      st(I, 4) = PART(I,J)%UP; st(I, 5) = PART(I,J)%VP; st(I, 6) = PART(I,J)%WP
+      st(I, 7) = PART(I,J)%ELLQUAT%a; st(I, 8) = PART(I,J)%ELLQUAT%b
+      st(I, 9) = PART(I,J)%ELLQUAT%c; st(I, 10) = PART(I,J)%ELLQUAT%d
+      st(I, 11) = PART(I,J)%OMEGAX; st(I, 12) = PART(I,J)%OMEGAY; st(I, 13) = PART(I,J)%OMEGAZ
+      sp(I) = PART(I,J)%IDP - 1
+    end do
+    rc = gp_set_particles(GPU_CTX(J), np, st, sp)
+    call GPU_CHECK(rc, GPU_CTX(J), 'gp_set_particles')
+    deallocate(st, sp)
+  end subroutine GPU_PUT_PART
+
+  !! device -> PART(1:NPART_LOC(J), J)
+  subroutine GPU_GET_PART(J)
+    integer, intent(in) :: J
+    integer :: I, np
+    integer(c_int) :: rc
+    real(c_double), allocatable :: st(:,:)
+    np = NPART_LOC(J)
+    if (np < 1) return
+    allocate(st(np, 13))
+    rc = gp_get_particles(GPU_CTX(J), np, st)
+    call GPU_CHECK(rc, GPU_CTX(J), 'gp_get_particles')
+    do I = 1, np
+      PART(I,J)%XP = st(I, 1); PART(I,J)%YP = st(I, 2); PART(I,J)%ZP = st(I, 3)
+      PART(I,J)%UP = st(I, 4); PART(I,J)%VP = st(I, 5); PART(I,J)%WP = st(I, 6)
+      PART(I,J)%ELLQUAT%a = st(I, 7); PART(I,J)%ELLQUAT%b = st(I, 8)
+      PART(I,J)%ELLQUAT%c = st(I, 9); PART(I,J)%ELLQUAT%d = st(I, 10)
+      PART(I,J)%OMEGAX = st(I, 11); PART(I,J)%OMEGAY = st(I, 12); PART(I,J)%OMEGAZ = st(I, 13)
+    end do
+    deallocate(st)
+  end subroutine GPU_GET_PART
+
+  !! refresh PART from the device (resident mode)
+  subroutine GPU_SYNC_PART
+    integer :: J
+    if (.not. GPU_DEVICE_AHEAD) return
+    do J = 1, NIG
+      call GPU_GET_PART(J)
+    end do
+    GPU_DEVICE_AHEAD = .false.
+  end subroutine GPU_SYNC_PART
+
+  !! ADV_PARTICLE_ELLP (adv_particle_new.f90:7), same interface
+  subroutine GPU_ADV_PARTICLE_ELLP(NCYCLE)
+    integer, intent(in) :: NCYCLE
+    integer :: J
+    integer(c_int) :: rc
+    do J = 1, NIG
+      if (NPART_LOC(J) < 1) cycle
+      if (GPU_MIRROR) call GPU_PUT_PART(J)
+      rc = gp_adv_particle_ellp(GPU_CTX(J), int(NCYCLE, c_int))
+      call GPU_CHECK(rc, GPU_CTX(J), 'ADV_PARTICLE_ELLP')
+      if (GPU_MIRROR) call GPU_GET_PART(J)
+    end do
+    if (.not. GPU_MIRROR) GPU_DEVICE_AHEAD = .true.
+  end subroutine GPU_ADV_PARTICLE_ELLP
+
+  !! COLLISION (collision.f90:7) at NPROC = 1, same interface
+  subroutine GPU_COLLISION(NCYCLE, TIME)
+    integer, intent(in) :: NCYCLE
+    real(kind=8), intent(in) :: TIME
+    integer :: J
+    integer(c_int) :: rc
+    do J = 1, NIG
+      if (COLLIDEF(J) /= 1 .or. NPART_LOC(J) < 1) cycle
+      if (GPU_MIRROR) call GPU_PUT_PART(J)
+      rc = gp_collision(GPU_CTX(J), int(NCYCLE, c_int), real(TIME, c_double))
+      call GPU_CHECK(rc, GPU_CTX(J), 'COLLISION')
+      if (GPU_MIRROR) call GPU_GET_PART(J)
+    end do
+    if (.not. GPU_MIRROR) GPU_DEVICE_AHEAD = .true.
+  end subroutine GPU_COLLISION
+
+  !! INTERPH (interpH.f90:26), same interface; UAIN is passed to the GPU
+  subroutine GPU_INTERPH(INT_SCHEME, XAIN, YAIN, ZAIN, UAIN, NINT, XINT, YINT, ZINT, UINT)
+    integer, intent(in) :: INT_SCHEME
+    real(kind=8), dimension(ISTART(1):IEND(1)), intent(in) :: XAIN
+    real(kind=8), dimension(ISTART(2):IEND(2)), intent(in) :: YAIN
+    real(kind=8), dimension(ISTART(3):IEND(3)), intent(in) :: ZAIN
+    real(kind=8), dimension(ISTART(1):IEND(1), ISTART(2)-NGHTCELL:IEND(2)+NGHTCELL, &
+                           ISTART(3)-NGHTCELL:IEND(3)+NGHTCELL), intent(in) :: UAIN
+    integer, intent(in) :: NINT
+    real(kind=8), dimension(NINT), intent(in) :: XINT, YINT, ZINT
+    real(kind=8), dimension(NINT), intent(out) :: UINT
+    integer(c_int) :: rc
+    if (INT_SCHEME /= 3 .and. INT_SCHEME /= 4) then
+      stop 'GPU_INTERPH: only INT_SCHEME 3 and 4 run on the GPU'
+    end if
+    if (XAIN(ISTART(1)+1) - XAIN(ISTART(1)) /= DX) stop 'GPU_INTERPH: mesh is not XMESH'
+    rc = gp_interph_field(GPU_CTX(1), int(INT_SCHEME, c_int), UAIN, int(NINT, c_int), &
+                          XINT, YINT, ZINT, UINT)
+    call GPU_CHECK(rc, GPU_CTX(1), 'INTERPH')
+  end subroutine GPU_INTERPH
+
+end module gp_dropin

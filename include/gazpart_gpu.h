@@ -73,7 +73,15 @@ typedef struct {
                           of their largest semi-axes (flag is then false in
                           the reference too); 0: LM on every pair */
   int xcd_remap;       /* 1: XCD-aware block order for the gather */
-  int reserved[6];
+  int keep_fluid;      /* 1: keep UFAP..VORTZAP of the last advance
+                          (gp_get_fluid_at_particles) */
+  int timing;          /* 1: accumulate per-stage HIP-event times (gp_stats) */
+  int graph;           /* 1: gp_step replays a captured hipGraph of two full
+                          steps (no host synchronisation inside a step);
+                          buffers are sized from a first eager step with
+                          headroom, and an overflow is reported as
+                          GP_ERR_NCLOSE_OVERFLOW at the next gp_sync */
+  int reserved[3];
 } gp_params;
 
 typedef struct {
@@ -85,7 +93,7 @@ typedef struct {
   int64_t nwall;        /* wall bounces in the last advance */
   int64_t greedy_rounds;
   /* stage times accumulated with HIP events on the context's stream when
-   * gp_params.reserved[1] = 1 (0 otherwise): advance (gather + RK2 + walls
+   * gp_params.timing = 1 (0 otherwise): advance (gather + RK2 + walls
    * + wrap + binning), sort (scan, stable scatter, pair counts), contact
    * (prefilter + LM), greedy (one-collision pass + impulses) */
   double t_advance_ms, t_sort_ms, t_contact_ms, t_greedy_ms;
@@ -109,7 +117,7 @@ int gp_set_particles(gp_ctx *ctx, int np, const double *state, const int *specie
 /* device -> PART(1:np) in the order given to gp_set_particles           */
 int gp_get_particles(gp_ctx *ctx, int np, double *state);
 /* fluid velocity and vorticity at x^n of the last advance (UFAP..VORTZAP),
- * double[6][np]; requires gp_params.reserved[0] = 1 at create time      */
+ * double[6][np]; requires gp_params.keep_fluid = 1 at create time       */
 int gp_get_fluid_at_particles(gp_ctx *ctx, int np, double *fl);
 
 /* hot-path entry points ------------------------------------------------ */

@@ -242,6 +242,30 @@ def test_step_equals_adv_plus_collision_and_is_deterministic(c1):
     np.testing.assert_array_equal(sb, sc)
 
 
+@pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
+def test_graph_replays_equal_eager_steps(c1, prec):
+    """The captured two-step hipGraph replayed several times, with eager
+    work and synchronisations in between, gives the eager result bit for
+    bit after every replay."""
+    eager = G.Context.from_workload(c1, precision=prec)
+    graph = G.Context.from_workload(c1, precision=prec, graph=True)
+    for n in (3, 2, 4, 1, 2):
+        eager.step(n)
+        graph.step(n)
+        a, b = eager.get_particles(), graph.get_particles()
+        np.testing.assert_array_equal(a, b)
+        # eager calls on the graph context between replays
+        graph.collision_detection_connectivity()
+        graph.sync()
+    graph.adv_particle_ellp(1)
+    graph.collision(1)
+    eager.adv_particle_ellp(1)
+    eager.collision(1)
+    graph.step(4)
+    eager.step(4)
+    np.testing.assert_array_equal(eager.get_particles(), graph.get_particles())
+
+
 def test_error_nan_drag(c1):
     """u_p = u_f exactly gives Re_p = 0 and C_D = NaN: the reference stops
     (fluid_force_ellipsoid.f90:150-158); the library returns the code."""
