@@ -42,7 +42,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 # every symbol include/gazpart_gpu.h declares
 EXPORTS = (
     "gp_create", "gp_destroy", "gp_last_error_string", "gp_sync", "gp_bind_field",
-    "gp_set_species", "gp_set_particles", "gp_get_particles", "gp_get_fluid_at_particles",
+    "gp_set_species", "gp_set_particles", "gp_set_groups", "gp_get_particles", "gp_get_fluid_at_particles",
     "gp_interph", "gp_interph_field", "gp_adv_particle_ellp", "gp_collision_detection", "gp_contact_batch",
     "gp_collision", "gp_step", "gp_get_stats", "gp_reset_timers", "gp_collision_grid",
 )
@@ -96,6 +96,7 @@ def load_library(path: str = LIB_PATH):
     L.gp_bind_field.argtypes = [vp, dp]
     L.gp_set_species.argtypes = [vp, C.c_int, dp]
     L.gp_set_particles.argtypes = [vp, C.c_int, dp, ip]
+    L.gp_set_groups.argtypes = [vp, C.c_int, ip]
     L.gp_get_particles.argtypes = [vp, C.c_int, dp]
     L.gp_get_fluid_at_particles.argtypes = [vp, C.c_int, dp]
     L.gp_interph.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, C.c_int, dp]
@@ -181,6 +182,13 @@ class Context:
         assert st.shape[0] == 13 and sp.shape == (st.shape[1],)
         self._chk(self.lib.gp_set_particles(self.h, st.shape[1], _ptr(st), _ptr(sp)))
         self.np = st.shape[1]
+
+    def set_groups(self, group):
+        """Assign each particle (reference order) to an independent
+        ensemble; ensembles share the field but never collide."""
+        g = np.ascontiguousarray(group, dtype=np.int32)
+        assert g.shape == (self.np,)
+        self._chk(self.lib.gp_set_groups(self.h, self.np, _ptr(g)))
 
     def get_particles(self):
         out = np.empty((13, self.np))

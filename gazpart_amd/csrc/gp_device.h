@@ -70,48 +70,84 @@ struct Rot {
 template <typename T> __device__ __forceinline__ T dsqrt(T x);
 template <> __device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
-template <typename T> __device__ __forceinline__ T dpow(T x, T y);
-template <> __device__ __forceinline__ double dpow<double>(double x, double y) { return pow(x, y); }
-template <> __device__ __forceinline__ float dpow<float>(float x, float y) { return powf(x, y); }
-template <typename T> __device__ __forceinline__ T datan(T x);
-template <> __device__ __forceinline__ double datan<double>(double x) { return atan(x); }
-template <> __device__ __forceinline__ float datan<float>(float x) { return atanf(x); }
-template <typename T> __device__ __forceinline__ void dsincos(T x, T* s, T* c);
-template <> __device__ __forceinline__ void dsincos<double>(double x, double* s, double* c) { sincos(x, s, c); }
-template <> __device__ __forceinline__ void dsincos<float>(float x, float* s, float* c) { sincosf(x, s, c); }
+
+template <typename T> __device__ __forceinline__ T dlog(T x);
+template <> __device__ __forceinline__ double dlog<double>(double x) { return log(x); }
+template <> __device__ __forceinline__ float dlog<float>(float x) { return logf(x); }
+template <typename T> __device__ __forceinline__ T dexp(T x);
+template <> __device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float dexp<float>(float x) { return expf(x); }
+
+// ---------------------------------------------------------------------------
+// Zastawny correlations (fluid_force_ellipsoid.f90:48-56, 92-99 and
+// fluid_torque_ellipsoid.f90:64-82, 137-142).  Every power of a Reynolds
+// number is exp(a log Re) with log Re taken once; the incidence angle
+// phi = |atan(sqrt(v'^2 + w'^2) / u')| in [0, pi/2] enters only through
+// sin(phi) = sqrt(v'^2 + w'^2) / |u'| and cos(phi) = |u'_x| / |u'|.
+// ---------------------------------------------------------------------------
+template <typename CT>
+struct DragCoef {          // functions of the particle Reynolds number only
+  CT cd0, cd90;            // C_D at phi = 0 and 90 degrees
+  CT cl0, esl, ecl;        // lift: prefactor and the sin / cos exponents
+  __device__ __forceinline__ explicit DragCoef(CT rep) {
+    const CT l = dlog(rep);
+    cd0 = CT(18.12) * dexp(CT(-1.023) * l) + CT(4.26) * dexp(CT(-0.384) * l);
+    cd90 = CT(21.52) * dexp(CT(-0.990) * l) + CT(2.86) * dexp(CT(-0.260) * l);
+    cl0 = CT(0.083) * dexp(CT(0.21) * l) + CT(1.582) * dexp(CT(-0.851) * l);
+    esl = CT(1.842) + CT(-0.802) * dexp(CT(-0.006) * l);
+    ecl = CT(0.874) + CT(0.009) * dexp(CT(0.570) * l);
+  }
+};
+
+template <typename CT>
+struct PitchCoef {         // pitching-torque coefficient pieces, f(Re)
+  CT ct0, esp, ecp;
+  __device__ __forceinline__ explicit PitchCoef(CT rep) {
+    const CT l = dlog(rep);
+    ct0 = CT(0.935) * dexp(CT(-0.146) * l) + CT(-0.469) * dexp(CT(-0.145) * l);
+    esp = CT(0.116) + CT(0.748) * dexp(CT(0.041) * l);
+    ecp = CT(0.221) + CT(0.657) * dexp(CT(0.044) * l);
+  }
+};
+
+// relative velocity in the body frame and sin / cos of the incidence angle
+template <typename CT>
+struct Incidence {
+  CT u0, u1, u2, mag, syz, sphi, cphi;
+  __device__ __forceinline__ Incidence(const Rot<CT>& R, CT urx, CT ury, CT urz) {
+    R.mulT(urx, ury, urz, u0, u1, u2);
+    syz = dsqrt(u1 * u1 + u2 * u2);
+    mag = dsqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    sphi = syz / mag;
+    cphi = fabs(u0) / mag;
+  }
+};
+
+// x^e for x in [0, 1], e > 0 (0^e = 0)
+template <typename CT>
+__device__ __forceinline__ CT upow(CT lx, CT e) { return dexp(e * lx); }
 
 // ---------------------------------------------------------------------------
 // FLUID_FORCE_ELLIPSOID (fluid_force_ellipsoid.f90:1-161): drag + lift per
-// unit mass in the lab frame.  R = rotation of the particle's quaternion.
-// Returns false when C_D is NaN (the reference's `stop`, :150-158).
+// unit mass in the lab frame for relative velocity (urx, ury, urz), whose
+// Reynolds-number terms are in dc.  Returns false when C_D is NaN (the
+// reference's `stop`, :150-158).
 // ---------------------------------------------------------------------------
 template <typename CT>
-__device__ __forceinline__ bool fluid_force(const Rot<CT>& R, CT pmass, CT deq, CT visc,
-                                            CT rhof, CT urx, CT ury, CT urz, CT f[3]) {
-  const CT a0 = CT(1.95), a1 = CT(18.12), a2 = CT(1.023), a3 = CT(4.26), a4 = CT(0.384),
-           a5 = CT(21.52), a6 = CT(0.990), a7 = CT(2.86), a8 = CT(0.260);
-  const CT b1 = CT(0.083), b2 = CT(-0.21), b3 = CT(1.582), b4 = CT(0.851), b5 = CT(1.842),
-           b6 = CT(-0.802), b7 = CT(-0.006), b8 = CT(0.874), b9 = CT(0.009), b10 = CT(0.570);
-  CT mag = dsqrt(urx * urx + ury * ury + urz * urz);
-  CT rep = (mag * deq) / visc;
-  CT u0, u1, u2;                       // relative velocity in the body frame
-  R.mulT(urx, ury, urz, u0, u1, u2);
-  CT syz = dsqrt(u1 * u1 + u2 * u2);
-  CT phi = fabs(datan(syz / u0));
-  CT sphi, cphi;
-  dsincos(phi, &sphi, &cphi);
-  CT cd0 = a1 / dpow(rep, a2) + a3 / dpow(rep, a4);
-  CT cd90 = a5 / dpow(rep, a6) + a7 / dpow(rep, a8);
-  CT cd = cd0 + (cd90 - cd0) * dpow(sphi, a0);
-  CT cl = (b1 / dpow(rep, b2) + b3 / dpow(rep, b4)) * dpow(sphi, b5 + b6 * dpow(rep, b7)) *
-          dpow(cphi, b8 + b9 * dpow(rep, b10));
-  CT pre = CT(0.125) * (rhof / pmass) * (CT(PI) * (deq * deq));
-  CT kd = pre * cd * mag;             // |u_rel| is invariant under rotation
-  CT kl = pre * cl * (mag * mag);
-  CT sg = copysign(CT(1), -u0);
-  CT fx = kd * u0 + kl * sphi * sg;
-  CT fy = kd * u1 + kl * cphi * (u1 / syz);
-  CT fz = kd * u2 + kl * cphi * (u2 / syz);
+__device__ __forceinline__ bool fluid_force(const Rot<CT>& R, CT pmass, CT deq, CT rhof,
+                                            const DragCoef<CT>& dc, CT urx, CT ury, CT urz,
+                                            CT f[3]) {
+  const Incidence<CT> in(R, urx, ury, urz);
+  const CT ls = dlog(in.sphi), lc = dlog(in.cphi);
+  const CT cd = dc.cd0 + (dc.cd90 - dc.cd0) * upow(ls, CT(1.95));
+  const CT cl = dc.cl0 * upow(ls, dc.esl) * upow(lc, dc.ecl);
+  const CT pre = CT(0.125) * (rhof / pmass) * (CT(PI) * (deq * deq));
+  const CT kd = pre * cd * in.mag;
+  const CT kl = pre * cl * (in.mag * in.mag);
+  const CT sg = copysign(CT(1), -in.u0);
+  const CT fx = kd * in.u0 + kl * in.sphi * sg;
+  const CT fy = kd * in.u1 + kl * in.cphi * (in.u1 / in.syz);
+  const CT fz = kd * in.u2 + kl * in.cphi * (in.u2 / in.syz);
   R.mul(fx, fy, fz, f[0], f[1], f[2]);
   return cd == cd;
 }
@@ -119,42 +155,34 @@ __device__ __forceinline__ bool fluid_force(const Rot<CT>& R, CT pmass, CT deq, 
 // ---------------------------------------------------------------------------
 // FLUID_TORQUE_ELLIPSOID (fluid_torque_ellipsoid.f90:1-190): pitching +
 // rotational torque per unit mass, body frame.  The relative velocity is the
-// stage-n one (PART%UFAP - PART%UP, :88-105) in every RK stage; the curl is
-// the interpolated vorticity.
+// stage-n one (PART%UFAP - PART%UP, :88-105) in every RK stage, so its
+// Reynolds-number terms pc are shared by both stages; the curl is the
+// interpolated vorticity.
 // ---------------------------------------------------------------------------
 template <typename CT>
 __device__ __forceinline__ void fluid_torque(const Rot<CT>& R, CT pmass, CT deq, CT visc,
-                                             CT rhof, CT urx, CT ury, CT urz, CT wx, CT wy,
-                                             CT wz, CT ox, CT oy, CT oz, CT t[3]) {
-  const CT c1 = CT(0.935), c2 = CT(0.146), c3 = CT(-0.469), c4 = CT(0.145), c5 = CT(0.116),
-           c6 = CT(0.748), c7 = CT(0.041), c8 = CT(0.221), c9 = CT(0.657), c10 = CT(0.044);
-  const CT r1 = CT(0.573), r2 = CT(-0.154), r3 = CT(116.61);
-  const CT r1_ = CT(1.244), r2_ = CT(0.239), r3_ = CT(378.12), r4_ = CT(0.789);
-  CT mag = dsqrt(urx * urx + ury * ury + urz * urz);
-  CT rep = (mag * deq) / visc;
-  CT u0, u1, u2;
-  R.mulT(urx, ury, urz, u0, u1, u2);
-  CT syz = dsqrt(u1 * u1 + u2 * u2);
-  CT phi = fabs(datan(syz / u0));
-  CT sphi, cphi;
-  dsincos(phi, &sphi, &cphi);
+                                             CT rhof, const PitchCoef<CT>& pc, CT urx, CT ury,
+                                             CT urz, CT wx, CT wy, CT wz, CT ox, CT oy, CT oz,
+                                             CT t[3]) {
+  const Incidence<CT> in(R, urx, ury, urz);
+  const CT ctp = pc.ct0 * upow(dlog(in.sphi), pc.esp) * upow(dlog(in.cphi), pc.ecp);
   CT cx, cy, cz;
   R.mulT(wx, wy, wz, cx, cy, cz);
-  CT o0 = CT(0.5) * cx - ox, o1 = CT(0.5) * cy - oy, o2 = CT(0.5) * cz - oz;
-  CT omag = dsqrt(o0 * o0 + o1 * o1 + o2 * o2);
-  CT oyz = dsqrt(o2 * o2 + o1 * o1);
-  CT rer = (omag * (deq * deq)) / visc;
-  CT ctp = (c1 / dpow(rep, c2) + c3 / dpow(rep, c4)) * dpow(sphi, c5 + c6 * dpow(rep, c7)) *
-           dpow(cphi, c8 + c9 * dpow(rep, c10));
-  CT ctr = r1 * dpow(rer, r2) + r3 / rer;                 // r4 = 1
-  CT ctr2 = r1_ * dpow(rer, r2_) + r3_ / dpow(rer, r4_);
-  CT pre = CT(0.0625) * (rhof / pmass) * CT(PI) * (deq * deq * deq) * ctp * (mag * mag);
-  CT h = deq * CT(0.5);
-  CT h5 = h * h * h * h * h;
-  CT rr = (CT(0.5) * rhof / pmass) * h5;
+  const CT o0 = CT(0.5) * cx - ox, o1 = CT(0.5) * cy - oy, o2 = CT(0.5) * cz - oz;
+  const CT omag = dsqrt(o0 * o0 + o1 * o1 + o2 * o2);
+  const CT oyz = dsqrt(o2 * o2 + o1 * o1);
+  const CT rer = (omag * (deq * deq)) / visc;
+  const CT lr = dlog(rer);
+  const CT ctr = CT(0.573) * dexp(CT(-0.154) * lr) + CT(116.61) / rer;      // r4 = 1
+  const CT ctr2 = CT(1.244) * dexp(CT(0.239) * lr) + CT(378.12) * dexp(CT(-0.789) * lr);
+  const CT pre = CT(0.0625) * (rhof / pmass) * CT(PI) * (deq * deq * deq) * ctp *
+                 (in.mag * in.mag);
+  const CT h = deq * CT(0.5);
+  const CT h5 = h * h * h * h * h;
+  const CT rr = (CT(0.5) * rhof / pmass) * h5;
   t[0] = ctr * rr * fabs(o0) * o0;
-  t[1] = pre * (fabs(u2) / syz) * copysign(CT(1), u0 * u2) + ctr2 * rr * oyz * o1;
-  t[2] = pre * (fabs(u1) / syz) * copysign(CT(1), u0 * u1) + ctr2 * rr * oyz * o2;
+  t[1] = pre * (fabs(in.u2) / in.syz) * copysign(CT(1), in.u0 * in.u2) + ctr2 * rr * oyz * o1;
+  t[2] = pre * (fabs(in.u1) / in.syz) * copysign(CT(1), in.u0 * in.u1) + ctr2 * rr * oyz * o2;
 }
 
 // ---------------------------------------------------------------------------
@@ -344,13 +372,17 @@ __device__ __forceinline__ unsigned advance_particle(const StepParams& P, const 
   const double ox = s[10], oy = s[11], oz = s[12];
   const CT ufx = fl[0], ufy = fl[1], ufz = fl[2];
 
-  // stage 1 at (x^n, u^n, omega^n, q^n)
+  // stage 1 at (x^n, u^n, omega^n, q^n); the stage-n relative velocity and
+  // its Reynolds number also serve the stage-2 torque
   CT f[3], t[3];
+  const CT unx = ufx - CT(up), uny = ufy - CT(vp), unz = ufz - CT(wp);
+  const CT ren = (dsqrt(unx * unx + uny * uny + unz * unz) * dq) / vs;
+  const PitchCoef<CT> pcn(ren);
   {
     Rot<CT> R{CT(q[0]), CT(q[1]), CT(q[2]), CT(q[3])};
-    CT urx = ufx - CT(up), ury = ufy - CT(vp), urz = ufz - CT(wp);
-    if (!fluid_force(R, pm, dq, vs, rf, urx, ury, urz, f)) err |= EB_NAN_DRAG;
-    fluid_torque(R, pm, dq, vs, rf, urx, ury, urz, fl[3], fl[4], fl[5], CT(ox), CT(oy),
+    const DragCoef<CT> dcn(ren);
+    if (!fluid_force(R, pm, dq, rf, dcn, unx, uny, unz, f)) err |= EB_NAN_DRAG;
+    fluid_torque(R, pm, dq, vs, rf, pcn, unx, uny, unz, fl[3], fl[4], fl[5], CT(ox), CT(oy),
                  CT(oz), t);
   }
   if (!(ox == ox)) err |= EB_NAN_OMEGA;
@@ -363,6 +395,16 @@ __device__ __forceinline__ unsigned advance_particle(const StepParams& P, const 
   double o1[3] = {ox + dt * k1o[0], oy + dt * k1o[1], oz + dt * k1o[2]};
   double q1[4] = {q[0], q[1], q[2], q[3]};
   quat_integrate(q1, o1[0], o1[1], o1[2], dt);
+
+  // stage-2 force (at u1, q1) and torque (stage-n velocity, omega1, q1)
+  auto stage2 = [&](const double* qq, const double* uu, const double* oo, CT* ff, CT* tt) {
+    Rot<CT> R{CT(qq[0]), CT(qq[1]), CT(qq[2]), CT(qq[3])};
+    const CT urx = ufx - CT(uu[0]), ury = ufy - CT(uu[1]), urz = ufz - CT(uu[2]);
+    const DragCoef<CT> dc((dsqrt(urx * urx + ury * ury + urz * urz) * dq) / vs);
+    if (!fluid_force(R, pm, dq, rf, dc, urx, ury, urz, ff)) err |= EB_NAN_DRAG;
+    fluid_torque(R, pm, dq, vs, rf, pcn, unx, uny, unz, fl[3], fl[4], fl[5], CT(oo[0]),
+                 CT(oo[1]), CT(oo[2]), tt);
+  };
 
   WallHit w = wall_test(P.L, sp.a, sp.b, sp.c, xp1, q1);
   *wall = 0;
@@ -387,13 +429,7 @@ __device__ __forceinline__ unsigned advance_particle(const StepParams& P, const 
     q1[0] = q[0]; q1[1] = q[1]; q1[2] = q[2]; q1[3] = q[3];
     quat_integrate(q1, o1[0], o1[1], o1[2], tpart);
     if (!wall_rebound(q1, ixx, iyy, izz, P.ecw, P.muw, u1, o1, w.arm, w.nx)) err |= EB_KE_WALL;
-    {
-      Rot<CT> R{CT(q1[0]), CT(q1[1]), CT(q1[2]), CT(q1[3])};
-      CT urx = ufx - CT(u1[0]), ury = ufy - CT(u1[1]), urz = ufz - CT(u1[2]);
-      if (!fluid_force(R, pm, dq, vs, rf, urx, ury, urz, f)) err |= EB_NAN_DRAG;
-      fluid_torque(R, pm, dq, vs, rf, ufx - CT(up), ufy - CT(vp), ufz - CT(wp), fl[3], fl[4],
-                   fl[5], CT(o1[0]), CT(o1[1]), CT(o1[2]), t);
-    }
+    stage2(q1, u1, o1, f, t);
     const double k2u[3] = {double(f[0]), double(f[1]), double(f[2]) + sp.grav};
     const double k2o[3] = {(double(t[0]) + o1[1] * o1[2] * (iyy - izz)) / ixx,
                            (double(t[1]) + o1[2] * o1[0] * (izz - ixx)) / iyy,
@@ -410,13 +446,7 @@ __device__ __forceinline__ unsigned advance_particle(const StepParams& P, const 
     quat_integrate(q1, s[10], s[11], s[12], delt);
   } else {
     // unbounced particle (adv_particle_new.f90:514-640)
-    {
-      Rot<CT> R{CT(q1[0]), CT(q1[1]), CT(q1[2]), CT(q1[3])};
-      CT urx = ufx - CT(u1[0]), ury = ufy - CT(u1[1]), urz = ufz - CT(u1[2]);
-      if (!fluid_force(R, pm, dq, vs, rf, urx, ury, urz, f)) err |= EB_NAN_DRAG;
-      fluid_torque(R, pm, dq, vs, rf, ufx - CT(up), ufy - CT(vp), ufz - CT(wp), fl[3], fl[4],
-                   fl[5], CT(o1[0]), CT(o1[1]), CT(o1[2]), t);
-    }
+    stage2(q1, u1, o1, f, t);
     const double k2u[3] = {double(f[0]), double(f[1]), double(f[2]) + sp.grav};
     const double k2o[3] = {(double(t[0]) + o1[1] * o1[2] * (iyy - izz)) / ixx,
                            (double(t[1]) + o1[2] * o1[0] * (izz - ixx)) / iyy,

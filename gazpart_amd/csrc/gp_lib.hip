@@ -49,6 +49,7 @@ struct PState {
   VT *ox, *oy, *oz;
   int *sp;   // species row
   int *id;   // reference index (0-based)
+  int *grp;  // ensemble: particles of different ensembles never collide
 };
 
 struct CellGrid {
@@ -355,6 +356,7 @@ __global__ __launch_bounds__(BLOCK) void k_advance(
   store_state(dst, i, st);
   dst.sp[i] = sid;
   dst.id[i] = pid;
+  dst.grp[i] = src.grp[s];
   if (cell) {
     bool oob;
     int c = cell_of(g, st[0], st[1], st[2], &oob);
@@ -390,12 +392,14 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(const int* __restrict__ cell,
   if (c >= 0) tmp[off[c] + slot[i]] = i;
 }
 
-// stable order inside each cell: rank by reference index
+// stable order inside each cell: by ensemble, then reference index (the
+// reference bins particles in index order); grp == nullptr: one ensemble
 __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
                                                 const int* __restrict__ off,
                                                 const int* __restrict__ cnt,
                                                 const int* __restrict__ tmp,
                                                 const int* __restrict__ id,
+                                                const int* __restrict__ grp,
                                                 int* __restrict__ sorted, int np) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
@@ -403,8 +407,28 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
   if (c < 0) return;
   const int o = off[c], m = cnt[c], me = id[i];
   int r = 0;
-  for (int t = 0; t < m; ++t) r += (id[tmp[o + t]] < me) ? 1 : 0;
+  if (grp) {
+    const int g = grp[i];
+    for (int t = 0; t < m; ++t) {
+      const int j = tmp[o + t], gj = grp[j];
+      r += (gj < g || (gj == g && id[j] < me)) ? 1 : 0;
+    }
+  } else {
+    for (int t = 0; t < m; ++t) r += (id[tmp[o + t]] < me) ? 1 : 0;
+  }
   sorted[o + r] = i;
+}
+
+// members of cell range [o, o+m) (sorted by ensemble) in ensemble g
+__device__ __forceinline__ void group_range(const int* __restrict__ sorted,
+                                            const int* __restrict__ grp, int o, int m, int g,
+                                            int* lo, int* hi) {
+  int a = o, b = o + m;
+  while (a < b && grp[sorted[a]] < g) ++a;
+  int e = a;
+  while (e < b && grp[sorted[e]] == g) ++e;
+  *lo = a;
+  *hi = e;
 }
 
 // candidate pairs emitted for the home particle at sorted position p
@@ -412,15 +436,29 @@ __global__ __launch_bounds__(BLOCK) void k_pair_count(CellGrid g, const int* __r
                                                       const int* __restrict__ cell,
                                                       const int* __restrict__ off,
                                                       const int* __restrict__ cnt,
+                                                      const int* __restrict__ grp,
                                                       int* __restrict__ pc, int nsorted) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nsorted) return;
-  const int c = cell[sorted[p]];
+  const int hi = sorted[p];
+  const int c = cell[hi];
   int nb[13];
   half_shell(g, c, nb);
-  int k = cnt[c] - 1 - (p - off[c]);
+  int k;
+  if (!grp) {
+    k = cnt[c] - 1 - (p - off[c]);
 #pragma unroll
-  for (int v = 0; v < 13; ++v) k += cnt[nb[v]];
+    for (int v = 0; v < 13; ++v) k += cnt[nb[v]];
+  } else {
+    const int gr = grp[hi];
+    int lo, up;
+    group_range(sorted, grp, off[c], cnt[c], gr, &lo, &up);
+    k = up - 1 - p;
+    for (int v = 0; v < 13; ++v) {
+      group_range(sorted, grp, off[nb[v]], cnt[nb[v]], gr, &lo, &up);
+      k += up - lo;
+    }
+  }
   pc[p] = k;
 }
 
@@ -431,6 +469,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair_emit(CellGrid g, const int* __re
                                                      const int* __restrict__ cnt,
                                                      const long long* __restrict__ poff,
                                                      const int* __restrict__ id,
+                                                     const int* __restrict__ grp,
                                                      int* __restrict__ home, int* __restrict__ nr,
                                                      int nsorted) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -439,13 +478,17 @@ __global__ __launch_bounds__(BLOCK) void k_pair_emit(CellGrid g, const int* __re
   const int c = cell[hi];
   const int h = id[hi] + 1;
   long long k = poff[p];
-  const int o = off[c], m = cnt[c];
-  for (int t = p + 1; t < o + m; ++t) { home[k] = h; nr[k] = id[sorted[t]] + 1; ++k; }
+  int lo = off[c], up = off[c] + cnt[c];
+  const int gr = grp ? grp[hi] : 0;
+  if (grp) group_range(sorted, grp, off[c], cnt[c], gr, &lo, &up);
+  for (int t = p + 1; t < up; ++t) { home[k] = h; nr[k] = id[sorted[t]] + 1; ++k; }
   int nb[13];
   half_shell(g, c, nb);
   for (int v = 0; v < 13; ++v) {
-    const int o2 = off[nb[v]], m2 = cnt[nb[v]];
-    for (int t = 0; t < m2; ++t) { home[k] = h; nr[k] = id[sorted[o2 + t]] + 1; ++k; }
+    lo = off[nb[v]];
+    up = lo + cnt[nb[v]];
+    if (grp) group_range(sorted, grp, off[nb[v]], cnt[nb[v]], gr, &lo, &up);
+    for (int t = lo; t < up; ++t) { home[k] = h; nr[k] = id[sorted[t]] + 1; ++k; }
   }
 }
 
@@ -468,8 +511,9 @@ template <typename VT>
 __global__ __launch_bounds__(BLOCK) void k_prefilter(
     CellGrid g, PState<VT> p, const Species* __restrict__ spec, const int* __restrict__ sorted,
     const int* __restrict__ cell, const int* __restrict__ off, const int* __restrict__ cnt,
-    const long long* __restrict__ poff, double L, int use_filter, WorkRec* __restrict__ work,
-    unsigned long long* nwork, unsigned long long cap, unsigned* err, int nsorted) {
+    const long long* __restrict__ poff, const int* __restrict__ grp, double L, int use_filter,
+    WorkRec* __restrict__ work, unsigned long long* nwork, unsigned long long cap, unsigned* err,
+    int nsorted) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nsorted) return;
   const int hi = sorted[q];
@@ -493,13 +537,17 @@ __global__ __launch_bounds__(BLOCK) void k_prefilter(
     }
     ++k;
   };
-  const int o = off[c], m = cnt[c];
-  for (int t = q + 1; t < o + m; ++t) test(sorted[t]);
+  int lo = off[c], up = off[c] + cnt[c];
+  const int gr = grp ? grp[hi] : 0;
+  if (grp) group_range(sorted, grp, off[c], cnt[c], gr, &lo, &up);
+  for (int t = q + 1; t < up; ++t) test(sorted[t]);
   int nb[13];
   half_shell(g, c, nb);
   for (int v = 0; v < 13; ++v) {
-    const int o2 = off[nb[v]], m2 = cnt[nb[v]];
-    for (int t = 0; t < m2; ++t) test(sorted[o2 + t]);
+    lo = off[nb[v]];
+    up = lo + cnt[nb[v]];
+    if (grp) group_range(sorted, grp, off[nb[v]], cnt[nb[v]], gr, &lo, &up);
+    for (int t = lo; t < up; ++t) test(sorted[t]);
   }
 }
 
@@ -680,6 +728,12 @@ __global__ __launch_bounds__(BLOCK) void k_contact_batch(
 }
 
 template <typename VT>
+__global__ void k_set_groups(PState<VT> p, const int* __restrict__ g, int np) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) p.grp[i] = g[p.id[i]];
+}
+
+template <typename VT>
 __global__ void k_pos_of_id(PState<VT> p, int* pos, int np) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < np) pos[p.id[i]] = i;
@@ -704,6 +758,7 @@ __global__ void k_scatter_in(PState<VT> p, const double* in, const int* sp, int 
   store_state(p, i, st);
   p.sp[i] = sp[i];
   p.id[i] = i;
+  p.grp[i] = 0;
 }
 
 }  // namespace gp
@@ -737,6 +792,7 @@ struct gp_ctx {
   void* pbuf[2] = {nullptr, nullptr};
   int cur = 0;
   int* perm = nullptr;          // order in which the next advance reads cur
+  int ngroups = 1;              // ensembles sharing the context
   bool perm_identity = true;
   // collision grid
   CellGrid g{};
@@ -814,13 +870,13 @@ static PState<VT> pstate(gp_ctx* ctx, int which) {
   VT* v = (VT*)(d + 7 * n);
   s.u = v; s.v = v + n; s.w = v + 2 * n; s.ox = v + 3 * n; s.oy = v + 4 * n; s.oz = v + 5 * n;
   int* ip = (int*)(v + 6 * n);
-  s.sp = ip; s.id = ip + n;
+  s.sp = ip; s.id = ip + n; s.grp = ip + 2 * n;
   return s;
 }
 
 static size_t pstate_bytes(int np, bool mixed) {
   const size_t n = (size_t)np;
-  return n * 7 * 8 + n * 6 * (mixed ? 4 : 8) + n * 2 * 4 + 256;
+  return n * 7 * 8 + n * 6 * (mixed ? 4 : 8) + n * 3 * 4 + 256;
 }
 
 template <typename T>
@@ -1102,6 +1158,36 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   hipFree(dsp);
   ctx->perm_identity = true;
   ctx->cells_valid = false;
+  ctx->ngroups = 1;
+  return GP_OK;
+}
+
+int gp_set_groups(gp_ctx* ctx, int np, const int* group) {
+  if (!ctx || !group || np != ctx->np) return GP_ERR_ARG;
+  int gmax = 0;
+  for (int i = 0; i < np; ++i) {
+    if (group[i] < 0) {
+      ctx->err = "negative ensemble index at particle " + std::to_string(i);
+      return GP_ERR_ARG;
+    }
+    gmax = std::max(gmax, group[i]);
+  }
+  hipStreamSynchronize(ctx->st);
+  drop_graph(ctx);
+  int* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, (size_t)np * sizeof(int)));
+  HIPCHK(hipMemcpyAsync(d, group, (size_t)np * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_set_groups<float>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st,
+                       pstate<float>(ctx, ctx->cur), d, np);
+  else
+    hipLaunchKernelGGL(k_set_groups<double>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st,
+                       pstate<double>(ctx, ctx->cur), d, np);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  hipFree(d);
+  ctx->ngroups = gmax + 1;
+  ctx->cells_valid = false;
   return GP_OK;
 }
 
@@ -1295,10 +1381,11 @@ static int launch_sort(gp_ctx* ctx) {
   auto ps = pstate<VT>(ctx, ctx->cur);
   hipLaunchKernelGGL(k_scatter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->slot,
                      ctx->off, ctx->tmp, np);
+  const int* grp = ctx->ngroups > 1 ? ps.grp : nullptr;
   hipLaunchKernelGGL(k_rank, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->off,
-                     ctx->cnt, ctx->tmp, ps.id, ctx->sorted, np);
+                     ctx->cnt, ctx->tmp, ps.id, grp, ctx->sorted, np);
   hipLaunchKernelGGL(k_pair_count, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, g, ctx->sorted,
-                     ctx->cell, ctx->off, ctx->cnt, ctx->pcount, np);
+                     ctx->cell, ctx->off, ctx->cnt, grp, ctx->pcount, np);
   scan_exclusive<int, long long>(ctx->pcount, ctx->poff, np, (long long*)ctx->scan_scr, ctx->st);
   HIPCHK(hipGetLastError());
   mark(ctx, 2);
@@ -1348,8 +1435,9 @@ static int launch_contacts(gp_ctx* ctx) {
   hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(8), 0, ctx->st, c + 1, 0ull, 6ll);
   auto ps = pstate<VT>(ctx, ctx->cur);
   hipLaunchKernelGGL(k_prefilter<VT>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->g, ps,
-                     ctx->spec, ctx->sorted, ctx->cell, ctx->off, ctx->cnt, ctx->poff, ctx->prm.L,
-                     ctx->prm.prefilter, ctx->work, c + 1, ctx->work_cap, ctx->d_err, np);
+                     ctx->spec, ctx->sorted, ctx->cell, ctx->off, ctx->cnt, ctx->poff,
+                     ctx->ngroups > 1 ? ps.grp : nullptr, ctx->prm.L, ctx->prm.prefilter,
+                     ctx->work, c + 1, ctx->work_cap, ctx->d_err, np);
   const unsigned grid = 2048;
   hipLaunchKernelGGL(k_contact<VT>, dim3(grid), dim3(BLOCK), 0, ctx->st, ps, ctx->spec, ctx->work,
                      c + 1, ctx->work_cap, ctx->prm.L, ctx->act, c + 2, ctx->act_cap, c + 3,
@@ -1500,8 +1588,11 @@ int gp_collision_detection(gp_ctx* ctx, int64_t nclosemax, int* home, int* near_
   HIPCHK(hipMalloc((void**)&dh, (size_t)total * sizeof(int)));
   HIPCHK(hipMalloc((void**)&dn, (size_t)total * sizeof(int)));
   const int* id = ctx->mixed ? pstate<float>(ctx, ctx->cur).id : pstate<double>(ctx, ctx->cur).id;
+  const int* gp =
+      ctx->mixed ? pstate<float>(ctx, ctx->cur).grp : pstate<double>(ctx, ctx->cur).grp;
   hipLaunchKernelGGL(k_pair_emit, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st, ctx->g,
-                     ctx->sorted, ctx->cell, ctx->off, ctx->cnt, ctx->poff, id, dh, dn, ctx->np);
+                     ctx->sorted, ctx->cell, ctx->off, ctx->cnt, ctx->poff, id,
+                     ctx->ngroups > 1 ? gp : nullptr, dh, dn, ctx->np);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(home, dh, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipMemcpyAsync(near_, dn, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, ctx->st));

@@ -61,6 +61,12 @@ module gazpart_gpu
       real(c_double), intent(in) :: state(*)
       integer(c_int), intent(in) :: species(*)
     end function
+    integer(c_int) function gp_set_groups(ctx, np, group) bind(C, name='gp_set_groups')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: np
+      integer(c_int), intent(in) :: group(*)
+    end function
     integer(c_int) function gp_get_particles(ctx, np, state) bind(C, name='gp_get_particles')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -273,6 +273,61 @@ def build_workload(name: str, n: int, np_: int, *, k0: float = 4.0, seed: int = 
                     meta=dict(tau_flow=tau_flow, cfl=cfl, phi=phi))
 
 
+def build_ensembles(name: str, n: int, counts, *, k0: float = 4.0, seed: int = 77,
+                    scheme: int = 3, steps: int = 100, visc: float = 0.01, rhof: float = 1.0,
+                    cfl: float = 0.1, req: float = 3.9e-3, apr=(0.2, 5.0), st=(0.1, 10.0),
+                    field_dtype=np.float64, field=None) -> Workload:
+    """Independent ensembles on one shared field (no reference counterpart:
+    the reference runs one ensemble per process).  Ensemble e has counts[e]
+    spheroids with its own aspect ratio and Stokes number (log-uniform in
+    the given ranges, one draw per ensemble) and volume-equivalent radius
+    req; Workload.meta["group"] holds each particle's ensemble and
+    meta["ranges"] the particle index range of each ensemble."""
+    L = TWOPI
+    dx = L / (n - 1)
+    if field is None:
+        fld, umax = random_fourier_field(n, k0, seed, dtype=field_dtype)
+    else:
+        fld, umax = field
+    dtime = cfl * dx / umax
+    tau_flow = 1.0 / k0
+    g = rng(seed + 5)
+    ne = len(counts)
+    aprs = np.exp(g.uniform(math.log(apr[0]), math.log(apr[1]), ne))
+    sts = np.exp(g.uniform(math.log(st[0]), math.log(st[1]), ne))
+    emaj = req * aprs ** (2.0 / 3.0)
+    deq = 2.0 * emaj / aprs ** (2.0 / 3.0)
+    spec = species_spheroid(emaj, aprs, rhop_for_stokes(sts, deq, tau_flow, rhof, visc))
+    np_ = int(sum(counts))
+    state = np.empty((NSTATE, np_))
+    sid = np.empty(np_, dtype=np.int32)
+    grp = np.empty(np_, dtype=np.int32)
+    ranges = []
+    o = 0
+    for e, c in enumerate(counts):
+        amax = max(emaj[e], emaj[e] / aprs[e])
+        state[:, o:o + c] = random_particles(int(c), L, amax, seed + 100 + e)
+        sid[o:o + c] = e
+        grp[o:o + c] = e
+        ranges.append((o, o + int(c)))
+        o += int(c)
+    return Workload(name=name, n=n, L=L, k0=k0, seed=seed, np_=np_, scheme=scheme, steps=steps,
+                    visc=visc, rhof=rhof, dtime=dtime, field=fld, umax=umax,
+                    species=np.atleast_2d(spec), species_id=sid, state=state,
+                    dpmax=2.0 * float(emaj.max()),
+                    meta=dict(group=grp, ranges=ranges, apr=aprs, st=sts, tau_flow=tau_flow))
+
+
+def c4(n: int = 256, nens: int = 64, steps: int = 100, seed: int = 4096,
+       field_dtype=np.float64) -> Workload:
+    """Batching: nens independent ensembles on one shared 256^3 field,
+    particle counts log-uniform in [1e3, 5e5] (seeded), per-ensemble St in
+    [0.1, 10] and aspect ratio in [0.2, 5]."""
+    g = rng(seed)
+    counts = np.exp(g.uniform(math.log(1e3), math.log(5e5), nens)).astype(int)
+    return build_ensembles("C4", n, counts, seed=seed, steps=steps, field_dtype=field_dtype)
+
+
 def c1(n: int = 64, np_: int = 10_000, steps: int = 100, seed: int = 1234,
        scheme: int = 3) -> Workload:
     """Plain FP64, CPU-runnable: periodic 2*pi box at 64^3, k0 = 4, 10,000

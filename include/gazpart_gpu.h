@@ -114,6 +114,12 @@ int gp_bind_field(gp_ctx *ctx, const double *field);
 int gp_set_species(gp_ctx *ctx, int nspec, const double *rows);
 /* PART(1:np) -> device; species[i] in [0, nspec)                       */
 int gp_set_particles(gp_ctx *ctx, int np, const double *state, const int *species);
+/* ensembles: group[i] >= 0 (reference order) assigns each particle to an
+ * independent ensemble sharing the context's field; particles of different
+ * ensembles never form candidate pairs, and within an ensemble the pair
+ * list keeps the reference order (no counterpart in the reference, which
+ * runs one ensemble per process; gp_set_particles resets to one ensemble) */
+int gp_set_groups(gp_ctx *ctx, int np, const int *group);
 /* device -> PART(1:np) in the order given to gp_set_particles           */
 int gp_get_particles(gp_ctx *ctx, int np, double *state);
 /* fluid velocity and vorticity at x^n of the last advance (UFAP..VORTZAP),

@@ -266,6 +266,38 @@ def test_graph_replays_equal_eager_steps(c1, prec):
     np.testing.assert_array_equal(eager.get_particles(), graph.get_particles())
 
 
+@pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
+def test_ensembles_share_a_field_and_never_collide(prec):
+    """Three ensembles (different aspect ratios, sizes and counts) in one
+    context give, particle for particle, the bits of three separate
+    contexts; each matches its own oracle run."""
+    wl = W.build_ensembles("E3", 32, [700, 1500, 400], k0=3.0, seed=5, req=0.05)
+    grp = wl.meta["group"]
+    both = G.Context.from_workload(wl, precision=prec, graph=True)
+    both.set_groups(grp)
+    both.step(8)
+    got = both.get_particles()
+    ncoll = 0
+    for e, (a, b) in enumerate(wl.meta["ranges"]):
+        sub = W.Workload(name="e", n=wl.n, L=wl.L, k0=wl.k0, seed=0, np_=b - a, scheme=wl.scheme,
+                         steps=0, visc=wl.visc, rhof=wl.rhof, dtime=wl.dtime, field=wl.field,
+                         umax=wl.umax, species=wl.species, species_id=wl.species_id[a:b],
+                         state=wl.state[:, a:b], dpmax=wl.dpmax)
+        one = G.Context.from_workload(sub, precision=prec)
+        one.step(8)
+        ncoll += one.stats()["ncollision"]
+        np.testing.assert_array_equal(got[:, a:b], one.get_particles())
+        if prec == G.PREC_FP64:
+            stp = O.Stepper(sub)
+            so = sub.state.copy()
+            for _ in range(8):
+                so, err, _, _ = stp.step(so)
+                assert err == 0
+            for name, sl in QUANT.items():
+                assert relerr(got[:, a:b], so, sl) <= 1e-9, (e, name)
+    assert ncoll > 0
+
+
 def test_error_nan_drag(c1):
     """u_p = u_f exactly gives Re_p = 0 and C_D = NaN: the reference stops
     (fluid_force_ellipsoid.f90:150-158); the library returns the code."""
