@@ -70,7 +70,8 @@ class Stats(C.Structure):
                 ("ncollision", C.c_int64), ("lm_nfev", C.c_int64), ("nwall", C.c_int64),
                 ("greedy_rounds", C.c_int64), ("t_advance_ms", C.c_double),
                 ("t_sort_ms", C.c_double), ("t_contact_ms", C.c_double),
-                ("t_greedy_ms", C.c_double), ("ntimed", C.c_int64), ("reserved", C.c_int64 * 4)]
+                ("t_greedy_ms", C.c_double), ("ntimed", C.c_int64), ("nlm", C.c_int64),
+                ("reserved", C.c_int64 * 3)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -122,7 +123,7 @@ class Context:
     the GPU, plus the reference-shaped entry points of the step."""
 
     def __init__(self, n, L, visc, rhof, dtime, *, scheme=3, precision=PREC_FP64, ecw=1.0,
-                 muw=0.0, dpmax=0.0, prefilter=1, xcd_remap=1, renorm_quat=0,
+                 muw=0.0, dpmax=0.0, prefilter=2, xcd_remap=1, renorm_quat=0,
                  keep_fluid=False, timing=False, graph=False):
         self.lib = load_library()
         p = Params()

@@ -36,6 +36,56 @@ __device__ __forceinline__ void symv(const Sym3& e, const double v[3], double o[
   o[2] = e.xz * v[0] + e.yz * v[1] + e.zz * v[2];
 }
 
+// Perram-Wertheim separation test.  For ellipsoids x^T E^-1 x <= 1 about
+// centres b1, b2 and r = b2 - b1,
+//   F(l) = l (1 - l) r^T [(1 - l) E1 + l E2]^-1 r,   0 < l < 1,
+// is concave with max F = s^2, s the common scale factor at which the two
+// just touch: max F > 1 <=> disjoint.  Any l with F(l) > 1 + 1e-6 therefore
+// proves every point of ellipsoid 2 to lie outside ellipsoid 1 by that
+// margin, so the contact flag of the common-normal solve (a support point
+// of one ellipsoid strictly inside the other) is false -- the solve can be
+// skipped without changing any result.  Golden-section search on l with an
+// early exit; false means "not proven disjoint" (the solve then decides).
+__device__ __forceinline__ double pw_quad(const Sym3& A, const Sym3& B, double l,
+                                          const double r[3], double& det) {
+  const double m = 1.0 - l;
+  const double xx = m * A.xx + l * B.xx, xy = m * A.xy + l * B.xy, xz = m * A.xz + l * B.xz;
+  const double yy = m * A.yy + l * B.yy, yz = m * A.yz + l * B.yz, zz = m * A.zz + l * B.zz;
+  const double c00 = yy * zz - yz * yz, c01 = xz * yz - xy * zz, c02 = xy * yz - xz * yy;
+  const double c11 = xx * zz - xz * xz, c12 = xy * xz - xx * yz, c22 = xx * yy - xy * xy;
+  det = xx * c00 + xy * c01 + xz * c02;
+  // r^T adj(M) r
+  return r[0] * r[0] * c00 + r[1] * r[1] * c11 + r[2] * r[2] * c22 +
+         2.0 * (r[0] * r[1] * c01 + r[0] * r[2] * c02 + r[1] * r[2] * c12);
+}
+
+__device__ __forceinline__ bool pw_disjoint(const Sym3& A, const Sym3& B, const double r[3]) {
+  constexpr double kMargin = 1.0 + 1.0e-6;
+  constexpr double kInvPhi = 0.6180339887498949;
+  // F(l) > margin  <=>  l (1-l) q > margin det   (det > 0: M is SPD)
+  auto excess = [&](double l) {
+    double det;
+    const double q = pw_quad(A, B, l, r, det);
+    return l * (1.0 - l) * q / det;
+  };
+  double lo = 0.0, hi = 1.0;
+  double l1 = hi - kInvPhi * (hi - lo), l2 = lo + kInvPhi * (hi - lo);
+  double f1 = excess(l1), f2 = excess(l2);
+  for (int it = 0; it < 16; ++it) {
+    if (f1 > kMargin || f2 > kMargin) return true;
+    if (f1 < f2) {
+      lo = l1; l1 = l2; f1 = f2;
+      l2 = lo + kInvPhi * (hi - lo);
+      f2 = excess(l2);
+    } else {
+      hi = l2; l2 = l1; f2 = f1;
+      l1 = hi - kInvPhi * (hi - lo);
+      f1 = excess(l1);
+    }
+  }
+  return f1 > kMargin || f2 > kMargin;
+}
+
 // margin function (mod_particle_parallel.f90:145-201): (p-c)^T Q (p-c),
 // Q = R diag(1/a^2,1/b^2,1/c^2) R^T, evaluated in the body frame.
 __device__ __forceinline__ double fmargin(const double q[4], double a, double b, double c,

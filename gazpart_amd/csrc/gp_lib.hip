@@ -551,10 +551,63 @@ __global__ __launch_bounds__(BLOCK) void k_prefilter(
   }
 }
 
+// exact disjointness test (Perram-Wertheim) on the bounding-sphere
+// survivors; pairs not proven disjoint go to the LM list in any order (the
+// list index k keeps the reference order).  use_pw = 0 passes every pair.
+// Quaternions must be unit to 1e-9 for the test to stand for the margin
+// function of the flag; others go to the LM.
+template <typename VT>
+__global__ __launch_bounds__(BLOCK) void k_pw_filter(
+    PState<VT> p, const Species* __restrict__ spec, const WorkRec* __restrict__ work,
+    const unsigned long long* nwork, unsigned long long cap, double L, int use_pw,
+    WorkRec* __restrict__ lm, unsigned long long* nlm) {
+  const unsigned long long nw = min(*nwork, cap);
+  const int lane = threadIdx.x & 63;
+  // the loop bound is uniform over the workgroup, so every lane of a
+  // wavefront takes part in each ballot
+  for (unsigned long long b0 = (unsigned long long)blockIdx.x * blockDim.x; b0 < nw;
+       b0 += (unsigned long long)gridDim.x * blockDim.x) {
+    const unsigned long long w = b0 + threadIdx.x;
+    bool pass = false;
+    WorkRec r{};
+    if (w < nw) {
+      r = work[w];
+      pass = true;
+      if (use_pw) {
+        double bi[3], bj[3];
+        min_image(p, r.i, r.j, L, bi, bj);
+        const double qi[4] = {p.qa[r.i], p.qb[r.i], p.qc[r.i], p.qd[r.i]};
+        const double qj[4] = {p.qa[r.j], p.qb[r.j], p.qc[r.j], p.qd[r.j]};
+        const double ni = qi[0] * qi[0] + qi[1] * qi[1] + qi[2] * qi[2] + qi[3] * qi[3];
+        const double nj = qj[0] * qj[0] + qj[1] * qj[1] + qj[2] * qj[2] + qj[3] * qj[3];
+        if (fabs(ni - 1.0) < 1.0e-9 && fabs(nj - 1.0) < 1.0e-9) {
+          const Species si = spec[p.sp[r.i]], sj = spec[p.sp[r.j]];
+          const Sym3 E1 = ell_matrix(qi, si.a, si.b, si.c);
+          const Sym3 E2 = ell_matrix(qj, sj.a, sj.b, sj.c);
+          const double rr[3] = {bj[0] - bi[0], bj[1] - bi[1], bj[2] - bi[2]};
+          pass = !pw_disjoint(E1, E2, rr);
+        }
+      }
+    }
+    // one atomic per wavefront: ballot + prefix
+    const unsigned long long m = __ballot(pass);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(nlm, (unsigned long long)__popcll(m));
+      base = __shfl(base, leader);
+      if (pass) lm[base + __popcll(m & ((1ull << lane) - 1ull))] = r;
+    }
+  }
+}
+
 // LM contact solve on the work list + NVREL test
 // (collision_interaction.f90:221-348); approaching contacts are appended.
+constexpr int LM_BLOCK = 64;
+constexpr int LM_GRID = 4096;
+
 template <typename VT>
-__global__ __launch_bounds__(BLOCK) void k_contact(
+__global__ __launch_bounds__(LM_BLOCK) void k_contact(
     PState<VT> p, const Species* __restrict__ spec, const WorkRec* __restrict__ work,
     const unsigned long long* nwork, unsigned long long cap, double L,
     ActiveRec* __restrict__ act, unsigned long long* nact, unsigned long long acap,
@@ -810,14 +863,15 @@ struct gp_ctx {
   int* used = nullptr;
   int* pos_of_id = nullptr;
   // work lists
-  WorkRec* work = nullptr;
+  WorkRec* work = nullptr;    // bounding-sphere survivors
+  WorkRec* lmwork = nullptr;  // pairs left for the LM (same capacity)
   unsigned long long work_cap = 0;
   ActiveRec* act = nullptr;
   unsigned char* act_status = nullptr;
   unsigned long long act_cap = 0;
   // counters (device)
   unsigned* d_err = nullptr;
-  unsigned long long* d_cnt = nullptr;   // 0 nwall 1 nwork 2 nact 3 nfev 4 ncontact 5 ncoll 6 rounds
+  unsigned long long* d_cnt = nullptr;   // 0 nwall 1 nwork 2 nact 3 nfev 4 ncontact 5 ncoll 6 rounds 7 nlm
   unsigned long long* h_cnt = nullptr;   // pinned mirror
   long long* h_total = nullptr;
   double* flout = nullptr;
@@ -1017,6 +1071,7 @@ int gp_destroy(gp_ctx* ctx) {
   void* bufs[] = {ctx->field, ctx->spec, ctx->pbuf[0], ctx->pbuf[1], ctx->perm, ctx->cell,
                   ctx->slot, ctx->cnt, ctx->off, ctx->tmp, ctx->sorted, ctx->pcount,
                   ctx->poff, ctx->scan_scr, ctx->mark, ctx->used, ctx->pos_of_id, ctx->work,
+                  ctx->lmwork,
                   ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -1415,6 +1470,7 @@ static int ensure_lists(gp_ctx* ctx, unsigned long long need) {
     drop_graph(ctx);
     const unsigned long long cap = need + need / 4 + 1024;
     if ((rc = dalloc(ctx, &ctx->work, (size_t)cap))) return rc;
+    if ((rc = dalloc(ctx, &ctx->lmwork, (size_t)cap))) return rc;
     ctx->work_cap = cap;
   }
   const unsigned long long acap = std::max<unsigned long long>(ctx->work_cap / 4, 4096);
@@ -1432,16 +1488,23 @@ template <typename VT>
 static int launch_contacts(gp_ctx* ctx) {
   const int np = ctx->np;
   unsigned long long* c = ctx->d_cnt;
-  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(8), 0, ctx->st, c + 1, 0ull, 6ll);
+  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(8), 0, ctx->st, c + 1, 0ull, 7ll);
   auto ps = pstate<VT>(ctx, ctx->cur);
   hipLaunchKernelGGL(k_prefilter<VT>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->g, ps,
                      ctx->spec, ctx->sorted, ctx->cell, ctx->off, ctx->cnt, ctx->poff,
                      ctx->ngroups > 1 ? ps.grp : nullptr, ctx->prm.L, ctx->prm.prefilter,
                      ctx->work, c + 1, ctx->work_cap, ctx->d_err, np);
-  const unsigned grid = 2048;
-  hipLaunchKernelGGL(k_contact<VT>, dim3(grid), dim3(BLOCK), 0, ctx->st, ps, ctx->spec, ctx->work,
-                     c + 1, ctx->work_cap, ctx->prm.L, ctx->act, c + 2, ctx->act_cap, c + 3,
-                     c + 4, ctx->d_err);
+  // the survivor count stays on the device: a fixed grid, grid-stride inside
+  const unsigned fgrid = (unsigned)std::min<unsigned long long>((ctx->work_cap + BLOCK - 1) / BLOCK,
+                                                                 2048ull);
+  hipLaunchKernelGGL(k_pw_filter<VT>, dim3(fgrid), dim3(BLOCK), 0, ctx->st, ps, ctx->spec,
+                     ctx->work, c + 1, ctx->work_cap, ctx->prm.L, ctx->prm.prefilter >= 2 ? 1 : 0,
+                     ctx->lmwork, c + 7);
+  // one wavefront per workgroup: the LM is latency-bound (one pair per lane),
+  // so the pairs are spread over as many CUs as possible
+  hipLaunchKernelGGL(k_contact<VT>, dim3(LM_GRID), dim3(LM_BLOCK), 0, ctx->st, ps, ctx->spec,
+                     ctx->lmwork, c + 7, ctx->work_cap, ctx->prm.L, ctx->act, c + 2, ctx->act_cap,
+                     c + 3, c + 4, ctx->d_err);
   mark(ctx, 3);
   hipLaunchKernelGGL(k_greedy<VT>, dim3(1), dim3(GREEDY_BLOCK), 0, ctx->st, ps, ctx->spec,
                      ctx->act, c + 2, ctx->act_cap, ctx->mark, ctx->used, ctx->act_status, c + 5,
@@ -1536,6 +1599,7 @@ static void accumulate_stats(gp_ctx* ctx) {
   unsigned long long* h = ctx->h_cnt;
   ctx->stats.nwall = (int64_t)h[0];
   ctx->stats.nprefilter = (int64_t)h[1];
+  ctx->stats.nlm = (int64_t)h[7];
   ctx->stats.ncontact = (int64_t)h[4];
   ctx->stats.ncollision = (int64_t)h[5];
   ctx->stats.lm_nfev = (int64_t)h[3];

@@ -20,8 +20,8 @@ module gazpart_gpu
     integer(c_int64_t) :: nclose, nprefilter, ncontact, ncollision, lm_nfev, nwall
     integer(c_int64_t) :: greedy_rounds
     real(c_double) :: t_advance_ms, t_sort_ms, t_contact_ms, t_greedy_ms
-    integer(c_int64_t) :: ntimed
-    integer(c_int64_t) :: reserved(4)
+    integer(c_int64_t) :: ntimed, nlm
+    integer(c_int64_t) :: reserved(3)
   end type gp_stats
 
   interface

@@ -71,7 +71,7 @@ contains
       p%ecw = ECW
       p%muw = MUW
       p%dpmax = 2.0*maxval(EMAJ_PART(:,:))        ! collision_initiation.f90:50
-      p%prefilter = 1
+      p%prefilter = 2
       p%xcd_remap = 1
       p%keep_fluid = 0
       p%timing = 0

@@ -69,9 +69,12 @@ typedef struct {
   double dtime;        /* DTIME */
   double ecw, muw;     /* wall restitution / friction (ECW, MUW) */
   double dpmax;        /* collision_initiation.f90:50 DPMAX; <= 0: 2*max(EMAJ) */
-  int prefilter;       /* 1: skip the LM for pairs farther apart than the sum
-                          of their largest semi-axes (flag is then false in
-                          the reference too); 0: LM on every pair */
+  int prefilter;       /* pairs proven disjoint skip the LM (their flag is
+                          false in the reference too, so results do not
+                          change): 0 none (LM on every pair); 1 centres
+                          farther apart than the sum of the largest
+                          semi-axes; 2 (recommended) 1 + the Perram-Wertheim
+                          function above 1 + 1e-6 */
   int xcd_remap;       /* 1: XCD-aware block order for the gather */
   int keep_fluid;      /* 1: keep UFAP..VORTZAP of the last advance
                           (gp_get_fluid_at_particles) */
@@ -86,7 +89,7 @@ typedef struct {
 
 typedef struct {
   int64_t nclose;       /* candidate pairs of the last collision pass */
-  int64_t nprefilter;   /* pairs that reached the LM */
+  int64_t nprefilter;   /* pairs that passed the bounding-sphere test */
   int64_t ncontact;     /* pairs with flag = .true. and NVREL < ZERO */
   int64_t ncollision;   /* pairs resolved (one collision per particle) */
   int64_t lm_nfev;      /* function evaluations summed over LM solves */
@@ -98,7 +101,8 @@ typedef struct {
    * (prefilter + LM), greedy (one-collision pass + impulses) */
   double t_advance_ms, t_sort_ms, t_contact_ms, t_greedy_ms;
   int64_t ntimed;       /* steps included in the stage times */
-  int64_t reserved[4];
+  int64_t nlm;          /* pairs that reached the LM (after every prefilter) */
+  int64_t reserved[3];
 } gp_stats;
 
 /* lifetime ----------------------------------------------------------- */

@@ -38,9 +38,9 @@ def test_struct_layouts_match(tmp_path):
     src = tmp_path / "layout.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "gazpart_gpu.h"\n'
-        "int main(void){printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(gp_params),"
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu\\n\", sizeof(gp_params),"
         " offsetof(gp_params, dpmax), offsetof(gp_params, prefilter), sizeof(gp_stats),"
-        " offsetof(gp_stats, greedy_rounds)); return 0;}\n")
+        " offsetof(gp_stats, greedy_rounds), offsetof(gp_stats, nlm)); return 0;}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                    check=True)
@@ -51,6 +51,7 @@ def test_struct_layouts_match(tmp_path):
     assert int(out[2]) == G.Params.prefilter.offset
     assert int(out[3]) == C.sizeof(G.Stats)
     assert int(out[4]) == G.Stats.greedy_rounds.offset
+    assert int(out[5]) == G.Stats.nlm.offset
 
 
 def test_no_cpu_fallback_without_gpu(lib):

@@ -228,6 +228,35 @@ def test_trajectory_against_oracle_c1(c1, prec, tol):
         assert relerr(sg, so, sl) <= tol, name
 
 
+@pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
+def test_prefilters_change_no_result(prec):
+    """The bounding-sphere and Perram-Wertheim prefilters only skip pairs
+    whose flag is false: 10 steps of the dense reference ensemble give the
+    same state bit for bit with prefilter 0 (LM on every candidate), 1 and
+    2, while the LM sees fewer pairs at each level."""
+    d = np.load(GOLD + "golden_trajectory.npz")
+    wl = golden_workload(d)
+    out, st = {}, {}
+    for lvl in (0, 1, 2):
+        ctx = G.Context.from_workload(wl, precision=prec, prefilter=lvl)
+        nlm = ncoll = 0
+        for s in range(10):
+            ctx.step(1)
+            t = ctx.stats()
+            nlm += t["nlm"]
+            ncoll += t["ncollision"]
+            if lvl == 0:
+                assert t["nlm"] == t["nclose"]
+            assert t["nlm"] <= t["nprefilter"] <= t["nclose"]
+        out[lvl] = ctx.get_particles()
+        st[lvl] = (nlm, ncoll)
+        ctx.close()
+    np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[0], out[2])
+    assert st[0][1] == st[1][1] == st[2][1] > 0
+    assert st[0][0] > st[1][0] > st[2][0]
+
+
 def test_step_equals_adv_plus_collision_and_is_deterministic(c1):
     a = G.Context.from_workload(c1)
     b = G.Context.from_workload(c1)
