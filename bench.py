@@ -51,10 +51,10 @@ def advance_bytes(np_, n, field_bytes, taps, s_rw, s_par=16):
     return np_ * (s_rw + s_par) + 6.0 * n ** 3 * field_bytes * touched
 
 
-def run_gpu(wl, precision, steps, warmup, G):
+def run_gpu(wl, precision, steps, warmup, G, brick=0):
     """Timed steps of the captured two-step hipGraph, then a separate eager
     run with HIP-event stage timers (for the per-kernel figures)."""
-    ctx = G.Context.from_workload(wl, precision=precision, graph=True)
+    ctx = G.Context.from_workload(wl, precision=precision, graph=True, brick=brick)
     ctx.step(warmup)
     ctx.sync()
     t0 = time.perf_counter()
@@ -64,7 +64,7 @@ def run_gpu(wl, precision, steps, warmup, G):
     st = ctx.stats()
     state = ctx.get_particles()
     ctx.close()
-    tctx = G.Context.from_workload(wl, precision=precision, timing=True)
+    tctx = G.Context.from_workload(wl, precision=precision, timing=True, brick=brick)
     tctx.step(warmup)
     tctx.reset_timers()
     tctx.step(min(steps, 10))
@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--scheme", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--brick", type=int, default=0,
+                    help="advance visiting order: bricks of BRICK^3 collision cells "
+                         "(0: the library default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (torch), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profile")
@@ -126,7 +129,7 @@ def main():
 
     res = {}
     for name, prec in (("fp64", G.PREC_FP64), ("mixed", G.PREC_MIXED)):
-        dt, st, state = run_gpu(wl, prec, args.steps, args.warmup, G)
+        dt, st, state = run_gpu(wl, prec, args.steps, args.warmup, G, args.brick)
         ms = 1e3 * dt / args.steps
         res[name] = dict(seconds=dt, ms_per_step=ms, stats=st, state=state,
                          updates_per_s=wl.np_ * args.steps / dt)

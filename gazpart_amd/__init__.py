@@ -45,7 +45,10 @@ EXPORTS = (
     "gp_set_species", "gp_set_particles", "gp_set_groups", "gp_get_particles", "gp_get_fluid_at_particles",
     "gp_interph", "gp_interph_field", "gp_adv_particle_ellp", "gp_collision_detection", "gp_contact_batch",
     "gp_collision", "gp_step", "gp_get_stats", "gp_reset_timers", "gp_collision_grid",
+    "gp_seed_net_set", "gp_seed_net_get", "gp_seed_net_forward", "gp_seed_net_loss_grad",
+    "gp_seed_net_train", "gp_seed_net_pairs", "gp_seed_net_fit", "gp_seed_net_eval",
 )
+SEED_NET_NPARAM = 5187
 
 
 class GpError(RuntimeError):
@@ -61,7 +64,8 @@ class Params(C.Structure):
                 ("rhof", C.c_double), ("dtime", C.c_double), ("ecw", C.c_double),
                 ("muw", C.c_double), ("dpmax", C.c_double), ("prefilter", C.c_int),
                 ("xcd_remap", C.c_int), ("keep_fluid", C.c_int), ("timing", C.c_int),
-                ("graph", C.c_int), ("reserved", C.c_int * 3)]
+                ("graph", C.c_int), ("brick", C.c_int), ("lm_seed", C.c_int),
+                ("reserved", C.c_int * 1)]
 
 
 class Stats(C.Structure):
@@ -71,7 +75,20 @@ class Stats(C.Structure):
                 ("greedy_rounds", C.c_int64), ("t_advance_ms", C.c_double),
                 ("t_sort_ms", C.c_double), ("t_contact_ms", C.c_double),
                 ("t_greedy_ms", C.c_double), ("ntimed", C.c_int64), ("nlm", C.c_int64),
-                ("reserved", C.c_int64 * 3)]
+                ("seed_fallbacks", C.c_int64), ("reserved", C.c_int64 * 2)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class SeedReport(C.Structure):
+    """gp_seed_report (include/gazpart_gpu.h)."""
+    _fields_ = [("npairs", C.c_int64), ("loss", C.c_double), ("label_ms", C.c_double),
+                ("train_ms", C.c_double), ("nfev_ref_mean", C.c_double),
+                ("nfev_net_mean", C.c_double), ("nfev_ref_max", C.c_int64),
+                ("nfev_net_max", C.c_int64), ("flags_ref", C.c_int64), ("flags_net", C.c_int64),
+                ("flag_agree", C.c_int64), ("fallbacks", C.c_int64), ("no_seed", C.c_int64),
+                ("cn_min_dot", C.c_double), ("reserved", C.c_double * 4)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -110,6 +127,16 @@ def load_library(path: str = LIB_PATH):
     L.gp_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.gp_collision_grid.argtypes = [vp, ip, dp]
     L.gp_reset_timers.argtypes = [vp]
+    L.gp_seed_net_set.argtypes = [vp, vp, C.c_int]
+    L.gp_seed_net_get.argtypes = [vp, vp, C.c_int]
+    L.gp_seed_net_forward.argtypes = [vp, C.c_int64, vp, vp]
+    L.gp_seed_net_loss_grad.argtypes = [vp, C.c_int64, vp, vp, vp, C.POINTER(C.c_double)]
+    L.gp_seed_net_train.argtypes = [vp, C.c_int64, vp, vp, C.c_int, C.c_int, C.c_float,
+                                    C.POINTER(C.c_double)]
+    L.gp_seed_net_pairs.argtypes = [vp, C.c_int64, C.c_uint64, vp, vp, vp, vp]
+    L.gp_seed_net_fit.argtypes = [vp, C.c_int64, C.c_uint64, C.c_int, C.c_int, C.c_float,
+                                  C.POINTER(SeedReport)]
+    L.gp_seed_net_eval.argtypes = [vp, C.c_int64, C.c_uint64, C.POINTER(SeedReport)]
     _lib = L
     return L
 
@@ -124,7 +151,7 @@ class Context:
 
     def __init__(self, n, L, visc, rhof, dtime, *, scheme=3, precision=PREC_FP64, ecw=1.0,
                  muw=0.0, dpmax=0.0, prefilter=2, xcd_remap=1, renorm_quat=0,
-                 keep_fluid=False, timing=False, graph=False):
+                 keep_fluid=False, timing=False, graph=False, brick=0, lm_seed=0):
         self.lib = load_library()
         p = Params()
         p.n, p.interp_scheme, p.precision, p.renorm_quat = n, scheme, precision, renorm_quat
@@ -133,6 +160,8 @@ class Context:
         p.keep_fluid = 1 if keep_fluid else 0
         p.timing = 1 if timing else 0
         p.graph = 1 if graph else 0
+        p.brick = brick
+        p.lm_seed = lm_seed
         self.params = p
         h = C.c_void_p()
         rc = self.lib.gp_create(C.byref(p), C.byref(h))
@@ -280,3 +309,58 @@ class Context:
         self._chk(self.lib.gp_contact_batch(self.h, n, _ptr(h), _ptr(nr), _ptr(flag), _ptr(e1),
                                             _ptr(e2), _ptr(cn), _ptr(nfev)))
         return flag, e1, e2, cn, nfev
+
+    # -- contact-seed network (no reference counterpart) -----------------
+    def seed_net_set(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        assert t.size == SEED_NET_NPARAM
+        self._chk(self.lib.gp_seed_net_set(self.h, _ptr(t), t.size))
+
+    def seed_net_get(self):
+        t = np.empty(SEED_NET_NPARAM, dtype=np.float32)
+        self._chk(self.lib.gp_seed_net_get(self.h, _ptr(t), t.size))
+        return t
+
+    def seed_net_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty((x.shape[0], 3), dtype=np.float32)
+        self._chk(self.lib.gp_seed_net_forward(self.h, x.shape[0], _ptr(x), _ptr(y)))
+        return y
+
+    def seed_net_loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(SEED_NET_NPARAM, dtype=np.float32)
+        loss = C.c_double(0.0)
+        self._chk(self.lib.gp_seed_net_loss_grad(self.h, x.shape[0], _ptr(x), _ptr(t), _ptr(g),
+                                                 C.byref(loss)))
+        return g, loss.value
+
+    def seed_net_train(self, x, t, epochs=1, batch=4096, lr=1e-3):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_double(0.0)
+        self._chk(self.lib.gp_seed_net_train(self.h, x.shape[0], _ptr(x), _ptr(t), epochs, batch,
+                                             lr, C.byref(loss)))
+        return loss.value
+
+    def seed_net_pairs(self, n, seed):
+        """(features (n,12), targets (n,3), evaluations (n,), poses (n,20):
+        b1, q1, a1, b2, q2, a2) of synthetic solver-labelled pairs."""
+        x = np.empty((n, 12), dtype=np.float32)
+        t = np.empty((n, 3), dtype=np.float32)
+        nf = np.empty(n, dtype=np.int32)
+        poses = np.empty((n, 20))
+        self._chk(self.lib.gp_seed_net_pairs(self.h, n, seed, _ptr(x), _ptr(t), _ptr(nf),
+                                             _ptr(poses)))
+        return x, t, nf, poses
+
+    def seed_net_fit(self, npairs, seed, epochs=4, batch=4096, lr=2e-3):
+        r = SeedReport()
+        self._chk(self.lib.gp_seed_net_fit(self.h, npairs, seed, epochs, batch, lr, C.byref(r)))
+        return r.as_dict()
+
+    def seed_net_eval(self, npairs, seed):
+        r = SeedReport()
+        self._chk(self.lib.gp_seed_net_eval(self.h, npairs, seed, C.byref(r)))
+        return r.as_dict()

@@ -357,8 +357,9 @@ __device__ __forceinline__ void lmpar2(double r[6], const int ipvt[2], const dou
 
 // lmder with lmder1's settings (minpack_reduc.f90:243-640, :732-745):
 // mode 1, factor 100, maxfev 300, ftol = xtol = sqrt(eps), gtol = 0.
-// Returns the number of function evaluations.
-__device__ int lm_solve(const ContactPair& cp, double x[2]) {
+// Returns the number of function evaluations; *info_out = MINPACK's info
+// (1 converged, 4 zero gradient, 5 maxfev or tolerances too small).
+__device__ int lm_solve(const ContactPair& cp, double x[2], int* info_out) {
   const double tol = 1.4901161193847656e-08;
   const double p1 = 0.1, p5 = 0.5, p25 = 0.25, p75 = 0.75, p0001 = 1.0e-4;
   const int maxfev = 300;
@@ -406,7 +407,7 @@ __device__ int lm_solve(const ContactPair& cp, double x[2]) {
         }
       }
     }
-    if (gnorm <= 0.0) break;   // info = 4 (gtol = 0)
+    if (gnorm <= 0.0) { info = 4; break; }   // gtol = 0
     diag[0] = fmax(diag[0], wa2[0]);
     diag[1] = fmax(diag[1], wa2[1]);
     for (;;) {
@@ -460,12 +461,13 @@ __device__ int lm_solve(const ContactPair& cp, double x[2]) {
     }
     if (info != 0) break;
   }
+  *info_out = info;
   return nfev;
 }
 
 struct ContactOut {
   double e1[3], e2[3], cn[3];
-  int flag, nfev;
+  int flag, nfev, info;
 };
 
 // ellipsoid_contact_detection (ellipsoid_contact_detection.f90:1-155) for
@@ -490,7 +492,7 @@ __device__ __forceinline__ ContactOut contact_detect(const double bi[3], const d
     x[0] = acos(c0 / cos(x[1]));
   }
   ContactOut o;
-  o.nfev = lm_solve(cp, x);
+  o.nfev = lm_solve(cp, x, &o.info);
   double cv[3], t1[3], t2[3], l1, l2;
   support(cp, x, cv, o.e1, o.e2, t1, t2, l1, l2);
   double d0 = o.e1[0] - o.e2[0], d1 = o.e1[1] - o.e2[1], d2 = o.e1[2] - o.e2[2];

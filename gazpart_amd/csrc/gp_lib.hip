@@ -15,7 +15,7 @@
 //                  then the 13 half-shell cells in CON order)
 //   k_prefilter    bounding-sphere test; survivors keep their list index
 //   k_contact      FP64 Levenberg-Marquardt contact solve + approach test
-//   k_greedy       one collision per particle in list order + impulses
+//   k_greedy_*     one collision per particle in list order + impulses
 // Particles are stored cell-sorted: the next advance reads them through the
 // sort permutation, so the field gather of neighbouring threads touches
 // neighbouring stencils.
@@ -32,6 +32,7 @@
 #include "../../include/gazpart_gpu.h"
 #include "gp_contact.h"
 #include "gp_device.h"
+#include "gp_mlp.h"
 #include "gp_scan.h"
 
 namespace gp {
@@ -47,6 +48,7 @@ struct PState {
   VT *u, *v, *w;
   double *qa, *qb, *qc, *qd;
   VT *ox, *oy, *oz;
+  float *rm; // largest semi-axis of the particle's species, rounded up to float
   int *sp;   // species row
   int *id;   // reference index (0-based)
   int *grp;  // ensemble: particles of different ensembles never collide
@@ -55,6 +57,12 @@ struct PState {
 struct CellGrid {
   int nx, ny, nz, nbox;
   double xmin, ymin, zmin, dx, dy, dz;
+};
+
+// bricks of b x b x b collision cells, row-major: the order in which the
+// advance visits the particles (a compact set of stencils per workgroup)
+struct BrickGrid {
+  int b, nx, ny, nz, n;
 };
 
 struct ActiveRec {   // a contact that is approaching (flag and NVREL < ZERO)
@@ -215,6 +223,18 @@ __device__ __forceinline__ int cell_of(const CellGrid& g, double x, double y, do
   return (i0 - 1) + (j0 - 1) * g.nx + (k0 - 1) * g.nx * g.ny;
 }
 
+__device__ __forceinline__ int brick_of(const CellGrid& g, const BrickGrid& bg, int c) {
+  const int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny);
+  return i / bg.b + bg.nx * (j / bg.b + bg.ny * (k / bg.b));
+}
+
+// float not below the double (the bounding-sphere test stays conservative)
+__device__ __forceinline__ float float_up(double r) {
+  float f = (float)r;
+  if ((double)f < r) f = nextafterf(f, __builtin_huge_valf());
+  return f;
+}
+
 // the 13-cell half shell of COLLISION_BOX_CONNECTIVITY
 // (collision_box_connectivity.f90:127-316) with the periodic wrap of a
 // single rank (all 13 present); 0-based boxes in CON order
@@ -328,12 +348,34 @@ __global__ __launch_bounds__(BLOCK) void k_interph1(const FT* __restrict__ f, in
   out[p] = (double)v[0];
 }
 
+// slot in a counter shared by lanes with the same key: one atomic per
+// distinct key in the wavefront (lanes of a brick-ordered wavefront mostly
+// share one brick, where per-lane atomics on one address would serialise).
+// Every lane that needs a slot must call it together (wave-uniform loop).
+__device__ __forceinline__ int grouped_slot(int* cnt, int key) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(1);
+  int slot = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lkey = __shfl(key, leader);
+    const unsigned long long same = __ballot(key == lkey) & todo;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[lkey], __popcll(same));
+    base = __shfl(base, leader);
+    if ((same >> lane) & 1ull) slot = base + __popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  return slot;
+}
+
 // fused gather + advance + wrap + cell binning; dst[i] = advance(src[perm[i]])
 template <typename FT, typename VT, typename CT, int S>
 __global__ __launch_bounds__(BLOCK) void k_advance(
     StepParams P, const FT* __restrict__ f, const Species* __restrict__ spec, PState<VT> src,
     PState<VT> dst, const int* __restrict__ perm, int np, CellGrid g, int* __restrict__ cell,
-    int* __restrict__ slot, int* __restrict__ cnt, unsigned* err, unsigned long long* nwall,
+    int* __restrict__ slot, int* __restrict__ cnt, BrickGrid bg, int* __restrict__ brick,
+    int* __restrict__ bslot, int* __restrict__ bcnt, unsigned* err, unsigned long long* nwall,
     double* __restrict__ flout, int xcd) {
   const int i = block_index(xcd) * BLOCK + threadIdx.x;
   if (i >= np) return;
@@ -354,15 +396,23 @@ __global__ __launch_bounds__(BLOCK) void k_advance(
   int wall = 0;
   e |= advance_particle<CT>(P, sp, st, fl, &wall);
   store_state(dst, i, st);
+  dst.rm[i] = src.rm[s];
   dst.sp[i] = sid;
   dst.id[i] = pid;
   dst.grp[i] = src.grp[s];
+  int c = 0;
   if (cell) {
     bool oob;
-    int c = cell_of(g, st[0], st[1], st[2], &oob);
+    c = cell_of(g, st[0], st[1], st[2], &oob);
     if (oob) e |= EB_OUT_OF_GRID;
     cell[i] = c;
-    slot[i] = (c >= 0) ? atomicAdd(&cnt[c], 1) : 0;
+    slot[i] = atomicAdd(&cnt[c], 1);
+  }
+  if (bcnt) {
+    // every lane of the wavefront that is still here takes part
+    const int b = brick_of(g, bg, c);
+    brick[i] = b;
+    bslot[i] = grouped_slot(bcnt, b);
   }
   if (wall) atomicAdd(nwall, 1ull);
   if (e) atomicOr(err, e);
@@ -372,14 +422,32 @@ __global__ __launch_bounds__(BLOCK) void k_advance(
 template <typename VT>
 __global__ __launch_bounds__(BLOCK) void k_cells(PState<VT> p, int np, CellGrid g,
                                                  int* __restrict__ cell, int* __restrict__ slot,
-                                                 int* __restrict__ cnt, unsigned* err) {
+                                                 int* __restrict__ cnt, BrickGrid bg,
+                                                 int* __restrict__ brick, int* __restrict__ bslot,
+                                                 int* __restrict__ bcnt, unsigned* err) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   bool oob;
   int c = cell_of(g, p.x[i], p.y[i], p.z[i], &oob);
   if (oob) atomicOr(err, EB_OUT_OF_GRID);
   cell[i] = c;
-  slot[i] = (c >= 0) ? atomicAdd(&cnt[c], 1) : 0;
+  slot[i] = atomicAdd(&cnt[c], 1);
+  if (bcnt) {
+    const int b = brick_of(g, bg, c);
+    brick[i] = b;
+    bslot[i] = grouped_slot(bcnt, b);
+  }
+}
+
+// visiting order of the next advance: particles grouped by brick (any
+// order inside a brick; results do not depend on the storage order)
+__global__ __launch_bounds__(BLOCK) void k_brick_scatter(const int* __restrict__ brick,
+                                                         const int* __restrict__ bslot,
+                                                         const int* __restrict__ boff,
+                                                         int* __restrict__ trav, int np) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  trav[boff[brick[i]] + bslot[i]] = i;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_scatter(const int* __restrict__ cell,
@@ -400,11 +468,15 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
                                                 const int* __restrict__ tmp,
                                                 const int* __restrict__ id,
                                                 const int* __restrict__ grp,
-                                                int* __restrict__ sorted, int np) {
+                                                const double* __restrict__ px,
+                                                const double* __restrict__ py,
+                                                const double* __restrict__ pz,
+                                                const float* __restrict__ rm,
+                                                int* __restrict__ sorted,
+                                                double4* __restrict__ pos4, int np) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   int c = cell[i];
-  if (c < 0) return;
   const int o = off[c], m = cnt[c], me = id[i];
   int r = 0;
   if (grp) {
@@ -417,6 +489,9 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
     for (int t = 0; t < m; ++t) r += (id[tmp[o + t]] < me) ? 1 : 0;
   }
   sorted[o + r] = i;
+  // cell-ordered copy of what the bounding-sphere test reads: a cell's
+  // members are contiguous here
+  pos4[o + r] = make_double4(px[i], py[i], pz[i], (double)rm[i]);
 }
 
 // members of cell range [o, o+m) (sorted by ensemble) in ensemble g
@@ -507,47 +582,72 @@ __device__ __forceinline__ void min_image(const PState<VT>& p, int i, int j, dou
 // the index k); survivors go to the LM work list.  Exact: a contact needs
 // the support point of one ellipsoid inside the other, which is impossible
 // when the centres are farther apart than the sum of the largest semi-axes.
-template <typename VT>
+// Survivors are staged in LDS and appended with one global atomic per
+// workgroup (appends from every lane to one counter would serialise);
+// LDS overflow, in dense suspensions, appends directly.
+constexpr int PF_STAGE = 1024;
+
 __global__ __launch_bounds__(BLOCK) void k_prefilter(
-    CellGrid g, PState<VT> p, const Species* __restrict__ spec, const int* __restrict__ sorted,
+    CellGrid g, const double4* __restrict__ pos4, const int* __restrict__ sorted,
     const int* __restrict__ cell, const int* __restrict__ off, const int* __restrict__ cnt,
     const long long* __restrict__ poff, const int* __restrict__ grp, double L, int use_filter,
     WorkRec* __restrict__ work, unsigned long long* nwork, unsigned long long cap, unsigned* err,
     int nsorted) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nsorted) return;
-  const int hi = sorted[q];
-  const int c = cell[hi];
-  long long k = poff[q];
-  const double xi = p.x[hi], yi = p.y[hi], zi = p.z[hi];
-  const double ri = spec[p.sp[hi]].rmax;
-  auto test = [&](int j) {
-    double dx = p.x[j] - xi, dy = p.y[j] - yi, dz = p.z[j] - zi;
-    if (fabs(dx) > L / 2.0) dx -= copysign(L, dx);
-    if (fabs(dy) > L / 2.0) dy -= copysign(L, dy);
-    if (fabs(dz) > L / 2.0) dz -= copysign(L, dz);
-    const double rs = ri + spec[p.sp[j]].rmax;
-    if (!use_filter || dx * dx + dy * dy + dz * dz < rs * rs * (1.0 + 1.0e-9)) {
-      unsigned long long w = atomicAdd(nwork, 1ull);
-      if (w < cap) {
-        work[w].k = k; work[w].i = hi; work[w].j = j;
-      } else {
-        atomicOr(err, EB_ACTIVE_OVERFLOW);
+  __shared__ WorkRec stage[PF_STAGE];
+  __shared__ int nstage;
+  __shared__ unsigned long long gbase;
+  if (threadIdx.x == 0) nstage = 0;
+  __syncthreads();
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nsorted) {
+    const int hi = sorted[q];
+    const int c = cell[hi];
+    long long k = poff[q];
+    const double4 h = pos4[q];
+    const double xi = h.x, yi = h.y, zi = h.z, ri = h.w;
+    auto test = [&](int t) {
+      const double4 o = pos4[t];
+      double dx = o.x - xi, dy = o.y - yi, dz = o.z - zi;
+      if (fabs(dx) > L / 2.0) dx -= copysign(L, dx);
+      if (fabs(dy) > L / 2.0) dy -= copysign(L, dy);
+      if (fabs(dz) > L / 2.0) dz -= copysign(L, dz);
+      const double rs = ri + o.w;
+      if (!use_filter || dx * dx + dy * dy + dz * dz < rs * rs * (1.0 + 1.0e-9)) {
+        WorkRec r;
+        r.k = k; r.i = hi; r.j = sorted[t];
+        const int sidx = atomicAdd(&nstage, 1);
+        if (sidx < PF_STAGE) {
+          stage[sidx] = r;
+        } else {
+          const unsigned long long w = atomicAdd(nwork, 1ull);
+          if (w < cap) work[w] = r;
+          else atomicOr(err, EB_ACTIVE_OVERFLOW);
+        }
       }
+      ++k;
+    };
+    int lo = off[c], up = off[c] + cnt[c];
+    const int gr = grp ? grp[hi] : 0;
+    if (grp) group_range(sorted, grp, off[c], cnt[c], gr, &lo, &up);
+    for (int t = q + 1; t < up; ++t) test(t);
+    int nb[13];
+    half_shell(g, c, nb);
+    for (int v = 0; v < 13; ++v) {
+      lo = off[nb[v]];
+      up = lo + cnt[nb[v]];
+      if (grp) group_range(sorted, grp, off[nb[v]], cnt[nb[v]], gr, &lo, &up);
+      for (int t = lo; t < up; ++t) test(t);
     }
-    ++k;
-  };
-  int lo = off[c], up = off[c] + cnt[c];
-  const int gr = grp ? grp[hi] : 0;
-  if (grp) group_range(sorted, grp, off[c], cnt[c], gr, &lo, &up);
-  for (int t = q + 1; t < up; ++t) test(sorted[t]);
-  int nb[13];
-  half_shell(g, c, nb);
-  for (int v = 0; v < 13; ++v) {
-    lo = off[nb[v]];
-    up = lo + cnt[nb[v]];
-    if (grp) group_range(sorted, grp, off[nb[v]], cnt[nb[v]], gr, &lo, &up);
-    for (int t = lo; t < up; ++t) test(sorted[t]);
+  }
+  __syncthreads();
+  const int ns = min(nstage, PF_STAGE);
+  if (ns == 0) return;
+  if (threadIdx.x == 0) gbase = atomicAdd(nwork, (unsigned long long)ns);
+  __syncthreads();
+  for (int t = threadIdx.x; t < ns; t += blockDim.x) {
+    const unsigned long long w = gbase + (unsigned long long)t;
+    if (w < cap) work[w] = stage[t];
+    else atomicOr(err, EB_ACTIVE_OVERFLOW);
   }
 }
 
@@ -603,64 +703,206 @@ __global__ __launch_bounds__(BLOCK) void k_pw_filter(
 
 // LM contact solve on the work list + NVREL test
 // (collision_interaction.f90:221-348); approaching contacts are appended.
+// One pair per lane and one wavefront per workgroup, launched with one
+// wavefront per SIMD.  Pairs are dealt lane-major (pair = lane * waves +
+// wave), so a short list leaves few active lanes in every wavefront rather
+// than a few full ones: a wavefront runs the union of its lanes' iteration
+// paths, and the kernel lasts as long as its slowest wavefront.
 constexpr int LM_BLOCK = 64;
-constexpr int LM_GRID = 4096;
 
 template <typename VT>
 __global__ __launch_bounds__(LM_BLOCK) void k_contact(
     PState<VT> p, const Species* __restrict__ spec, const WorkRec* __restrict__ work,
     const unsigned long long* nwork, unsigned long long cap, double L,
     ActiveRec* __restrict__ act, unsigned long long* nact, unsigned long long acap,
-    unsigned long long* nfev_sum, unsigned long long* ncontact, unsigned* err) {
+    unsigned long long* nfev_sum, unsigned long long* ncontact, unsigned* err,
+    const double* __restrict__ seeds, unsigned long long* nfallback) {
   const unsigned long long nw = min(*nwork, cap);
-  for (unsigned long long w = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
-       w += (unsigned long long)gridDim.x * blockDim.x) {
-    const WorkRec r = work[w];
-    double bi[3], bj[3];
-    min_image(p, r.i, r.j, L, bi, bj);
-    const double qi[4] = {p.qa[r.i], p.qb[r.i], p.qc[r.i], p.qd[r.i]};
-    const double qj[4] = {p.qa[r.j], p.qb[r.j], p.qc[r.j], p.qd[r.j]};
-    const Species si = spec[p.sp[r.i]], sj = spec[p.sp[r.j]];
-    ContactOut o = contact_detect(bi, qi, si, bj, qj, sj, nullptr);
-    atomicAdd(nfev_sum, (unsigned long long)o.nfev);
-    if (!o.flag) continue;
-    // velocities of the deepest points (collision_interaction.f90:269-324)
-    double r1[3] = {o.e1[0] - bi[0], o.e1[1] - bi[1], o.e1[2] - bi[2]};
-    double r2[3] = {o.e2[0] - bj[0], o.e2[1] - bj[1], o.e2[2] - bj[2]};
-    Rot<double> R1(qi[0], qi[1], qi[2], qi[3]), R2(qj[0], qj[1], qj[2], qj[3]);
-    double w1[3], w2[3];
-    R1.mul((double)p.ox[r.i], (double)p.oy[r.i], (double)p.oz[r.i], w1[0], w1[1], w1[2]);
-    R2.mul((double)p.ox[r.j], (double)p.oy[r.j], (double)p.oz[r.j], w2[0], w2[1], w2[2]);
-    double v1[3] = {(double)p.u[r.i] + (w1[1] * r1[2] - w1[2] * r1[1]),
-                    (double)p.v[r.i] + (w1[2] * r1[0] - w1[0] * r1[2]),
-                    (double)p.w[r.i] + (w1[0] * r1[1] - w1[1] * r1[0])};
-    double v2[3] = {(double)p.u[r.j] + (w2[1] * r2[2] - w2[2] * r2[1]),
-                    (double)p.v[r.j] + (w2[2] * r2[0] - w2[0] * r2[2]),
-                    (double)p.w[r.j] + (w2[0] * r2[1] - w2[1] * r2[0])};
-    double nvrel = (v2[0] - v1[0]) * o.cn[0] + (v2[1] - v1[1]) * o.cn[1] +
-                   (v2[2] - v1[2]) * o.cn[2];
-    if (nvrel < ZERO_R) {
-      atomicAdd(ncontact, 1ull);
-      unsigned long long a = atomicAdd(nact, 1ull);
-      if (a < acap) {
-        ActiveRec& ar = act[a];
-        ar.k = r.k; ar.i = r.i; ar.j = r.j;
-        for (int c = 0; c < 3; ++c) { ar.r1[c] = r1[c]; ar.r2[c] = r2[c]; ar.cn[c] = o.cn[c]; }
-      } else {
-        atomicOr(err, EB_ACTIVE_OVERFLOW);
+  const unsigned long long waves = gridDim.x;
+  const int lane = threadIdx.x;
+  unsigned long long nfev = 0, nfb = 0;
+  // the loop bound is uniform over the wavefront (w - lane * waves is), so
+  // the ballot below sees every lane
+  for (unsigned long long w0 = blockIdx.x; w0 < nw; w0 += waves * LM_BLOCK) {
+    const unsigned long long w = w0 + (unsigned long long)lane * waves;
+    bool approach = false;
+    ActiveRec ar;
+    if (w < nw) {
+      const WorkRec r = work[w];
+      double bi[3], bj[3];
+      min_image(p, r.i, r.j, L, bi, bj);
+      const double qi[4] = {p.qa[r.i], p.qb[r.i], p.qc[r.i], p.qd[r.i]};
+      const double qj[4] = {p.qa[r.j], p.qb[r.j], p.qc[r.j], p.qd[r.j]};
+      const Species si = spec[p.sp[r.i]], sj = spec[p.sp[r.j]];
+      // network seed when given; a seeded solve that ends without
+      // converging (maxfev / tolerances) is redone from the centre line
+      const double* sd = (seeds && seeds[2 * w] == seeds[2 * w]) ? seeds + 2 * w : nullptr;
+      ContactOut o;
+      for (int pass = 0; pass < 2; ++pass) {
+        o = contact_detect(bi, qi, si, bj, qj, sj, sd);
+        nfev += (unsigned long long)o.nfev;
+        if (!(sd && o.info == 5)) break;
+        sd = nullptr;
+        ++nfb;
       }
+      if (o.flag) {
+        // velocities of the deepest points (collision_interaction.f90:269-324)
+        double r1[3] = {o.e1[0] - bi[0], o.e1[1] - bi[1], o.e1[2] - bi[2]};
+        double r2[3] = {o.e2[0] - bj[0], o.e2[1] - bj[1], o.e2[2] - bj[2]};
+        Rot<double> R1(qi[0], qi[1], qi[2], qi[3]), R2(qj[0], qj[1], qj[2], qj[3]);
+        double w1[3], w2[3];
+        R1.mul((double)p.ox[r.i], (double)p.oy[r.i], (double)p.oz[r.i], w1[0], w1[1], w1[2]);
+        R2.mul((double)p.ox[r.j], (double)p.oy[r.j], (double)p.oz[r.j], w2[0], w2[1], w2[2]);
+        double v1[3] = {(double)p.u[r.i] + (w1[1] * r1[2] - w1[2] * r1[1]),
+                        (double)p.v[r.i] + (w1[2] * r1[0] - w1[0] * r1[2]),
+                        (double)p.w[r.i] + (w1[0] * r1[1] - w1[1] * r1[0])};
+        double v2[3] = {(double)p.u[r.j] + (w2[1] * r2[2] - w2[2] * r2[1]),
+                        (double)p.v[r.j] + (w2[2] * r2[0] - w2[0] * r2[2]),
+                        (double)p.w[r.j] + (w2[0] * r2[1] - w2[1] * r2[0])};
+        double nvrel = (v2[0] - v1[0]) * o.cn[0] + (v2[1] - v1[1]) * o.cn[1] +
+                       (v2[2] - v1[2]) * o.cn[2];
+        if (nvrel < ZERO_R) {
+          approach = true;
+          ar.k = r.k; ar.i = r.i; ar.j = r.j;
+          for (int c = 0; c < 3; ++c) { ar.r1[c] = r1[c]; ar.r2[c] = r2[c]; ar.cn[c] = o.cn[c]; }
+        }
+      }
+    }
+    const unsigned long long m = __ballot(approach);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) {
+        base = atomicAdd(nact, (unsigned long long)__popcll(m));
+        atomicAdd(ncontact, (unsigned long long)__popcll(m));
+      }
+      base = __shfl(base, leader);
+      if (approach) {
+        const unsigned long long a = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (a < acap) act[a] = ar;
+        else atomicOr(err, EB_ACTIVE_OVERFLOW);
+      }
+    }
+  }
+  // one atomic per wavefront for the evaluation and fallback counts
+  for (int o = 32; o > 0; o >>= 1) {
+    nfev += __shfl_xor(nfev, o);
+    nfb += __shfl_xor(nfb, o);
+  }
+  if (lane == 0 && nfev) atomicAdd(nfev_sum, nfev);
+  if (lane == 0 && nfb) atomicAdd(nfallback, nfb);
+}
+
+// network forward for the 64 lanes of a one-wavefront workgroup: features
+// -> seed angles.  Every lane calls it (valid or not).
+__device__ __forceinline__ bool wave_seed(const MlpDev& W, bool valid, const float f[12],
+                                          const double q1[4], double x[2], float (*sF)[16],
+                                          float (*sY)[4]) {
+  const int lane = threadIdx.x;
+  for (int c = 0; c < 12; ++c) sF[lane][c] = valid ? f[c] : 0.f;
+  for (int c = 12; c < 16; ++c) sF[lane][c] = 0.f;
+  __syncthreads();
+  const int rq = 4 * (lane >> 4);
+  for (int tile = 0; tile < 4; ++tile) {
+    const f32x4 xv = *reinterpret_cast<const f32x4*>(&sF[16 * tile + (lane & 15)][rq]);
+    MlpAct a;
+    const f32x4 y = mlp_forward_tile(W, to_bf(xv), a, lane);
+    if (lane < 16)
+      for (int c = 0; c < 3; ++c) sY[16 * tile + lane][c] = y[c];
+  }
+  __syncthreads();
+  const bool ok = valid && seed_from_output(q1, sY[lane][0], sY[lane][1], sY[lane][2], x);
+  __syncthreads();
+  return ok;
+}
+
+// network seeds for the LM list (NaN: use the centre line)
+template <typename VT>
+__global__ __launch_bounds__(64) void k_mlp_seed(PState<VT> p, const Species* __restrict__ spec,
+                                                 const WorkRec* __restrict__ work,
+                                                 const unsigned long long* nwork,
+                                                 unsigned long long cap, double L, MlpDev W,
+                                                 double* __restrict__ seeds) {
+  __shared__ __attribute__((aligned(16))) float sF[64][16];
+  __shared__ float sY[64][4];
+  const unsigned long long nw = min(*nwork, cap);
+  for (unsigned long long w0 = (unsigned long long)blockIdx.x * 64; w0 < nw;
+       w0 += (unsigned long long)gridDim.x * 64) {
+    const unsigned long long w = w0 + threadIdx.x;
+    const bool valid = w < nw;
+    float f[12];
+    double qi[4] = {1.0, 0.0, 0.0, 0.0};
+    if (valid) {
+      const WorkRec r = work[w];
+      double bi[3], bj[3];
+      min_image(p, r.i, r.j, L, bi, bj);
+      qi[0] = p.qa[r.i]; qi[1] = p.qb[r.i]; qi[2] = p.qc[r.i]; qi[3] = p.qd[r.i];
+      const double qj[4] = {p.qa[r.j], p.qb[r.j], p.qc[r.j], p.qd[r.j]};
+      const Species si = spec[p.sp[r.i]], sj = spec[p.sp[r.j]];
+      const double ai[3] = {si.a, si.b, si.c}, aj[3] = {sj.a, sj.b, sj.c};
+      pair_features(bi, qi, ai, bj, qj, aj, f);
+    }
+    double x[2];
+    const bool ok = wave_seed(W, valid, f, qi, x, sF, sY);
+    if (valid) {
+      seeds[2 * w] = ok ? x[0] : __builtin_nan("");
+      seeds[2 * w + 1] = ok ? x[1] : __builtin_nan("");
     }
   }
 }
 
-__device__ __forceinline__ long long ld_acq(long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ int ld_acq(int* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// synthetic pairs solved from both seeds (the network's gain):
+// per pair the evaluation counts, flags, fallback and |cn_ref . cn_net|
+struct SeedEvalOut {
+  int* nfev_ref;
+  int* nfev_net;
+  unsigned char* flag_ref;
+  unsigned char* flag_net;
+  unsigned char* fallback;
+  float* cn_dot;
+};
+__global__ __launch_bounds__(64) void k_mlp_eval(unsigned long long seed, long long n, MlpDev W,
+                                                 SeedEvalOut out) {
+  __shared__ __attribute__((aligned(16))) float sF[64][16];
+  __shared__ float sY[64][4];
+  for (long long k0 = (long long)blockIdx.x * 64; k0 < n; k0 += (long long)gridDim.x * 64) {
+    const long long k = k0 + threadIdx.x;
+    const bool valid = k < n;
+    PairPose pp{};
+    float f[12];
+    if (valid) {
+      pp = sample_pair(seed, k);
+      pair_features(pp.b1, pp.q1, pp.a1, pp.b2, pp.q2, pp.a2, f);
+    } else {
+      pp.q1[0] = 1.0;
+    }
+    double x[2];
+    const bool ok = wave_seed(W, valid, f, pp.q1, x, sF, sY);
+    if (!valid) continue;
+    const Species s1 = pose_species(pp.a1), s2 = pose_species(pp.a2);
+    const ContactOut r = contact_detect(pp.b1, pp.q1, s1, pp.b2, pp.q2, s2, nullptr);
+    const double* sd = ok ? x : nullptr;
+    ContactOut o;
+    int nf = 0;
+    unsigned char fb = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      o = contact_detect(pp.b1, pp.q1, s1, pp.b2, pp.q2, s2, sd);
+      nf += o.nfev;
+      if (!(sd && o.info == 5)) break;
+      sd = nullptr;
+      fb = 1;
+    }
+    out.nfev_ref[k] = r.nfev;
+    out.nfev_net[k] = nf;
+    out.flag_ref[k] = (unsigned char)r.flag;
+    out.flag_net[k] = (unsigned char)o.flag;
+    out.fallback[k] = (unsigned char)(fb | (ok ? 0 : 2));
+    out.cn_dot[k] = (float)fabs(r.cn[0] * o.cn[0] + r.cn[1] * o.cn[1] + r.cn[2] * o.cn[2]);
+  }
 }
 
 constexpr int GREEDY_BLOCK = 1024;
+constexpr int GREEDY_GRID = 256;
 
 // One collision per particle in list order (collision_interaction.f90:190-
 // 207, 397-398).  Contact flags depend only on positions and orientations,
@@ -669,20 +911,90 @@ constexpr int GREEDY_BLOCK = 1024;
 // Rounds: every undecided contact that holds the lowest list index at both
 // of its particles is accepted; contacts touching an accepted particle are
 // dropped.  Then the accepted (disjoint) pairs exchange impulses.
-// mark[] (per particle, LLONG_MAX when idle) and used[] are restored on exit.
-template <typename VT>
-__global__ __launch_bounds__(GREEDY_BLOCK) void k_greedy(
-    PState<VT> p, const Species* __restrict__ spec, ActiveRec* __restrict__ act,
-    const unsigned long long* nact, unsigned long long acap, long long* mark, int* used,
-    unsigned char* __restrict__ status, unsigned long long* ncoll,
-    unsigned long long* rounds_out, unsigned* err) {
+// The first round runs over many workgroups (k_greedy_mark / _accept /
+// _drop); the rare later rounds in one workgroup (k_greedy_rounds, which
+// returns at once when nothing is undecided); then k_greedy_impulse.
+// status: 0 undecided, 1 accepted, 2 dropped.  mark[] (per particle,
+// LLONG_MAX when idle) and used[] are back at their idle values on exit.
+__global__ __launch_bounds__(BLOCK) void k_greedy_mark(const ActiveRec* __restrict__ act,
+                                                       const unsigned long long* nact,
+                                                       unsigned long long acap,
+                                                       long long* mark) {
+  const long long na = (long long)min(*nact, acap);
+  for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na;
+       a += (long long)gridDim.x * blockDim.x) {
+    atomicMin(&mark[act[a].i], act[a].k);
+    atomicMin(&mark[act[a].j], act[a].k);
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_greedy_accept(const ActiveRec* __restrict__ act,
+                                                         const unsigned long long* nact,
+                                                         unsigned long long acap,
+                                                         const long long* __restrict__ mark,
+                                                         int* __restrict__ used,
+                                                         unsigned char* __restrict__ status) {
+  const long long na = (long long)min(*nact, acap);
+  for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na;
+       a += (long long)gridDim.x * blockDim.x) {
+    const ActiveRec& r = act[a];
+    if (mark[r.i] == r.k && mark[r.j] == r.k) {
+      // accepted pairs are disjoint: each particle is written by one pair
+      status[a] = 1;
+      used[r.i] = 1;
+      used[r.j] = 1;
+    } else {
+      status[a] = 0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_greedy_drop(const ActiveRec* __restrict__ act,
+                                                       const unsigned long long* nact,
+                                                       unsigned long long acap,
+                                                       long long* __restrict__ mark,
+                                                       const int* __restrict__ used,
+                                                       unsigned char* __restrict__ status,
+                                                       unsigned long long* left) {
+  const long long na = (long long)min(*nact, acap);
+  for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < na;
+       b0 += (long long)gridDim.x * blockDim.x) {
+    const long long a = b0 + threadIdx.x;
+    bool undecided = false;
+    if (a < na) {
+      const ActiveRec& r = act[a];
+      if (status[a] == 0) {
+        if (used[r.i] || used[r.j]) status[a] = 2;
+        else undecided = true;
+      }
+      mark[r.i] = LLONG_MAX;
+      mark[r.j] = LLONG_MAX;
+    }
+    const unsigned long long m = __ballot(undecided);
+    if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1)
+      atomicAdd(left, (unsigned long long)__popcll(m));
+  }
+}
+
+__device__ __forceinline__ long long ld_relaxed(long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_relaxed(int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// rounds 2, 3, ... over the contacts still undecided (one workgroup)
+__global__ __launch_bounds__(GREEDY_BLOCK) void k_greedy_rounds(
+    const ActiveRec* __restrict__ act, const unsigned long long* nact, unsigned long long acap,
+    long long* mark, int* used, unsigned char* __restrict__ status,
+    const unsigned long long* left, unsigned long long* rounds_out) {
+  if (*left == 0) {
+    if (threadIdx.x == 0) *rounds_out += 1ull;
+    return;
+  }
   const long long na = (long long)min(*nact, acap);
   __shared__ int s_left;
-  __shared__ int s_ncoll;
-  if (threadIdx.x == 0) s_ncoll = 0;
-  for (long long a = threadIdx.x; a < na; a += blockDim.x) status[a] = 0;
-  __syncthreads();
-  int rounds = 0;
+  int rounds = 1;
   for (;;) {
     if (threadIdx.x == 0) s_left = 0;
     for (long long a = threadIdx.x; a < na; a += blockDim.x) {
@@ -694,18 +1006,19 @@ __global__ __launch_bounds__(GREEDY_BLOCK) void k_greedy(
     for (long long a = threadIdx.x; a < na; a += blockDim.x) {
       if (status[a] != 0) continue;
       const long long k = act[a].k;
-      if (ld_acq(&mark[act[a].i]) == k && ld_acq(&mark[act[a].j]) == k) {
-        status[a] = 1;
+      if (ld_relaxed(&mark[act[a].i]) == k && ld_relaxed(&mark[act[a].j]) == k) {
+        status[a] = 3;   // accepted this round (still resets its marks below)
         atomicExch(&used[act[a].i], 1);
         atomicExch(&used[act[a].j], 1);
       }
     }
     __syncthreads();
     for (long long a = threadIdx.x; a < na; a += blockDim.x) {
-      if (status[a] != 0) {
-        continue;
-      }
-      if (ld_acq(&used[act[a].i]) || ld_acq(&used[act[a].j])) {
+      const unsigned char st = status[a];
+      if (st == 1 || st == 2) continue;
+      if (st == 3) {
+        status[a] = 1;
+      } else if (ld_relaxed(&used[act[a].i]) || ld_relaxed(&used[act[a].j])) {
         status[a] = 2;
       } else {
         atomicAdd(&s_left, 1);
@@ -715,40 +1028,48 @@ __global__ __launch_bounds__(GREEDY_BLOCK) void k_greedy(
     }
     __syncthreads();
     ++rounds;
-    const int left = s_left;
+    const int nleft = s_left;
     __syncthreads();
-    if (left == 0) break;
+    if (nleft == 0) break;
   }
-  // impulses on the accepted, disjoint pairs
-  for (long long a = threadIdx.x; a < na; a += blockDim.x) {
-    const ActiveRec r = act[a];
-    if (status[a] == 1) {
-      atomicAdd(&s_ncoll, 1);
-      double u1[3] = {(double)p.u[r.i], (double)p.v[r.i], (double)p.w[r.i]};
-      double o1[3] = {(double)p.ox[r.i], (double)p.oy[r.i], (double)p.oz[r.i]};
-      double u2[3] = {(double)p.u[r.j], (double)p.v[r.j], (double)p.w[r.j]};
-      double o2[3] = {(double)p.ox[r.j], (double)p.oy[r.j], (double)p.oz[r.j]};
-      const double q1[4] = {p.qa[r.i], p.qb[r.i], p.qc[r.i], p.qd[r.i]};
-      const double q2[4] = {p.qa[r.j], p.qb[r.j], p.qc[r.j], p.qd[r.j]};
-      const Species s1 = spec[p.sp[r.i]];
-      if (!collision_response(s1, q1, q2, u1, o1, u2, o2, r.r1, r.r2, r.cn))
-        atomicOr(err, EB_KE_COLLISION);
-      p.u[r.i] = (VT)u1[0]; p.v[r.i] = (VT)u1[1]; p.w[r.i] = (VT)u1[2];
-      p.ox[r.i] = (VT)o1[0]; p.oy[r.i] = (VT)o1[1]; p.oz[r.i] = (VT)o1[2];
-      p.u[r.j] = (VT)u2[0]; p.v[r.j] = (VT)u2[1]; p.w[r.j] = (VT)u2[2];
-      p.ox[r.j] = (VT)o2[0]; p.oy[r.j] = (VT)o2[1]; p.oz[r.j] = (VT)o2[2];
+  if (threadIdx.x == 0) *rounds_out += (unsigned long long)rounds;
+}
+
+// impulses on the accepted, disjoint pairs; used[] back to 0
+template <typename VT>
+__global__ __launch_bounds__(BLOCK) void k_greedy_impulse(
+    PState<VT> p, const Species* __restrict__ spec, const ActiveRec* __restrict__ act,
+    const unsigned long long* nact, unsigned long long acap, int* __restrict__ used,
+    const unsigned char* __restrict__ status, unsigned long long* ncoll, unsigned* err) {
+  const long long na = (long long)min(*nact, acap);
+  for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < na;
+       b0 += (long long)gridDim.x * blockDim.x) {
+    const long long a = b0 + threadIdx.x;
+    bool acc = false;
+    if (a < na) {
+      const ActiveRec r = act[a];
+      used[r.i] = 0;
+      used[r.j] = 0;
+      if (status[a] == 1) {
+        acc = true;
+        double u1[3] = {(double)p.u[r.i], (double)p.v[r.i], (double)p.w[r.i]};
+        double o1[3] = {(double)p.ox[r.i], (double)p.oy[r.i], (double)p.oz[r.i]};
+        double u2[3] = {(double)p.u[r.j], (double)p.v[r.j], (double)p.w[r.j]};
+        double o2[3] = {(double)p.ox[r.j], (double)p.oy[r.j], (double)p.oz[r.j]};
+        const double q1[4] = {p.qa[r.i], p.qb[r.i], p.qc[r.i], p.qd[r.i]};
+        const double q2[4] = {p.qa[r.j], p.qb[r.j], p.qc[r.j], p.qd[r.j]};
+        const Species s1 = spec[p.sp[r.i]];
+        if (!collision_response(s1, q1, q2, u1, o1, u2, o2, r.r1, r.r2, r.cn))
+          atomicOr(err, EB_KE_COLLISION);
+        p.u[r.i] = (VT)u1[0]; p.v[r.i] = (VT)u1[1]; p.w[r.i] = (VT)u1[2];
+        p.ox[r.i] = (VT)o1[0]; p.oy[r.i] = (VT)o1[1]; p.oz[r.i] = (VT)o1[2];
+        p.u[r.j] = (VT)u2[0]; p.v[r.j] = (VT)u2[1]; p.w[r.j] = (VT)u2[2];
+        p.ox[r.j] = (VT)o2[0]; p.oy[r.j] = (VT)o2[1]; p.oz[r.j] = (VT)o2[2];
+      }
     }
-  }
-  __syncthreads();
-  for (long long a = threadIdx.x; a < na; a += blockDim.x) {
-    used[act[a].i] = 0;
-    used[act[a].j] = 0;
-    mark[act[a].i] = LLONG_MAX;
-    mark[act[a].j] = LLONG_MAX;
-  }
-  if (threadIdx.x == 0) {
-    *ncoll += (unsigned long long)s_ncoll;
-    *rounds_out += (unsigned long long)rounds;
+    const unsigned long long m = __ballot(acc);
+    if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1)
+      atomicAdd(ncoll, (unsigned long long)__popcll(m));
   }
 }
 
@@ -803,15 +1124,24 @@ __global__ void k_gather_out(PState<VT> p, double* out, int np) {
 }
 
 template <typename VT>
-__global__ void k_scatter_in(PState<VT> p, const double* in, const int* sp, int np) {
+__global__ void k_scatter_in(PState<VT> p, const double* in, const int* sp,
+                             const Species* __restrict__ spec, int np) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   double st[13];
   for (int c = 0; c < 13; ++c) st[c] = in[(long long)c * np + i];
   store_state(p, i, st);
   p.sp[i] = sp[i];
+  p.rm[i] = float_up(spec[sp[i]].rmax);
   p.id[i] = i;
   p.grp[i] = 0;
+}
+
+// per-particle copy of the species' largest semi-axis after a new table
+template <typename VT>
+__global__ void k_refresh_rm(PState<VT> p, const Species* __restrict__ spec, int np) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) p.rm[i] = float_up(spec[p.sp[i]].rmax);
 }
 
 }  // namespace gp
@@ -847,6 +1177,30 @@ struct gp_ctx {
   int* perm = nullptr;          // order in which the next advance reads cur
   int ngroups = 1;              // ensembles sharing the context
   bool perm_identity = true;
+  int max_sp = -1;              // largest species index of the particles
+  int lm_waves = 1024;          // LM launch: one wavefront per SIMD
+  // brick visiting order of the advance (gp_params.brick)
+  BrickGrid bg{};
+  bool use_bricks = false;
+  int* brick = nullptr;
+  int* bslot = nullptr;
+  int* bcnt = nullptr;
+  int* boff = nullptr;
+  int* trav = nullptr;
+  double4* pos4 = nullptr;      // (x, y, z, rmax) in cell (sorted) order
+  // contact-seed network (fp32 masters, Adam moments, bf16 copies)
+  float* mlp_th = nullptr;
+  float* mlp_m = nullptr;
+  float* mlp_v = nullptr;
+  float* mlp_grad = nullptr;
+  short* mlp_wb = nullptr;      // w1 | w2 | w3 | w2t | w3t (bf16)
+  float* mlp_bias = nullptr;    // b1 | b2 | b3
+  float* mlp_part = nullptr;    // per-wavefront partial gradients
+  float* mlp_lpart = nullptr;
+  float* mlp_loss = nullptr;
+  int mlp_step = 0;
+  bool mlp_ready = false;
+  double* seeds = nullptr;      // 2 per LM-list entry
   // collision grid
   CellGrid g{};
   int* cell = nullptr;
@@ -871,7 +1225,8 @@ struct gp_ctx {
   unsigned long long act_cap = 0;
   // counters (device)
   unsigned* d_err = nullptr;
-  unsigned long long* d_cnt = nullptr;   // 0 nwall 1 nwork 2 nact 3 nfev 4 ncontact 5 ncoll 6 rounds 7 nlm
+  unsigned long long* d_cnt = nullptr;   // 0 nwall 1 nwork 2 nact 3 nfev 4 ncontact 5 ncoll 6 rounds 7 nlm 8 undecided
+                                         // 9 seed fallbacks
   unsigned long long* h_cnt = nullptr;   // pinned mirror
   long long* h_total = nullptr;
   double* flout = nullptr;
@@ -895,6 +1250,20 @@ struct gp_ctx {
 static void drop_graph(gp_ctx* ctx) {
   if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
   ctx->gexec = nullptr;
+}
+
+// bf16 block offsets (in shorts) and bias offsets of the seed network
+constexpr int MW_W1 = 0, MW_W2 = MW_W1 + 64 * 16, MW_W3 = MW_W2 + 64 * 64,
+              MW_W2T = MW_W3 + 16 * 64, MW_W3T = MW_W2T + 64 * 64, MW_N = MW_W3T + 64 * 16;
+constexpr int MB_B1 = 0, MB_B2 = 64, MB_B3 = 128, MB_N = 144;
+constexpr int MLP_PARTS = 512;   // wavefronts of the training kernel
+
+static MlpDev mlp_dev(gp_ctx* ctx) {
+  MlpDev d;
+  d.w1 = ctx->mlp_wb + MW_W1; d.w2 = ctx->mlp_wb + MW_W2; d.w3 = ctx->mlp_wb + MW_W3;
+  d.w2t = ctx->mlp_wb + MW_W2T; d.w3t = ctx->mlp_wb + MW_W3T;
+  d.b1 = ctx->mlp_bias + MB_B1; d.b2 = ctx->mlp_bias + MB_B2; d.b3 = ctx->mlp_bias + MB_B3;
+  return d;
 }
 
 #define HIPCHK(x)                                                               \
@@ -923,14 +1292,15 @@ static PState<VT> pstate(gp_ctx* ctx, int which) {
   s.qa = d + 3 * n; s.qb = d + 4 * n; s.qc = d + 5 * n; s.qd = d + 6 * n;
   VT* v = (VT*)(d + 7 * n);
   s.u = v; s.v = v + n; s.w = v + 2 * n; s.ox = v + 3 * n; s.oy = v + 4 * n; s.oz = v + 5 * n;
-  int* ip = (int*)(v + 6 * n);
+  s.rm = (float*)(v + 6 * n);
+  int* ip = (int*)(s.rm + n);
   s.sp = ip; s.id = ip + n; s.grp = ip + 2 * n;
   return s;
 }
 
 static size_t pstate_bytes(int np, bool mixed) {
   const size_t n = (size_t)np;
-  return n * 7 * 8 + n * 6 * (mixed ? 4 : 8) + n * 3 * 4 + 256;
+  return n * 7 * 8 + n * 6 * (mixed ? 4 : 8) + n * 4 + n * 3 * 4 + 256;
 }
 
 template <typename T>
@@ -999,6 +1369,15 @@ static void grid_init(gp_ctx* ctx) {
   g.dy = (ymax - ymin) / (double)g.ny;
   g.dz = (ymax - ymin) / (double)g.nz;
   g.nbox = g.nx * g.ny * g.nz;
+  // brick visiting order: b cells a side (0 or < 0: reference cell order)
+  const int b = ctx->prm.brick == 0 ? -1 : ctx->prm.brick;
+  ctx->use_bricks = b > 0;
+  BrickGrid& bg = ctx->bg;
+  bg.b = b > 0 ? b : 1;
+  bg.nx = (g.nx + bg.b - 1) / bg.b;
+  bg.ny = (g.ny + bg.b - 1) / bg.b;
+  bg.nz = (g.nz + bg.b - 1) / bg.b;
+  bg.n = bg.nx * bg.ny * bg.nz;
 }
 
 static int ensure_grid_buffers(gp_ctx* ctx) {
@@ -1006,7 +1385,10 @@ static int ensure_grid_buffers(gp_ctx* ctx) {
   int rc;
   if ((rc = dalloc(ctx, &ctx->cnt, (size_t)g.nbox))) return rc;
   if ((rc = dalloc(ctx, &ctx->off, (size_t)g.nbox + 1))) return rc;
-  const int64_t sc = std::max(scan_scratch(g.nbox), scan_scratch((int64_t)ctx->np + 1)) + 16;
+  if ((rc = dalloc(ctx, &ctx->bcnt, (size_t)ctx->bg.n))) return rc;
+  if ((rc = dalloc(ctx, &ctx->boff, (size_t)ctx->bg.n + 1))) return rc;
+  const int64_t sc = std::max(std::max(scan_scratch(g.nbox), scan_scratch((int64_t)ctx->np + 1)),
+                              scan_scratch(ctx->bg.n)) + 16;
   if (ctx->scan_scr) hipFree(ctx->scan_scr);
   ctx->scan_scr = nullptr;
   HIPCHK(hipMalloc(&ctx->scan_scr, (size_t)sc * sizeof(long long)));
@@ -1028,6 +1410,13 @@ int gp_create(const gp_params* p, gp_ctx** out) {
   ctx->prm = *p;
   ctx->mixed = p->precision == GP_PREC_MIXED;
   hipGetDevice(&ctx->device);
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+            hipSuccess || ncu < 1)
+      ncu = 256;
+    ctx->lm_waves = 4 * ncu;   // one wavefront per SIMD
+  }
   StepParams& s = ctx->sp;
   s.n = p->n;
   s.scheme = p->interp_scheme;
@@ -1071,7 +1460,9 @@ int gp_destroy(gp_ctx* ctx) {
   void* bufs[] = {ctx->field, ctx->spec, ctx->pbuf[0], ctx->pbuf[1], ctx->perm, ctx->cell,
                   ctx->slot, ctx->cnt, ctx->off, ctx->tmp, ctx->sorted, ctx->pcount,
                   ctx->poff, ctx->scan_scr, ctx->mark, ctx->used, ctx->pos_of_id, ctx->work,
-                  ctx->lmwork,
+                  ctx->lmwork, ctx->brick, ctx->bslot, ctx->bcnt, ctx->boff, ctx->trav,
+                  ctx->pos4, ctx->mlp_th, ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, ctx->mlp_wb,
+                  ctx->mlp_bias, ctx->mlp_part, ctx->mlp_lpart, ctx->mlp_loss, ctx->seeds,
                   ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -1123,6 +1514,11 @@ int gp_bind_field(gp_ctx* ctx, const double* field) {
 
 int gp_set_species(gp_ctx* ctx, int nspec, const double* rows) {
   if (!ctx || nspec < 1 || !rows) return GP_ERR_ARG;
+  if (ctx->np > 0 && ctx->max_sp >= nspec) {
+    ctx->err = "the particles use species index " + std::to_string(ctx->max_sp) +
+               " beyond the new table of " + std::to_string(nspec) + " rows";
+    return GP_ERR_ARG;
+  }
   hipStreamSynchronize(ctx->st);
   drop_graph(ctx);
   std::vector<Species> h((size_t)nspec);
@@ -1149,6 +1545,16 @@ int gp_set_species(gp_ctx* ctx, int nspec, const double* rows) {
   grid_init(ctx);
   if (ctx->np > 0) {
     if ((rc = ensure_grid_buffers(ctx))) return rc;
+    if (ctx->mixed)
+      hipLaunchKernelGGL(k_refresh_rm<float>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st,
+                         pstate<float>(ctx, ctx->cur), ctx->spec, ctx->np);
+    else
+      hipLaunchKernelGGL(k_refresh_rm<double>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st,
+                         pstate<double>(ctx, ctx->cur), ctx->spec, ctx->np);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    // the grid may have changed: the next advance starts from storage order
+    ctx->perm_identity = true;
   }
   ctx->cells_valid = false;
   return GP_OK;
@@ -1162,12 +1568,16 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
     ctx->err = "gp_set_species must precede gp_set_particles";
     return GP_ERR_STATE;
   }
-  for (int i = 0; i < np; ++i)
+  int max_sp = 0;
+  for (int i = 0; i < np; ++i) {
     if (species[i] < 0 || species[i] >= ctx->nspec) {
       ctx->err = "species index out of range at particle " + std::to_string(i);
       return GP_ERR_ARG;
     }
+    max_sp = std::max(max_sp, species[i]);
+  }
   ctx->np = np;
+  ctx->max_sp = max_sp;
   const size_t pb = pstate_bytes(np, ctx->mixed);
   for (int k = 0; k < 2; ++k) {
     if (ctx->pbuf[k]) hipFree(ctx->pbuf[k]);
@@ -1185,6 +1595,10 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   if ((rc = dalloc(ctx, &ctx->mark, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->used, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->pos_of_id, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->brick, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->bslot, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->trav, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->pos4, (size_t)np))) return rc;
   if (ctx->prm.keep_fluid) {
     if ((rc = dalloc(ctx, &ctx->flout, (size_t)np * 6))) return rc;
   }
@@ -1203,10 +1617,10 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   ctx->cur = 0;
   if (ctx->mixed)
     hipLaunchKernelGGL(k_scatter_in<float>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st,
-                       pstate<float>(ctx, 0), din, dsp, np);
+                       pstate<float>(ctx, 0), din, dsp, ctx->spec, np);
   else
     hipLaunchKernelGGL(k_scatter_in<double>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st,
-                       pstate<double>(ctx, 0), din, dsp, np);
+                       pstate<double>(ctx, 0), din, dsp, ctx->spec, np);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->st));
   hipFree(din);
@@ -1372,10 +1786,15 @@ template <typename VT>
 static int launch_advance(gp_ctx* ctx, bool bin) {
   const int np = ctx->np;
   const int src = ctx->cur, dst = 1 - ctx->cur;
-  if (bin)
+  if (bin) {
     hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->g.nbox)), dim3(BLOCK), 0, ctx->st, ctx->cnt, 0,
                        (long long)ctx->g.nbox);
-  const int* perm = ctx->perm_identity ? nullptr : ctx->perm;
+    if (ctx->use_bricks)
+      hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->bg.n)), dim3(BLOCK), 0, ctx->st, ctx->bcnt,
+                         0, (long long)ctx->bg.n);
+  }
+  const int* perm = ctx->perm_identity ? nullptr : (ctx->use_bricks ? ctx->trav : ctx->perm);
+  int* bcnt = bin && ctx->use_bricks ? ctx->bcnt : nullptr;
   const int xcd = ctx->prm.xcd_remap;
   auto ps = pstate<VT>(ctx, src), pd = pstate<VT>(ctx, dst);
   int* cell = bin ? ctx->cell : nullptr;
@@ -1388,20 +1807,24 @@ static int launch_advance(gp_ctx* ctx, bool bin) {
     if (ctx->prm.interp_scheme == 3)
       hipLaunchKernelGGL((k_advance<float, VT, float, 4>), gr, bl, 0, ctx->st, ctx->sp,
                          (const float*)ctx->field, ctx->spec, ps, pd, perm, np, ctx->g, cell,
-                         ctx->slot, ctx->cnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd);
+                         ctx->slot, ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,
+                         ctx->d_cnt + 0, ctx->flout, xcd);
     else
       hipLaunchKernelGGL((k_advance<float, VT, float, 5>), gr, bl, 0, ctx->st, ctx->sp,
                          (const float*)ctx->field, ctx->spec, ps, pd, perm, np, ctx->g, cell,
-                         ctx->slot, ctx->cnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd);
+                         ctx->slot, ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,
+                         ctx->d_cnt + 0, ctx->flout, xcd);
   } else {
     if (ctx->prm.interp_scheme == 3)
       hipLaunchKernelGGL((k_advance<double, VT, double, 4>), gr, bl, 0, ctx->st, ctx->sp,
                          (const double*)ctx->field, ctx->spec, ps, pd, perm, np, ctx->g, cell,
-                         ctx->slot, ctx->cnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd);
+                         ctx->slot, ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,
+                         ctx->d_cnt + 0, ctx->flout, xcd);
     else
       hipLaunchKernelGGL((k_advance<double, VT, double, 5>), gr, bl, 0, ctx->st, ctx->sp,
                          (const double*)ctx->field, ctx->spec, ps, pd, perm, np, ctx->g, cell,
-                         ctx->slot, ctx->cnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd);
+                         ctx->slot, ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,
+                         ctx->d_cnt + 0, ctx->flout, xcd);
   }
   HIPCHK(hipGetLastError());
   mark(ctx, 1);
@@ -1416,8 +1839,12 @@ static int launch_bin(gp_ctx* ctx) {
   const int np = ctx->np;
   hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->g.nbox)), dim3(BLOCK), 0, ctx->st, ctx->cnt, 0,
                      (long long)ctx->g.nbox);
+  if (ctx->use_bricks)
+    hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->bg.n)), dim3(BLOCK), 0, ctx->st, ctx->bcnt, 0,
+                       (long long)ctx->bg.n);
   hipLaunchKernelGGL(k_cells<VT>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st,
                      pstate<VT>(ctx, ctx->cur), np, ctx->g, ctx->cell, ctx->slot, ctx->cnt,
+                     ctx->bg, ctx->brick, ctx->bslot, ctx->use_bricks ? ctx->bcnt : nullptr,
                      ctx->d_err);
   HIPCHK(hipGetLastError());
   ctx->cells_valid = true;
@@ -1438,10 +1865,16 @@ static int launch_sort(gp_ctx* ctx) {
                      ctx->off, ctx->tmp, np);
   const int* grp = ctx->ngroups > 1 ? ps.grp : nullptr;
   hipLaunchKernelGGL(k_rank, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->off,
-                     ctx->cnt, ctx->tmp, ps.id, grp, ctx->sorted, np);
+                     ctx->cnt, ctx->tmp, ps.id, grp, ps.x, ps.y, ps.z, ps.rm, ctx->sorted,
+                     ctx->pos4, np);
   hipLaunchKernelGGL(k_pair_count, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, g, ctx->sorted,
                      ctx->cell, ctx->off, ctx->cnt, grp, ctx->pcount, np);
   scan_exclusive<int, long long>(ctx->pcount, ctx->poff, np, (long long*)ctx->scan_scr, ctx->st);
+  if (ctx->use_bricks) {
+    scan_exclusive<int, int>(ctx->bcnt, ctx->boff, ctx->bg.n, (int*)ctx->scan_scr, ctx->st);
+    hipLaunchKernelGGL(k_brick_scatter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->brick,
+                       ctx->bslot, ctx->boff, ctx->trav, np);
+  }
   HIPCHK(hipGetLastError());
   mark(ctx, 2);
   return GP_OK;
@@ -1471,6 +1904,7 @@ static int ensure_lists(gp_ctx* ctx, unsigned long long need) {
     const unsigned long long cap = need + need / 4 + 1024;
     if ((rc = dalloc(ctx, &ctx->work, (size_t)cap))) return rc;
     if ((rc = dalloc(ctx, &ctx->lmwork, (size_t)cap))) return rc;
+    if ((rc = dalloc(ctx, &ctx->seeds, (size_t)cap * 2))) return rc;
     ctx->work_cap = cap;
   }
   const unsigned long long acap = std::max<unsigned long long>(ctx->work_cap / 4, 4096);
@@ -1488,10 +1922,10 @@ template <typename VT>
 static int launch_contacts(gp_ctx* ctx) {
   const int np = ctx->np;
   unsigned long long* c = ctx->d_cnt;
-  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(8), 0, ctx->st, c + 1, 0ull, 7ll);
+  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(16), 0, ctx->st, c + 1, 0ull, 9ll);
   auto ps = pstate<VT>(ctx, ctx->cur);
-  hipLaunchKernelGGL(k_prefilter<VT>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->g, ps,
-                     ctx->spec, ctx->sorted, ctx->cell, ctx->off, ctx->cnt, ctx->poff,
+  hipLaunchKernelGGL(k_prefilter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->pos4,
+                     ctx->sorted, ctx->cell, ctx->off, ctx->cnt, ctx->poff,
                      ctx->ngroups > 1 ? ps.grp : nullptr, ctx->prm.L, ctx->prm.prefilter,
                      ctx->work, c + 1, ctx->work_cap, ctx->d_err, np);
   // the survivor count stays on the device: a fixed grid, grid-stride inside
@@ -1502,18 +1936,30 @@ static int launch_contacts(gp_ctx* ctx) {
                      ctx->lmwork, c + 7);
   // one wavefront per workgroup: the LM is latency-bound (one pair per lane),
   // so the pairs are spread over as many CUs as possible
-  hipLaunchKernelGGL(k_contact<VT>, dim3(LM_GRID), dim3(LM_BLOCK), 0, ctx->st, ps, ctx->spec,
+  const bool net = ctx->prm.lm_seed == 1;
+  if (net)
+    hipLaunchKernelGGL(k_mlp_seed<VT>, dim3(ctx->lm_waves), dim3(64), 0, ctx->st, ps, ctx->spec,
+                       ctx->lmwork, c + 7, ctx->work_cap, ctx->prm.L, mlp_dev(ctx), ctx->seeds);
+  hipLaunchKernelGGL(k_contact<VT>, dim3(ctx->lm_waves), dim3(LM_BLOCK), 0, ctx->st, ps, ctx->spec,
                      ctx->lmwork, c + 7, ctx->work_cap, ctx->prm.L, ctx->act, c + 2, ctx->act_cap,
-                     c + 3, c + 4, ctx->d_err);
+                     c + 3, c + 4, ctx->d_err, net ? ctx->seeds : nullptr, c + 9);
   mark(ctx, 3);
-  hipLaunchKernelGGL(k_greedy<VT>, dim3(1), dim3(GREEDY_BLOCK), 0, ctx->st, ps, ctx->spec,
-                     ctx->act, c + 2, ctx->act_cap, ctx->mark, ctx->used, ctx->act_status, c + 5,
-                     c + 6, ctx->d_err);
+  hipLaunchKernelGGL(k_greedy_mark, dim3(GREEDY_GRID), dim3(BLOCK), 0, ctx->st, ctx->act, c + 2,
+                     ctx->act_cap, ctx->mark);
+  hipLaunchKernelGGL(k_greedy_accept, dim3(GREEDY_GRID), dim3(BLOCK), 0, ctx->st, ctx->act,
+                     c + 2, ctx->act_cap, ctx->mark, ctx->used, ctx->act_status);
+  hipLaunchKernelGGL(k_greedy_drop, dim3(GREEDY_GRID), dim3(BLOCK), 0, ctx->st, ctx->act, c + 2,
+                     ctx->act_cap, ctx->mark, ctx->used, ctx->act_status, c + 8);
+  hipLaunchKernelGGL(k_greedy_rounds, dim3(1), dim3(GREEDY_BLOCK), 0, ctx->st, ctx->act, c + 2,
+                     ctx->act_cap, ctx->mark, ctx->used, ctx->act_status, c + 8, c + 6);
+  hipLaunchKernelGGL(k_greedy_impulse<VT>, dim3(GREEDY_GRID), dim3(BLOCK), 0, ctx->st, ps,
+                     ctx->spec, ctx->act, c + 2, ctx->act_cap, ctx->used, ctx->act_status, c + 5,
+                     ctx->d_err);
   HIPCHK(hipGetLastError());
   mark(ctx, 4);
   ctx->ev_pending = ctx->timing;
-  // the next advance reads the particles in cell order
-  std::swap(ctx->perm, ctx->sorted);
+  // the next advance reads the particles in brick order (or cell order)
+  if (!ctx->use_bricks) std::swap(ctx->perm, ctx->sorted);
   ctx->perm_identity = false;
   ctx->cells_valid = false;
   return GP_OK;
@@ -1600,6 +2046,7 @@ static void accumulate_stats(gp_ctx* ctx) {
   ctx->stats.nwall = (int64_t)h[0];
   ctx->stats.nprefilter = (int64_t)h[1];
   ctx->stats.nlm = (int64_t)h[7];
+  ctx->stats.seed_fallbacks = (int64_t)h[9];
   ctx->stats.ncontact = (int64_t)h[4];
   ctx->stats.ncollision = (int64_t)h[5];
   ctx->stats.lm_nfev = (int64_t)h[3];
@@ -1626,7 +2073,7 @@ int gp_adv_particle_ellp(gp_ctx* ctx, int ncycle) {
                      1ll);
   rc = ctx->mixed ? launch_advance<float>(ctx, true) : launch_advance<double>(ctx, true);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 8 * sizeof(unsigned long long),
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 10 * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, ctx->st));
   rc = check_err(ctx);
   ctx->stats.nwall = (int64_t)ctx->h_cnt[0];
@@ -1719,7 +2166,7 @@ int gp_collision(gp_ctx* ctx, int ncycle, double time) {
   if (rc) return rc;
   rc = ctx->mixed ? launch_collide<float>(ctx) : launch_collide<double>(ctx);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 8 * sizeof(unsigned long long),
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 10 * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, ctx->st));
   rc = check_err(ctx);
   accumulate_stats(ctx);
@@ -1733,7 +2180,7 @@ int gp_step(gp_ctx* ctx, int nsteps) {
   if (ctx->prm.graph) {
     rc = ctx->mixed ? graph_steps<float>(ctx, nsteps) : graph_steps<double>(ctx, nsteps);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 8 * sizeof(unsigned long long),
+    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 10 * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, ctx->st));
     return GP_OK;
   }
@@ -1745,7 +2192,7 @@ int gp_step(gp_ctx* ctx, int nsteps) {
     rc = ctx->mixed ? launch_collide<float>(ctx) : launch_collide<double>(ctx);
     if (rc) return rc;
   }
-  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 8 * sizeof(unsigned long long),
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 10 * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, ctx->st));
   return GP_OK;
 }
@@ -1780,6 +2227,380 @@ int gp_collision_grid(gp_ctx* ctx, int* nbox_xyz, double* box_min_size) {
   box_min_size[0] = ctx->g.xmin; box_min_size[1] = ctx->g.ymin; box_min_size[2] = ctx->g.zmin;
   box_min_size[3] = ctx->g.dx; box_min_size[4] = ctx->g.dy; box_min_size[5] = ctx->g.dz;
   return GP_OK;
+}
+
+}  // extern "C"
+
+
+// ===========================================================================
+// contact-seed network
+// ===========================================================================
+static int mlp_alloc(gp_ctx* ctx) {
+  if (ctx->mlp_th) return GP_OK;
+  int rc;
+  if ((rc = dalloc(ctx, &ctx->mlp_th, (size_t)MLP_NPAD))) return rc;
+  if ((rc = dalloc(ctx, &ctx->mlp_m, (size_t)MLP_NPAD))) return rc;
+  if ((rc = dalloc(ctx, &ctx->mlp_v, (size_t)MLP_NPAD))) return rc;
+  if ((rc = dalloc(ctx, &ctx->mlp_grad, (size_t)MLP_NPAD))) return rc;
+  if ((rc = dalloc(ctx, &ctx->mlp_wb, (size_t)MW_N))) return rc;
+  if ((rc = dalloc(ctx, &ctx->mlp_bias, (size_t)MB_N))) return rc;
+  if ((rc = dalloc(ctx, &ctx->mlp_part, (size_t)MLP_PARTS * MLP_NPAD))) return rc;
+  if ((rc = dalloc(ctx, &ctx->mlp_lpart, (size_t)MLP_PARTS))) return rc;
+  if ((rc = dalloc(ctx, &ctx->mlp_loss, (size_t)1))) return rc;
+  HIPCHK(hipMemsetAsync(ctx->mlp_wb, 0, MW_N * sizeof(short), ctx->st));
+  HIPCHK(hipMemsetAsync(ctx->mlp_bias, 0, MB_N * sizeof(float), ctx->st));
+  return GP_OK;
+}
+
+// compact parameter vector <-> padded layout
+static void mlp_pad(const float* c, std::vector<float>& pd) {
+  pd.assign(MLP_NPAD, 0.f);
+  int o = 0;
+  for (int h = 0; h < 64; ++h)
+    for (int f = 0; f < 12; ++f) pd[MO_W1 + h * 16 + f] = c[o++];
+  for (int h = 0; h < 64; ++h) pd[MO_B1 + h] = c[o++];
+  for (int k = 0; k < 64 * 64; ++k) pd[MO_W2 + k] = c[o++];
+  for (int h = 0; h < 64; ++h) pd[MO_B2 + h] = c[o++];
+  for (int k = 0; k < 3 * 64; ++k) pd[MO_W3 + k] = c[o++];
+  for (int k = 0; k < 3; ++k) pd[MO_B3 + k] = c[o++];
+}
+static void mlp_unpad(const float* pd, float* c) {
+  int o = 0;
+  for (int h = 0; h < 64; ++h)
+    for (int f = 0; f < 12; ++f) c[o++] = pd[MO_W1 + h * 16 + f];
+  for (int h = 0; h < 64; ++h) c[o++] = pd[MO_B1 + h];
+  for (int k = 0; k < 64 * 64; ++k) c[o++] = pd[MO_W2 + k];
+  for (int h = 0; h < 64; ++h) c[o++] = pd[MO_B2 + h];
+  for (int k = 0; k < 3 * 64; ++k) c[o++] = pd[MO_W3 + k];
+  for (int k = 0; k < 3; ++k) c[o++] = pd[MO_B3 + k];
+}
+
+static void mlp_launch_adam(gp_ctx* ctx, int update, float lr) {
+  float c1 = 1.f, c2 = 1.f;
+  if (update) {
+    ++ctx->mlp_step;
+    c1 = 1.f - powf(0.9f, (float)ctx->mlp_step);
+    c2 = 1.f - powf(0.999f, (float)ctx->mlp_step);
+  }
+  short* wb = ctx->mlp_wb;
+  float* bb = ctx->mlp_bias;
+  hipLaunchKernelGGL(k_mlp_adam, dim3(nblk(MLP_NPAD)), dim3(BLOCK), 0, ctx->st, ctx->mlp_th,
+                     ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, lr, c1, c2, wb + MW_W1, wb + MW_W2,
+                     wb + MW_W3, wb + MW_W2T, wb + MW_W3T, bb + MB_B1, bb + MB_B2, bb + MB_B3,
+                     update);
+}
+
+// one gradient over n rows already on the device (x [n][16], t [n][4])
+static void mlp_launch_grad(gp_ctx* ctx, const float* dx, const float* dt, long long n) {
+  const long long ntile = (n + 15) / 16;
+  const int parts = (int)std::min<long long>(ntile, MLP_PARTS);
+  hipLaunchKernelGGL(k_mlp_train_part, dim3(parts), dim3(64), 0, ctx->st, mlp_dev(ctx), dx, dt,
+                     n, 1.0f / (float)n, ctx->mlp_part, ctx->mlp_lpart);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3(nblk(MLP_NPAD)), dim3(BLOCK), 0, ctx->st, ctx->mlp_part,
+                     parts, ctx->mlp_grad, ctx->mlp_lpart, ctx->mlp_loss);
+}
+
+// host rows (12 features, 3 targets) -> device rows (16, 4)
+static int mlp_upload(gp_ctx* ctx, long long n, const float* x, const float* t, float** dx,
+                      float** dt) {
+  std::vector<float> hx((size_t)n * 16, 0.f), ht;
+  for (long long k = 0; k < n; ++k)
+    for (int c = 0; c < 12; ++c) hx[(size_t)k * 16 + c] = x[k * 12 + c];
+  HIPCHK(hipMalloc((void**)dx, (size_t)n * 16 * sizeof(float)));
+  HIPCHK(hipMemcpy(*dx, hx.data(), hx.size() * sizeof(float), hipMemcpyHostToDevice));
+  if (t) {
+    ht.assign((size_t)n * 4, 0.f);
+    for (long long k = 0; k < n; ++k)
+      for (int c = 0; c < 3; ++c) ht[(size_t)k * 4 + c] = t[k * 3 + c];
+    HIPCHK(hipMalloc((void**)dt, (size_t)n * 4 * sizeof(float)));
+    HIPCHK(hipMemcpy(*dt, ht.data(), ht.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return GP_OK;
+}
+
+// epochs of mini-batch Adam over n device rows (batches in order)
+static int mlp_train_device(gp_ctx* ctx, const float* dx, const float* dt, long long n,
+                            int epochs, int batch, float lr, double* loss) {
+  float h_loss = 0.f;
+  double lsum = 0.0;
+  long long lrows = 0;
+  for (int e = 0; e < epochs; ++e) {
+    lsum = 0.0;
+    lrows = 0;
+    for (long long b0 = 0; b0 < n; b0 += batch) {
+      const long long nb = std::min<long long>(batch, n - b0);
+      mlp_launch_grad(ctx, dx + b0 * 16, dt + b0 * 4, nb);
+      mlp_launch_adam(ctx, 1, lr);
+      if (e == epochs - 1) {
+        HIPCHK(hipMemcpyAsync(&h_loss, ctx->mlp_loss, sizeof(float), hipMemcpyDeviceToHost,
+                              ctx->st));
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        lsum += (double)h_loss;
+        lrows += nb;
+      }
+    }
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if (loss) *loss = lrows ? lsum / (double)lrows : 0.0;
+  return GP_OK;
+}
+
+extern "C" {
+
+int gp_seed_net_set(gp_ctx* ctx, const float* theta, int nparam) {
+  if (!ctx || !theta || nparam != GP_SEED_NET_NPARAM) return GP_ERR_ARG;
+  int rc;
+  if ((rc = mlp_alloc(ctx))) return rc;
+  std::vector<float> pd;
+  mlp_pad(theta, pd);
+  HIPCHK(hipMemcpyAsync(ctx->mlp_th, pd.data(), MLP_NPAD * sizeof(float), hipMemcpyHostToDevice,
+                        ctx->st));
+  HIPCHK(hipMemsetAsync(ctx->mlp_m, 0, MLP_NPAD * sizeof(float), ctx->st));
+  HIPCHK(hipMemsetAsync(ctx->mlp_v, 0, MLP_NPAD * sizeof(float), ctx->st));
+  mlp_launch_adam(ctx, 0, 0.f);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->mlp_step = 0;
+  ctx->mlp_ready = true;
+  return GP_OK;
+}
+
+int gp_seed_net_get(gp_ctx* ctx, float* theta, int nparam) {
+  if (!ctx || !theta || nparam != GP_SEED_NET_NPARAM) return GP_ERR_ARG;
+  if (!ctx->mlp_ready) {
+    ctx->err = "the seed network has no parameters (gp_seed_net_set)";
+    return GP_ERR_STATE;
+  }
+  std::vector<float> pd(MLP_NPAD);
+  HIPCHK(hipMemcpyAsync(pd.data(), ctx->mlp_th, MLP_NPAD * sizeof(float), hipMemcpyDeviceToHost,
+                        ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  mlp_unpad(pd.data(), theta);
+  return GP_OK;
+}
+
+int gp_seed_net_forward(gp_ctx* ctx, int64_t n, const float* x, float* y) {
+  if (!ctx || n < 1 || !x || !y) return GP_ERR_ARG;
+  if (!ctx->mlp_ready) {
+    ctx->err = "the seed network has no parameters (gp_seed_net_set)";
+    return GP_ERR_STATE;
+  }
+  float *dx = nullptr, *dy = nullptr;
+  int rc = mlp_upload(ctx, n, x, nullptr, &dx, nullptr);
+  if (!rc) {
+    HIPCHK(hipMalloc((void**)&dy, (size_t)n * 3 * sizeof(float)));
+    const long long ntile = (n + 15) / 16;
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)std::min<long long>(ntile, 4096)), dim3(64),
+                       0, ctx->st, mlp_dev(ctx), dx, (long long)n, dy);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(y, dy, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+  }
+  if (dx) hipFree(dx);
+  if (dy) hipFree(dy);
+  return rc;
+}
+
+int gp_seed_net_loss_grad(gp_ctx* ctx, int64_t n, const float* x, const float* t, float* grad,
+                          double* loss) {
+  if (!ctx || n < 1 || !x || !t || !grad) return GP_ERR_ARG;
+  if (!ctx->mlp_ready) {
+    ctx->err = "the seed network has no parameters (gp_seed_net_set)";
+    return GP_ERR_STATE;
+  }
+  float *dx = nullptr, *dt = nullptr;
+  int rc = mlp_upload(ctx, n, x, t, &dx, &dt);
+  if (!rc) {
+    mlp_launch_grad(ctx, dx, dt, n);
+    HIPCHK(hipGetLastError());
+    std::vector<float> pd(MLP_NPAD);
+    float l = 0.f;
+    HIPCHK(hipMemcpyAsync(pd.data(), ctx->mlp_grad, MLP_NPAD * sizeof(float),
+                          hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(&l, ctx->mlp_loss, sizeof(float), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    mlp_unpad(pd.data(), grad);
+    if (loss) *loss = (double)l / (double)n;
+  }
+  if (dx) hipFree(dx);
+  if (dt) hipFree(dt);
+  return rc;
+}
+
+int gp_seed_net_train(gp_ctx* ctx, int64_t n, const float* x, const float* t, int epochs,
+                      int batch, float lr, double* loss) {
+  if (!ctx || n < 1 || !x || !t || epochs < 1 || batch < 1 || !(lr > 0.f)) return GP_ERR_ARG;
+  if (!ctx->mlp_ready) {
+    ctx->err = "the seed network has no parameters (gp_seed_net_set)";
+    return GP_ERR_STATE;
+  }
+  float *dx = nullptr, *dt = nullptr;
+  int rc = mlp_upload(ctx, n, x, t, &dx, &dt);
+  if (!rc) rc = mlp_train_device(ctx, dx, dt, n, epochs, batch, lr, loss);
+  if (dx) hipFree(dx);
+  if (dt) hipFree(dt);
+  return rc;
+}
+
+int gp_seed_net_pairs(gp_ctx* ctx, int64_t n, uint64_t seed, float* x, float* t, int* nfev,
+                      double* poses) {
+  if (!ctx || n < 1) return GP_ERR_ARG;
+  float *dx = nullptr, *dt = nullptr;
+  int* dn = nullptr;
+  double* dp = nullptr;
+  int rc = GP_OK;
+  if (hipMalloc((void**)&dx, (size_t)n * 16 * sizeof(float)) != hipSuccess ||
+      hipMalloc((void**)&dt, (size_t)n * 4 * sizeof(float)) != hipSuccess ||
+      hipMalloc((void**)&dn, (size_t)n * sizeof(int)) != hipSuccess ||
+      (poses && hipMalloc((void**)&dp, (size_t)n * 20 * sizeof(double)) != hipSuccess)) {
+    rc = GP_ERR_NOMEM;
+  } else {
+    hipLaunchKernelGGL(k_mlp_pairs, dim3((unsigned)std::min<long long>((n + 63) / 64, 8192)),
+                       dim3(64), 0, ctx->st, (unsigned long long)seed, 0ll, (long long)n, dx, dt,
+                       dn, dp);
+    if (poses && hipMemcpyAsync(poses, dp, (size_t)n * 20 * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->st) != hipSuccess)
+      rc = GP_ERR_HIP;
+    std::vector<float> hx((size_t)n * 16), ht((size_t)n * 4);
+    if (hipMemcpyAsync(hx.data(), dx, hx.size() * sizeof(float), hipMemcpyDeviceToHost,
+                       ctx->st) != hipSuccess ||
+        hipMemcpyAsync(ht.data(), dt, ht.size() * sizeof(float), hipMemcpyDeviceToHost,
+                       ctx->st) != hipSuccess ||
+        (nfev && hipMemcpyAsync(nfev, dn, (size_t)n * sizeof(int), hipMemcpyDeviceToHost,
+                                ctx->st) != hipSuccess) ||
+        hipStreamSynchronize(ctx->st) != hipSuccess) {
+      ctx->err = "seed-net pair generation failed";
+      rc = GP_ERR_HIP;
+    } else if (!rc) {
+      for (long long k = 0; k < n; ++k) {
+        if (x)
+          for (int c = 0; c < 12; ++c) x[k * 12 + c] = hx[(size_t)k * 16 + c];
+        if (t)
+          for (int c = 0; c < 3; ++c) t[k * 3 + c] = ht[(size_t)k * 4 + c];
+      }
+    }
+  }
+  if (dx) hipFree(dx);
+  if (dt) hipFree(dt);
+  if (dn) hipFree(dn);
+  if (dp) hipFree(dp);
+  return rc;
+}
+
+int gp_seed_net_fit(gp_ctx* ctx, int64_t npairs, uint64_t seed, int epochs, int batch,
+                    float lr, gp_seed_report* rep) {
+  if (!ctx || npairs < 16 || epochs < 1 || batch < 16 || !(lr > 0.f)) return GP_ERR_ARG;
+  if (!ctx->mlp_ready) {
+    ctx->err = "the seed network needs initial parameters (gp_seed_net_set)";
+    return GP_ERR_STATE;
+  }
+  float *dx = nullptr, *dt = nullptr;
+  int* dn = nullptr;
+  int rc = GP_OK;
+  hipEvent_t e0, e1, e2;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventCreate(&e2);
+  if (hipMalloc((void**)&dx, (size_t)npairs * 16 * sizeof(float)) != hipSuccess ||
+      hipMalloc((void**)&dt, (size_t)npairs * 4 * sizeof(float)) != hipSuccess ||
+      hipMalloc((void**)&dn, (size_t)npairs * sizeof(int)) != hipSuccess) {
+    ctx->err = "hipMalloc failed for the training pairs";
+    rc = GP_ERR_NOMEM;
+  } else {
+    hipEventRecord(e0, ctx->st);
+    hipLaunchKernelGGL(k_mlp_pairs,
+                       dim3((unsigned)std::min<long long>((npairs + 63) / 64, 16384)), dim3(64),
+                       0, ctx->st, (unsigned long long)seed, 0ll, (long long)npairs, dx, dt, dn,
+                       nullptr);
+    hipEventRecord(e1, ctx->st);
+    double loss = 0.0;
+    rc = mlp_train_device(ctx, dx, dt, npairs, epochs, batch, lr, &loss);
+    hipEventRecord(e2, ctx->st);
+    if (!rc && hipEventSynchronize(e2) == hipSuccess && rep) {
+      float ms1 = 0.f, ms2 = 0.f;
+      hipEventElapsedTime(&ms1, e0, e1);
+      hipEventElapsedTime(&ms2, e1, e2);
+      std::vector<int> hn((size_t)npairs);
+      hipMemcpy(hn.data(), dn, hn.size() * sizeof(int), hipMemcpyDeviceToHost);
+      double s = 0.0;
+      long long mx = 0;
+      for (int v : hn) { s += v; mx = std::max<long long>(mx, v); }
+      std::memset(rep, 0, sizeof(*rep));
+      rep->npairs = npairs;
+      rep->loss = loss;
+      rep->label_ms = ms1;
+      rep->train_ms = ms2;
+      rep->nfev_ref_mean = s / (double)npairs;
+      rep->nfev_ref_max = mx;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipEventDestroy(e2);
+  if (dx) hipFree(dx);
+  if (dt) hipFree(dt);
+  if (dn) hipFree(dn);
+  return rc;
+}
+
+int gp_seed_net_eval(gp_ctx* ctx, int64_t npairs, uint64_t seed, gp_seed_report* rep) {
+  if (!ctx || npairs < 1 || !rep) return GP_ERR_ARG;
+  if (!ctx->mlp_ready) {
+    ctx->err = "the seed network has no parameters (gp_seed_net_set)";
+    return GP_ERR_STATE;
+  }
+  const size_t n = (size_t)npairs;
+  int *nr = nullptr, *nn = nullptr;
+  unsigned char *fr = nullptr, *fn = nullptr, *fb = nullptr;
+  float* cd = nullptr;
+  int rc = GP_OK;
+  if (hipMalloc((void**)&nr, n * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&nn, n * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&fr, n) != hipSuccess || hipMalloc((void**)&fn, n) != hipSuccess ||
+      hipMalloc((void**)&fb, n) != hipSuccess ||
+      hipMalloc((void**)&cd, n * sizeof(float)) != hipSuccess) {
+    ctx->err = "hipMalloc failed for the evaluation pairs";
+    rc = GP_ERR_NOMEM;
+  } else {
+    SeedEvalOut o{nr, nn, fr, fn, fb, cd};
+    hipLaunchKernelGGL(k_mlp_eval, dim3((unsigned)std::min<long long>((npairs + 63) / 64, 16384)),
+                       dim3(64), 0, ctx->st, (unsigned long long)seed, (long long)npairs,
+                       mlp_dev(ctx), o);
+    std::vector<int> hr(n), hn(n);
+    std::vector<unsigned char> hfr(n), hfn(n), hfb(n);
+    std::vector<float> hcd(n);
+    if (hipMemcpy(hr.data(), nr, n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hn.data(), nn, n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hfr.data(), fr, n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hfn.data(), fn, n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hfb.data(), fb, n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hcd.data(), cd, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+      ctx->err = "seed-net evaluation failed";
+      rc = GP_ERR_HIP;
+    } else {
+      std::memset(rep, 0, sizeof(*rep));
+      rep->npairs = npairs;
+      double sr = 0.0, sn = 0.0, mind = 1.0;
+      for (size_t k = 0; k < n; ++k) {
+        sr += hr[k];
+        sn += hn[k];
+        rep->nfev_ref_max = std::max<int64_t>(rep->nfev_ref_max, hr[k]);
+        rep->nfev_net_max = std::max<int64_t>(rep->nfev_net_max, hn[k]);
+        rep->flags_ref += hfr[k];
+        rep->flags_net += hfn[k];
+        rep->flag_agree += (hfr[k] == hfn[k]) ? 1 : 0;
+        rep->fallbacks += (hfb[k] & 1) ? 1 : 0;
+        rep->no_seed += (hfb[k] & 2) ? 1 : 0;
+        if (hfr[k] && hfn[k]) mind = std::min<double>(mind, hcd[k]);
+      }
+      rep->nfev_ref_mean = sr / (double)n;
+      rep->nfev_net_mean = sn / (double)n;
+      rep->cn_min_dot = mind;
+    }
+  }
+  for (void* b : {(void*)nr, (void*)nn, (void*)fr, (void*)fn, (void*)fb, (void*)cd})
+    if (b) hipFree(b);
+  return rc;
 }
 
 }  // extern "C"

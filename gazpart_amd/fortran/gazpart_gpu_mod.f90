@@ -12,16 +12,16 @@ module gazpart_gpu
     integer(c_int) :: n, interp_scheme, precision, renorm_quat
     real(c_double) :: L, visc, rhof, dtime, ecw, muw, dpmax
     integer(c_int) :: prefilter, xcd_remap
-    integer(c_int) :: keep_fluid, timing, graph
-    integer(c_int) :: reserved(3)
+    integer(c_int) :: keep_fluid, timing, graph, brick, lm_seed
+    integer(c_int) :: reserved(1)
   end type gp_params
 
   type, bind(C) :: gp_stats
     integer(c_int64_t) :: nclose, nprefilter, ncontact, ncollision, lm_nfev, nwall
     integer(c_int64_t) :: greedy_rounds
     real(c_double) :: t_advance_ms, t_sort_ms, t_contact_ms, t_greedy_ms
-    integer(c_int64_t) :: ntimed, nlm
-    integer(c_int64_t) :: reserved(3)
+    integer(c_int64_t) :: ntimed, nlm, seed_fallbacks
+    integer(c_int64_t) :: reserved(2)
   end type gp_stats
 
   interface

@@ -76,6 +76,8 @@ contains
       p%keep_fluid = 0
       p%timing = 0
       p%graph = 0
+      p%brick = 0
+      p%lm_seed = 0
       p%reserved = 0
       GPU_CTX(J) = c_null_ptr
       rc = gp_create(p, GPU_CTX(J))

@@ -84,7 +84,16 @@ typedef struct {
                           buffers are sized from a first eager step with
                           headroom, and an overflow is reported as
                           GP_ERR_NCLOSE_OVERFLOW at the next gp_sync */
-  int reserved[3];
+  int brick;           /* the advance visits particles grouped in bricks of
+                          brick^3 collision cells (compact stencil sets per
+                          workgroup, less field re-fetch); 0: default (8);
+                          < 0: in reference cell order.  No effect on results */
+  int lm_seed;         /* initial guess of the contact solve: 0 the centre
+                          line (reference, ellipsoid_contact_detection.f90:
+                          53-62); 1 the seed network (gp_seed_net_*), with a
+                          centre-line re-solve when the seeded solve ends
+                          unconverged.  The solve decides every contact */
+  int reserved[1];
 } gp_params;
 
 typedef struct {
@@ -102,7 +111,8 @@ typedef struct {
   double t_advance_ms, t_sort_ms, t_contact_ms, t_greedy_ms;
   int64_t ntimed;       /* steps included in the stage times */
   int64_t nlm;          /* pairs that reached the LM (after every prefilter) */
-  int64_t reserved[3];
+  int64_t seed_fallbacks; /* network-seeded solves redone from the centre line */
+  int64_t reserved[2];
 } gp_stats;
 
 /* lifetime ----------------------------------------------------------- */
@@ -167,6 +177,54 @@ int gp_get_stats(gp_ctx *ctx, gp_stats *out);
 int gp_reset_timers(gp_ctx *ctx);
 /* introspection for benchmarks: collision grid of the context           */
 int gp_collision_grid(gp_ctx *ctx, int *nbox_xyz, double *box_min_size);
+
+/* contact-seed network -------------------------------------------------
+ * No reference counterpart: a 12 -> 64 -> 64 -> 3 ReLU network (bf16
+ * weights, fp32 accumulate on the matrix cores) mapping a candidate pair's
+ * relative pose and semi-axes (body frame of the first particle) to the
+ * common normal the contact solve converges to.  It only seeds the solve
+ * (gp_params.lm_seed = 1).  Parameters travel as one float vector:
+ * W1 (64x12), b1 (64), W2 (64x64), b2 (64), W3 (3x64), b3 (3), row-major.
+ * Feature rows: R1^T (b2 - b1)/s, a1/s, a2/s, rotation vector of q1* q2,
+ * s = max(a1) + max(a2).  Targets: the unit normal in body frame 1.      */
+#define GP_SEED_NET_NPARAM 5187
+
+typedef struct {
+  int64_t npairs;
+  double loss;           /* mean over pairs of |y - t|^2 (last epoch) */
+  double label_ms;       /* solver labelling of the training pairs */
+  double train_ms;       /* forward + backward + Adam on the GPU */
+  double nfev_ref_mean;  /* evaluations per solve from the centre line */
+  double nfev_net_mean;  /* ... from the network seed (fallbacks counted) */
+  int64_t nfev_ref_max, nfev_net_max;
+  int64_t flags_ref, flags_net, flag_agree;   /* contact flags per seed */
+  int64_t fallbacks;     /* seeded solves redone from the centre line */
+  int64_t no_seed;       /* pairs where the network gave no usable normal */
+  double cn_min_dot;     /* min |cn_ref . cn_net| over pairs flagged by both */
+  double reserved[4];
+} gp_seed_report;
+
+int gp_seed_net_set(gp_ctx *ctx, const float *theta, int nparam);
+int gp_seed_net_get(gp_ctx *ctx, float *theta, int nparam);
+/* y[n][3] = network(x[n][12]) */
+int gp_seed_net_forward(gp_ctx *ctx, int64_t n, const float *x, float *y);
+/* gradient (same layout as theta) of mean over rows of |y - t|^2 */
+int gp_seed_net_loss_grad(gp_ctx *ctx, int64_t n, const float *x, const float *t,
+                          float *grad, double *loss);
+/* Adam (lr, beta 0.9 / 0.999, eps 1e-8) over caller rows, batches in order */
+int gp_seed_net_train(gp_ctx *ctx, int64_t n, const float *x, const float *t, int epochs,
+                      int batch, float lr, double *loss);
+/* npairs synthetic tri-axial pairs (semi-axis ratios U[1,4], uniform
+ * orientations, centre distance U[0.2,1] x sum of largest semi-axes, not
+ * proven disjoint) labelled by the FP64 solve from the centre line; x, t,
+ * nfev, poses (n x 20: b1, q1, a1, b2, q2, a2) may be NULL */
+int gp_seed_net_pairs(gp_ctx *ctx, int64_t n, uint64_t seed, float *x, float *t, int *nfev,
+                      double *poses);
+/* label npairs such pairs on the GPU and train on them */
+int gp_seed_net_fit(gp_ctx *ctx, int64_t npairs, uint64_t seed, int epochs, int batch,
+                    float lr, gp_seed_report *rep);
+/* solve fresh pairs from both seeds: evaluation counts, flag agreement */
+int gp_seed_net_eval(gp_ctx *ctx, int64_t npairs, uint64_t seed, gp_seed_report *rep);
 
 #ifdef __cplusplus
 }

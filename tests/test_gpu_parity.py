@@ -257,6 +257,26 @@ def test_prefilters_change_no_result(prec):
     assert st[0][0] > st[1][0] > st[2][0]
 
 
+@pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
+def test_visiting_order_changes_no_result(c1, prec):
+    """The advance's visiting order (reference cell order, or bricks of 2,
+    8 or 16 cells a side) decides only where particles are stored: the
+    state after 6 steps is the same bit for bit, dense ensemble and C1."""
+    d = np.load(GOLD + "golden_trajectory.npz")
+    for wl in (golden_workload(d), c1):
+        out = []
+        for brick in (-1, 2, 0, 16):
+            ctx = G.Context.from_workload(wl, precision=prec, brick=brick)
+            ctx.step(3)
+            ctx.adv_particle_ellp(4)
+            ctx.collision(4)
+            ctx.step(2)
+            out.append(ctx.get_particles())
+            ctx.close()
+        for o in out[1:]:
+            np.testing.assert_array_equal(out[0], o)
+
+
 def test_step_equals_adv_plus_collision_and_is_deterministic(c1):
     a = G.Context.from_workload(c1)
     b = G.Context.from_workload(c1)

@@ -1,0 +1,111 @@
+"""The contact-seed network on the GPU (gazpart_amd/csrc/gp_mlp.h) against
+the CPU checker oracle/mlp_ref.py (bf16 emulation), and what it gains.
+
+Tolerances
+  forward      : bf16 operands are identical; fp32 accumulation order may
+                 move an activation across a bf16 rounding boundary, so the
+                 median error is ~1e-7 and the worst a few 1e-3 of the
+                 output scale.
+  gradient     : relative norm error <= 2e-3 per parameter block.
+  Adam step    : a first Adam step is lr * sign(g) where |g| >> 1e-8; at
+                 most 1 % of entries may differ by more than lr / 10
+                 (signs of gradients that are zero up to rounding).
+  labels       : features of the sampled poses match the numpy features to
+                 float32 rounding; the GPU solve's normal matches the C
+                 oracle's (a restatement of ellipsoid_contact_detection.f90)
+                 to 1e-6 for 98 % of pairs (the solves may end on either
+                 side of a tolerance test).
+  gain         : after on-GPU training the seeded solve needs >= 25 % fewer
+                 function evaluations on fresh pairs, and its contact flags
+                 agree with the centre-line solve on >= 99 % of them."""
+import numpy as np
+import pytest
+
+import gazpart_amd as G
+import oracle as O
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = G.Context(16, 2 * np.pi, 0.01, 1.0, 1e-3)
+    yield c
+    c.close()
+
+
+def _data(seed, n):
+    g = np.random.default_rng(seed)
+    X = g.normal(size=(n, M.NIN)).astype(np.float32)
+    T = g.normal(size=(n, M.NOUT)).astype(np.float32)
+    th = M.init_params(seed)
+    th = (th + 0.05 * g.normal(size=th.size)).astype(np.float32)
+    return X, T, th
+
+
+def test_forward_matches_oracle(ctx):
+    X, _, th = _data(1, 1000)
+    ctx.seed_net_set(th)
+    np.testing.assert_array_equal(ctx.seed_net_get(), th)
+    y = ctx.seed_net_forward(X).astype(np.float64)
+    yo, _ = M.forward(th, X.astype(np.float64), "bf16")
+    err = np.abs(y - yo) / np.abs(yo).max()
+    assert np.median(err) < 1e-5, np.median(err)
+    assert err.max() < 5e-3, err.max()
+
+
+def test_gradient_matches_oracle(ctx):
+    X, T, th = _data(2, 777)         # not a multiple of 16: the tail is masked
+    ctx.seed_net_set(th)
+    g, loss = ctx.seed_net_loss_grad(X, T)
+    go, yo = M.backward(th, X.astype(np.float64), T.astype(np.float64), "bf16")
+    lo = float(((yo - T) ** 2).sum() / len(X))
+    assert abs(loss - lo) <= 1e-4 * lo
+    o = 0
+    for name, shp in M.SHAPES:
+        k = int(np.prod(shp))
+        a, b = g[o:o + k].astype(np.float64), go[o:o + k]
+        assert np.linalg.norm(a - b) <= 2e-3 * np.linalg.norm(b), name
+        o += k
+
+
+def test_adam_step_matches_oracle(ctx):
+    X, T, th = _data(3, 512)
+    ctx.seed_net_set(th)
+    go, _ = M.backward(th, X.astype(np.float64), T.astype(np.float64), "bf16")
+    lr = 1e-3
+    ctx.seed_net_train(X, T, epochs=1, batch=512, lr=lr)
+    t1 = ctx.seed_net_get().astype(np.float64)
+    to, _, _ = M.adam(th, np.zeros_like(th), np.zeros_like(th), go.astype(np.float32), 1, lr)
+    bad = np.abs(t1 - to) > 0.1 * lr
+    assert bad.mean() <= 0.01, bad.mean()
+
+
+def test_labelled_pairs_match_oracles(ctx):
+    n = 400
+    x, t, nf, poses = ctx.seed_net_pairs(n, 11)
+    b1, q1, a1 = poses[:, 0:3], poses[:, 3:7], poses[:, 7:10]
+    b2, q2, a2 = poses[:, 10:13], poses[:, 13:17], poses[:, 17:20]
+    np.testing.assert_allclose(x, M.features(b1, q1, a1, b2, q2, a2), rtol=2e-7, atol=1e-6)
+    np.testing.assert_allclose(np.linalg.norm(t, axis=1), 1.0, atol=1e-6)
+    ok = 0
+    for k in range(n):
+        sp1 = np.zeros(O.SP_STRIDE)
+        sp2 = np.zeros(O.SP_STRIDE)
+        sp1[0:3], sp2[0:3] = a1[k], a2[k]
+        _, _, _, cn, nfo, _ = O.contact(sp1, sp2, b1[k], q1[k], b2[k], q2[k])
+        tn = M.target(q1[k:k + 1], cn[None, :])[0]
+        ok += np.abs(tn - t[k]).max() < 1e-6
+    assert ok >= 0.98 * n, ok
+    assert nf.min() >= 1 and nf.max() <= 300
+
+
+def test_trained_network_cuts_solver_evaluations(ctx):
+    ctx.seed_net_set(M.init_params(5))
+    fit = ctx.seed_net_fit(200_000, 21, epochs=4, batch=2048, lr=2e-3)
+    ev = ctx.seed_net_eval(20_000, 99)
+    print("fit", fit, "eval", ev)
+    assert ev["nfev_net_mean"] <= 0.75 * ev["nfev_ref_mean"], ev
+    assert ev["flag_agree"] >= 0.99 * ev["npairs"], ev
+    assert ev["fallbacks"] <= 0.01 * ev["npairs"], ev
