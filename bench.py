@@ -2,24 +2,35 @@
 """Benchmark of the MI355X Lagrangian ellipsoid step (interp + advance +
 collide) -- ellipsoid updates/s, mixed form vs the plain FP64 form.
 
-Workload (default): 256^3 field from random Fourier modes (k0 = 4, u_rms = 1,
-seeded), 1,000,000 spheroids with aspect ratio log-uniform in [0.2, 5] at
-volume fraction 1e-3, Stokes number log-uniform in [0.1, 10], lag3, dt at
-CFL 0.1, collisions on with the reference cell rule.  One step = one pass of
-gather + RK2/drag/torque/quaternion/walls/wrap + cell list + contact solve +
-ordered collision resolution over all particles; updates/s = Np * steps /
-time of the timed steps (inputs resident in HBM, setup excluded).
+Default workload (C2): 256^3 field from random Fourier modes (k0 = 4,
+u_rms = 1, seeded), 1,000,000 spheroids with aspect ratio log-uniform in
+[0.2, 5] at volume fraction 1e-3, Stokes number log-uniform in [0.1, 10],
+lag3, dt at CFL 0.1, collisions on with the reference cell rule.  One step
+= one pass of gather + RK2/drag/torque/quaternion/walls/wrap + cell list +
+contact solve + ordered collision resolution over all particles;
+updates/s = Np * steps / time of the timed steps (inputs resident in HBM,
+setup excluded).
 
 value        = updates/s of the mixed form (FP32 field, gather and drag;
                FP64 positions, quaternions, contact solve and impulses)
 fp64         = the same for the plain FP64 form
 roofline     = the fused advance kernel (the HBM-bound gather): algorithmic
                bytes per launch / its average duration (HIP events on the
-               library's stream over the timed steps) against 8 TB/s
-cpu_baseline = the FP64 CPU oracle (a restatement of the reference path)
-               on the host cores, one full step of the same workload
+               library's stream over timed eager steps) against 8 TB/s;
+               traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE +
+               WRITE_SIZE scaled by a copy calibration, when a summary for
+               this configuration is committed
+cpu_baseline = the FP64 CPU oracle (a restatement of the reference path,
+               OpenMP over particles) on the host cores, one full step of
+               the same workload (C2), or the compiled reference Fortran on
+               one core (C1)
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--np N] [--n N]
+Other configurations (--config): C1 (64^3, 10k prolate spheroids), C2L4
+(C2 with lag4), C3 (512^3 field, 8M tri-axial), C4 (64 ensembles on one
+field, autotuned launch shapes), C5 (2M tri-axial at 5 % volume fraction,
+network-seeded contact solve trained on the GPU).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C2]
 """
 from __future__ import annotations
 
@@ -51,10 +62,17 @@ def advance_bytes(np_, n, field_bytes, taps, s_rw, s_par=16):
     return np_ * (s_rw + s_par) + 6.0 * n ** 3 * field_bytes * touched
 
 
-def run_gpu(wl, precision, steps, warmup, G, brick=0):
+def run_gpu(wl, precision, steps, warmup, G, groups=None, theta=None, **kw):
     """Timed steps of the captured two-step hipGraph, then a separate eager
     run with HIP-event stage timers (for the per-kernel figures)."""
-    ctx = G.Context.from_workload(wl, precision=precision, graph=True, brick=brick)
+    def make(**extra):
+        c = G.Context.from_workload(wl, precision=precision, **kw, **extra)
+        if groups is not None:
+            c.set_groups(groups)
+        if theta is not None:
+            c.seed_net_set(theta)
+        return c
+    ctx = make(graph=True)
     ctx.step(warmup)
     ctx.sync()
     t0 = time.perf_counter()
@@ -64,25 +82,137 @@ def run_gpu(wl, precision, steps, warmup, G, brick=0):
     st = ctx.stats()
     state = ctx.get_particles()
     ctx.close()
-    tctx = G.Context.from_workload(wl, precision=precision, timing=True, brick=brick)
+    tctx = make(timing=True)
     tctx.step(warmup)
     tctx.reset_timers()
-    tctx.step(min(steps, 10))
+    nt = min(steps, 10)
+    nfev = nlm = ncoll = 0
+    for _ in range(nt):
+        tctx.step(1)
+        s = tctx.stats()
+        nfev += s["lm_nfev"]
+        nlm += s["nlm"]
+        ncoll += s["ncollision"]
     tctx.sync()
     tst = tctx.stats()
     tctx.close()
     for k in ("t_advance_ms", "t_sort_ms", "t_contact_ms", "t_greedy_ms", "ntimed"):
         st[k] = tst[k]
+    st["per_step"] = {"lm_nfev": nfev / nt, "nlm": nlm / nt, "ncollision": ncoll / nt}
     return (t1 - t0), st, state
 
 
-def cpu_baseline(wl, threads):
+def cpu_baseline_port(wl, threads):
     import oracle as O
     stp = O.Stepper(wl, nthreads=threads, nclosemax=max(64 * wl.np_, 1 << 20))
     t0 = time.perf_counter()
     st, err, nc, ncoll = stp.step(wl.state.copy())
     dt = time.perf_counter() - t0
-    return wl.np_ / dt, dt, err
+    return {"value": round(wl.np_ / dt, 1), "unit": "updates/s", "cores": threads, "kind": "port",
+            "sample": f"one full step (interp+advance+cell list+LM contact+collisions) of the "
+                      f"same {wl.np_} ellipsoids on the {wl.n}^3 field, FP64 CPU oracle with "
+                      f"{threads} OpenMP threads, {dt:.1f}s (err {err})"}
+
+
+def cpu_baseline_reference(wl, max_seconds=20.0):
+    """The reference's own ADV_PARTICLE_ELLP + COLLISION (compiled Fortran,
+    one rank, one core) on the same single-class workload; as many steps as
+    fit in max_seconds."""
+    import oracle as O
+    from gazpart_amd import workload as W
+    if not O.ref_available():
+        return None
+    sp = wl.species[0]
+    R = O.Ref()
+    R.setup(wl.n, wl.L, wl.visc, wl.rhof, wl.dtime, wl.ecw, wl.muw, wl.scheme,
+            [[sp[W.SP_EMAJ]]], [sp[W.SP_APR]], [[sp[W.SP_RHOP]]])
+    R.set_field(wl.field)
+    R.set_particles(1, wl.state, np.ones(wl.np_, dtype=np.int32))
+    R.collision_init()
+    t0 = time.perf_counter()
+    s = 0
+    while s < wl.steps and time.perf_counter() - t0 < max_seconds:
+        s += 1
+        R.adv(s)
+        R.collision(1, s, 0.0, 200 * wl.np_)
+    dt = time.perf_counter() - t0
+    return {"value": round(wl.np_ * s / dt, 1), "unit": "updates/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"{s} steps of the same {wl.np_} ellipsoids on the {wl.n}^3 field through "
+                      f"the reference's own Fortran ADV_PARTICLE_ELLP + COLLISION (amdflang, one "
+                      f"rank, one core), {dt:.1f}s"}
+
+
+def roofline(wl, res, scheme, cfg):
+    roof = {}
+    for name, fb, srw in (("fp64", 8, 208), ("mixed", 4, 160)):
+        if name not in res:
+            continue
+        st = res[name]["stats"]
+        t_adv = st["t_advance_ms"] / max(st["ntimed"], 1) * 1e-3
+        byt = advance_bytes(wl.np_, wl.n, fb, 64 if scheme == 3 else 125, srw)
+        ach = byt / t_adv / 1e9
+        roof[name] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                      "kernel": "k_advance", "bytes_per_launch": int(byt),
+                      "avg_launch_ms": round(t_adv * 1e3, 4)}
+    tr_file = os.path.join(ROOT, "profiles", f"traffic_k_advance_{cfg}.json")
+    if os.path.exists(tr_file):
+        tr = json.load(open(tr_file))
+        for name in roof:
+            if name in tr:
+                roof[name]["traffic"] = tr[name]
+    return roof
+
+
+def stage(st):
+    n = max(st["ntimed"], 1)
+    return {k: round(st[f"t_{k}_ms"] / n, 4) for k in ("advance", "sort", "contact", "greedy")}
+
+
+def relerr(a, b):
+    out = {}
+    for q, sl in (("x", slice(0, 3)), ("u", slice(3, 6)), ("q", slice(6, 10)),
+                  ("omega", slice(10, 13))):
+        out[q] = float(np.abs(a[sl] - b[sl]).max() / np.abs(b[sl]).max())
+    return out
+
+
+def describe(cfg, wl, scheme):
+    lag = f"lag{scheme}"
+    if cfg == "C1":
+        return (f"C1: {wl.n}^3 random-Fourier field (k0=4, u_rms=1), {wl.np_} prolate spheroids "
+                f"APR 2, a=0.2 DX, St 1, {lag}, CFL 0.1, collisions on (reference cell rule)")
+    if cfg in ("C2", "C2L4"):
+        return (f"C2: {wl.n}^3 random-Fourier field (k0=4, u_rms=1), {wl.np_} spheroids APR "
+                f"log-uniform [0.2,5], phi=1e-3, St log-uniform [0.1,10], {lag}, CFL 0.1, "
+                f"collisions on (reference cell rule)")
+    if cfg == "C3":
+        return (f"C3: {wl.n}^3 random-Fourier field (k0=8) stored FP32 in the mixed form, "
+                f"{wl.np_} tri-axial ellipsoids (a/b, a/c uniform [1,4]), phi=1e-3 (assumed), "
+                f"St 1, {lag}")
+    if cfg == "C4":
+        return (f"C4: {len(wl.meta.get('ranges', []))} independent ensembles ({wl.np_} "
+                f"ellipsoids, counts log-uniform [1e3,5e5]) on one shared {wl.n}^3 field, "
+                f"per-ensemble APR [0.2,5] and St [0.1,10], one captured graph, autotuned "
+                f"launch shapes")
+    return (f"C5: {wl.n}^3 field, {wl.np_} tri-axial ellipsoids (a/b, a/c uniform [1,4]) at "
+            f"phi=5%, St 1, {lag}, contact solve seeded by the on-GPU-trained network")
+
+
+def make_workload(cfg, args, W):
+    scheme = args.scheme or (4 if cfg == "C2L4" else 3)
+    if cfg == "C1":
+        return W.c1(np_=args.np or 10_000, n=args.n or 64, steps=100, scheme=scheme), scheme
+    if cfg in ("C2", "C2L4"):
+        return W.build_workload("C2", args.n or 256, args.np or 1_000_000, k0=4.0, seed=2024,
+                                scheme=scheme, steps=200, apr=(0.2, 5.0), phi=1e-3,
+                                st=(0.1, 10.0)), scheme
+    if cfg == "C3":
+        return W.c3(n=args.n or 512, np_=args.np or 8_000_000, scheme=scheme), scheme
+    if cfg == "C4":
+        return W.c4(n=args.n or 256), scheme
+    return W.c5(n=args.n or 256, np_=args.np or 2_000_000, scheme=scheme), scheme
 
 
 def main():
@@ -90,14 +220,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--np", type=int, default=1_000_000)
-    ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--scheme", type=int, default=3)
+    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C2L4", "C3", "C4", "C5"])
+    ap.add_argument("--np", type=int, default=0)
+    ap.add_argument("--n", type=int, default=0)
+    ap.add_argument("--scheme", type=int, default=0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--brick", type=int, default=0,
                     help="advance visiting order: bricks of BRICK^3 collision cells "
                          "(0: the library default)")
+    ap.add_argument("--pairs", type=int, default=10_000_000,
+                    help="C5: solver-labelled pairs the seed network is trained on")
+    ap.add_argument("--epochs", type=int, default=3, help="C5: training epochs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (torch), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profile")
@@ -121,58 +255,54 @@ def main():
     import gazpart_amd as G
     from gazpart_amd import workload as W
 
+    cfg = args.config
     t0 = time.perf_counter()
-    wl = W.build_workload("C2", args.n, args.np, k0=4.0, seed=2024, scheme=args.scheme,
-                          steps=args.steps, apr=(0.2, 5.0), phi=1e-3, st=(0.1, 10.0))
-    log(f"workload built in {time.perf_counter() - t0:.1f}s: n={wl.n} np={wl.np_} "
+    wl, scheme = make_workload(cfg, args, W)
+    log(f"{cfg} workload built in {time.perf_counter() - t0:.1f}s: n={wl.n} np={wl.np_} "
         f"dt={wl.dtime:.3e} dpmax={wl.dpmax:.3e}")
+    groups = wl.meta.get("group") if cfg == "C4" else None
+
+    kw = {"brick": args.brick}
+    extra = {}
+    theta = None
+    if cfg == "C4":
+        from gazpart_amd.autotune import tune
+        tr = tune(wl, G.PREC_MIXED, groups=groups, log=log)
+        kw = {"brick": tr["best"]["brick"], "lm_waves": tr["best"]["lm_waves"]}
+        extra["autotune"] = tr
+    if cfg == "C5":
+        from oracle import mlp_ref as M
+        nctx = G.Context(wl.n, wl.L, wl.visc, wl.rhof, wl.dtime)
+        nctx.seed_net_set(M.init_params(7))
+        fit = nctx.seed_net_fit(args.pairs, 2024, epochs=args.epochs, batch=4096, lr=2e-3)
+        ev = nctx.seed_net_eval(200_000, 77)
+        theta = nctx.seed_net_get()
+        nctx.close()
+        log(f"seed network: fit {fit}\n  eval {ev}")
+        extra["seed_net"] = {"fit": fit, "eval": ev}
 
     res = {}
-    for name, prec in (("fp64", G.PREC_FP64), ("mixed", G.PREC_MIXED)):
-        dt, st, state = run_gpu(wl, prec, args.steps, args.warmup, G, args.brick)
+    runs = [("fp64", G.PREC_FP64, {}), ("mixed", G.PREC_MIXED, {})]
+    if cfg == "C5":
+        runs = [("fp64", G.PREC_FP64, {}), ("mixed_centre_seed", G.PREC_MIXED, {}),
+                ("mixed", G.PREC_MIXED, {"lm_seed": 1, "theta": theta})]
+    for name, prec, ek in runs:
+        dt, st, state = run_gpu(wl, prec, args.steps, args.warmup, G, groups, **kw, **ek)
         ms = 1e3 * dt / args.steps
         res[name] = dict(seconds=dt, ms_per_step=ms, stats=st, state=state,
                          updates_per_s=wl.np_ * args.steps / dt)
         log(f"{name}: {ms:.3f} ms/step, {wl.np_ * args.steps / dt:.3e} updates/s, stats {st}")
 
-    # mixed vs FP64 after the same steps from the same state (warmup + timed)
-    a, b = res["mixed"]["state"], res["fp64"]["state"]
-    relerr = {}
-    for q, sl in (("x", slice(0, 3)), ("u", slice(3, 6)), ("q", slice(6, 10)),
-                  ("omega", slice(10, 13))):
-        relerr[q] = float(np.abs(a[sl] - b[sl]).max() / np.abs(b[sl]).max())
-
-    # roofline of the fused advance kernel
-    roof = {}
-    for name, fb, srw in (("fp64", 8, 208), ("mixed", 4, 160)):
-        st = res[name]["stats"]
-        t_adv = st["t_advance_ms"] / max(st["ntimed"], 1) * 1e-3
-        byt = advance_bytes(wl.np_, wl.n, fb, 64 if args.scheme == 3 else 125, srw)
-        ach = byt / t_adv / 1e9
-        roof[name] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                      "traffic": None, "kernel": "k_advance",
-                      "bytes_per_launch": int(byt), "avg_launch_ms": round(t_adv * 1e3, 4)}
-    tr_file = os.path.join(ROOT, "profiles", "traffic_k_advance_C2.json")
-    if os.path.exists(tr_file):
-        tr = json.load(open(tr_file))
-        for name in roof:
-            if name in tr:
-                roof[name]["traffic"] = tr[name]
-
+    roof = roofline(wl, res, scheme, cfg)
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        ups, secs, err = cpu_baseline(wl, threads)
-        cpu = {"value": round(ups, 1), "unit": "updates/s", "cores": threads, "kind": "port",
-               "sample": f"one full step (interp+advance+cell list+LM contact+collisions) of "
-                         f"the same {wl.np_} ellipsoids on the {wl.n}^3 field, FP64 CPU "
-                         f"oracle with {threads} OpenMP threads, {secs:.1f}s"}
-        log(f"cpu baseline: {ups:.3e} updates/s on {threads} threads ({secs:.1f}s), err {err}")
-
-    def stage(st):
-        n = max(st["ntimed"], 1)
-        return {k: round(st[f"t_{k}_ms"] / n, 4) for k in ("advance", "sort", "contact", "greedy")}
+        if cfg == "C1":
+            cpu = cpu_baseline_reference(wl)
+        if cfg in ("C1", "C2", "C2L4") and cpu is None:
+            threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+            cpu = cpu_baseline_port(wl, threads)
+        if cpu:
+            log(f"cpu baseline: {cpu}")
 
     m, f = res["mixed"], res["fp64"]
     line = {
@@ -188,21 +318,28 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 gather/drag + f64 positions/quaternion/contact",
         "data": "synthetic",
-        "config": {"workload": f"C2: {wl.n}^3 random-Fourier field (k0=4, u_rms=1), "
-                               f"{wl.np_} spheroids APR log-uniform [0.2,5], phi=1e-3, "
-                               f"St log-uniform [0.1,10], lag{args.scheme - 0}, CFL 0.1, "
-                               f"collisions on (reference cell rule)",
-                   "np": wl.np_, "grid": wl.n, "interp": f"lag{args.scheme}"},
+        "config": {"workload": describe(cfg, wl, scheme), "np": wl.np_, "grid": wl.n,
+                   "interp": f"lag{scheme}", "launch": kw},
         "fp64": {"value": round(f["updates_per_s"], 1), "ms_per_step": round(f["ms_per_step"], 4),
-                 "dtype": "f64", "roofline": roof["fp64"], "stage_ms": stage(f["stats"])},
-        "mixed_vs_fp64_relerr": relerr,
+                 "dtype": "f64", "roofline": roof["fp64"], "stage_ms": stage(f["stats"]),
+                 "per_step": f["stats"]["per_step"]},
+        "mixed_vs_fp64_relerr": relerr(m["state"], f["state"]),
         "speedup_mixed_over_fp64": round(m["updates_per_s"] / f["updates_per_s"], 3),
         "stage_ms": stage(m["stats"]),
+        "per_step": m["stats"]["per_step"],
         "collisions_per_step": m["stats"]["ncollision"],
         "candidate_pairs": m["stats"]["nclose"],
         "roofline": roof["mixed"],
         "cpu_baseline": cpu,
     }
+    if cfg == "C5":
+        c = res["mixed_centre_seed"]
+        line["mixed_centre_seed"] = {"value": round(c["updates_per_s"], 1),
+                                     "ms_per_step": round(c["ms_per_step"], 4),
+                                     "stage_ms": stage(c["stats"]),
+                                     "per_step": c["stats"]["per_step"]}
+        line["seeded_vs_centre_seed_relerr"] = relerr(m["state"], c["state"])
+    line.update(extra)
     print(json.dumps(line))
 
 

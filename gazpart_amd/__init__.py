@@ -65,7 +65,7 @@ class Params(C.Structure):
                 ("muw", C.c_double), ("dpmax", C.c_double), ("prefilter", C.c_int),
                 ("xcd_remap", C.c_int), ("keep_fluid", C.c_int), ("timing", C.c_int),
                 ("graph", C.c_int), ("brick", C.c_int), ("lm_seed", C.c_int),
-                ("reserved", C.c_int * 1)]
+                ("lm_waves", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -151,7 +151,8 @@ class Context:
 
     def __init__(self, n, L, visc, rhof, dtime, *, scheme=3, precision=PREC_FP64, ecw=1.0,
                  muw=0.0, dpmax=0.0, prefilter=2, xcd_remap=1, renorm_quat=0,
-                 keep_fluid=False, timing=False, graph=False, brick=0, lm_seed=0):
+                 keep_fluid=False, timing=False, graph=False, brick=0, lm_seed=0,
+                 lm_waves=0):
         self.lib = load_library()
         p = Params()
         p.n, p.interp_scheme, p.precision, p.renorm_quat = n, scheme, precision, renorm_quat
@@ -162,6 +163,7 @@ class Context:
         p.graph = 1 if graph else 0
         p.brick = brick
         p.lm_seed = lm_seed
+        p.lm_waves = lm_waves
         self.params = p
         h = C.c_void_p()
         rc = self.lib.gp_create(C.byref(p), C.byref(h))

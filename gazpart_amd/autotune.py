@@ -1,0 +1,53 @@
+"""Autotuner for the launch shapes that leave every result unchanged.
+
+Two knobs of :class:`gazpart_amd.Context` decide only how the work is laid
+out on the GPU, never what is computed:
+
+* ``brick`` -- the advance visits particles grouped in bricks of brick^3
+  collision cells (compact sets of stencils per workgroup) or, for values
+  <= 0, in the reference's cell order.  The collision cell itself stays at
+  the reference rule (collision_initiation.f90:50-102) because it fixes
+  the candidate-pair order that the one-collision-per-particle pass follows.
+* ``lm_waves`` -- the width of the contact-solve launch (one pair per lane,
+  pairs dealt lane-major over the wavefronts); 0 means one wavefront per
+  SIMD.
+
+:func:`tune` times a few captured (hipGraph) steps of the caller's own
+workload from the same initial state for every combination and returns
+the fastest, with the whole table.  The GPU tests check that the knobs do
+not change results.
+"""
+from __future__ import annotations
+
+import time
+
+from . import Context
+
+BRICKS = (-1, 2, 4, 8)
+LM_WAVES = (0, 2048, 4096)
+
+
+def tune(wl, precision, *, steps: int = 6, warmup: int = 2, bricks=BRICKS, lm_waves=LM_WAVES,
+         groups=None, log=None, **ctx_kw) -> dict:
+    """Returns {"best": {...}, "table": [...]}; each entry has brick,
+    lm_waves and ms_per_step (graph-mode steps after `warmup` steps)."""
+    table = []
+    for b in bricks:
+        for w in lm_waves:
+            ctx = Context.from_workload(wl, precision=precision, graph=True, brick=b,
+                                        lm_waves=w, **ctx_kw)
+            if groups is not None:
+                ctx.set_groups(groups)
+            ctx.step(warmup)
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.step(steps)
+            ctx.sync()
+            dt = time.perf_counter() - t0
+            ctx.close()
+            row = {"brick": b, "lm_waves": w, "ms_per_step": round(1e3 * dt / steps, 4)}
+            table.append(row)
+            if log:
+                log(f"autotune: {row}")
+    best = min(table, key=lambda r: r["ms_per_step"])
+    return {"best": best, "table": table}

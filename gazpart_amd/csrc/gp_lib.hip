@@ -1415,7 +1415,7 @@ int gp_create(const gp_params* p, gp_ctx** out) {
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
             hipSuccess || ncu < 1)
       ncu = 256;
-    ctx->lm_waves = 4 * ncu;   // one wavefront per SIMD
+    ctx->lm_waves = p->lm_waves > 0 ? p->lm_waves : 4 * ncu;   // default: one per SIMD
   }
   StepParams& s = ctx->sp;
   s.n = p->n;

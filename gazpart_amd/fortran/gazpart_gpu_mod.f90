@@ -12,8 +12,7 @@ module gazpart_gpu
     integer(c_int) :: n, interp_scheme, precision, renorm_quat
     real(c_double) :: L, visc, rhof, dtime, ecw, muw, dpmax
     integer(c_int) :: prefilter, xcd_remap
-    integer(c_int) :: keep_fluid, timing, graph, brick, lm_seed
-    integer(c_int) :: reserved(1)
+    integer(c_int) :: keep_fluid, timing, graph, brick, lm_seed, lm_waves
   end type gp_params
 
   type, bind(C) :: gp_stats

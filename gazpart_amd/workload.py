@@ -318,6 +318,25 @@ def build_ensembles(name: str, n: int, counts, *, k0: float = 4.0, seed: int = 7
                     meta=dict(group=grp, ranges=ranges, apr=aprs, st=sts, tau_flow=tau_flow))
 
 
+def c3(n: int = 512, np_: int = 8_000_000, steps: int = 500, seed: int = 3072,
+       scheme: int = 3) -> Workload:
+    """Precision: 512^3 field (k0 = 8) for FP32 storage, 8,000,000 tri-axial
+    ellipsoids with semi-axis ratios a/b, a/c uniform in [1, 4]; the
+    volume fraction is not given for this configuration, 1e-3 is assumed
+    (as C2); St = 1."""
+    return build_workload("C3", n, np_, k0=8.0, seed=seed, scheme=scheme, steps=steps,
+                          phi=1e-3, st=1.0, triaxial_ratio=(1.0, 4.0), emaj_dx=None)
+
+
+def c5(n: int = 256, np_: int = 2_000_000, steps: int = 100, seed: int = 5120,
+       scheme: int = 3) -> Workload:
+    """Learned contact seed: 2,000,000 tri-axial ellipsoids (semi-axis ratios
+    uniform in [1, 4]) at volume fraction 5 % on 256^3 (collision-heavy),
+    St = 1."""
+    return build_workload("C5", n, np_, k0=4.0, seed=seed, scheme=scheme, steps=steps,
+                          phi=0.05, st=1.0, triaxial_ratio=(1.0, 4.0), emaj_dx=None)
+
+
 def c4(n: int = 256, nens: int = 64, steps: int = 100, seed: int = 4096,
        field_dtype=np.float64) -> Workload:
     """Batching: nens independent ensembles on one shared 256^3 field,

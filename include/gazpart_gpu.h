@@ -93,7 +93,8 @@ typedef struct {
                           53-62); 1 the seed network (gp_seed_net_*), with a
                           centre-line re-solve when the seeded solve ends
                           unconverged.  The solve decides every contact */
-  int reserved[1];
+  int lm_waves;        /* wavefronts of the contact-solve launch (one pair per
+                          lane, pairs dealt lane-major); 0: one per SIMD */
 } gp_params;
 
 typedef struct {
