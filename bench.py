@@ -63,8 +63,9 @@ def advance_bytes(np_, n, field_bytes, taps, s_rw, s_par=16):
 
 
 def run_gpu(wl, precision, steps, warmup, G, groups=None, theta=None, **kw):
-    """Timed steps of the captured two-step hipGraph, then a separate eager
-    run with HIP-event stage timers (for the per-kernel figures)."""
+    """Timed steps of the captured hipGraph (8 steps, the next advance of the
+    particles outside the contact solves beside those solves), then a
+    separate eager run with HIP-event stage timers (per-kernel figures)."""
     def make(**extra):
         c = G.Context.from_workload(wl, precision=precision, **kw, **extra)
         if groups is not None:
@@ -81,6 +82,7 @@ def run_gpu(wl, precision, steps, warmup, G, groups=None, theta=None, **kw):
     t1 = time.perf_counter()
     st = ctx.stats()
     state = ctx.get_particles()
+    st["collision_counts"] = ctx.collision_counts()
     ctx.close()
     tctx = make(timing=True)
     tctx.step(warmup)
@@ -170,11 +172,29 @@ def stage(st):
     return {k: round(st[f"t_{k}_ms"] / n, 4) for k in ("advance", "sort", "contact", "greedy")}
 
 
-def relerr(a, b):
+QUANT = (("x", slice(0, 3)), ("u", slice(3, 6)), ("q", slice(6, 10)), ("omega", slice(10, 13)))
+
+
+def relerr(a, b, ca, cb):
+    """Mixed (a) against FP64 (b) after the same steps: per-quantity
+    max |a - b| / max |b| over the particles that took part in no collision
+    in either run (the advance alone: gather, drag, torque, quaternion,
+    walls), and separately over those that did.  The reference's contact
+    solve stops on ftol = 1.5e-8 with a non-zero residual, so the common
+    normal it returns is fixed only to ~1e-4 and moves with 1e-10 changes of
+    its inputs; collided particles carry that, whatever the precision.
+    Also: particles whose number of collisions differs between the runs."""
+    free = (ca == 0) & (cb == 0)
     out = {}
-    for q, sl in (("x", slice(0, 3)), ("u", slice(3, 6)), ("q", slice(6, 10)),
-                  ("omega", slice(10, 13))):
-        out[q] = float(np.abs(a[sl] - b[sl]).max() / np.abs(b[sl]).max())
+    for q, sl in QUANT:
+        scale = np.abs(b[sl]).max()
+        out[q] = float(np.abs(a[sl][:, free] - b[sl][:, free]).max() / scale)
+    hit = ~free
+    out["collided_particles"] = int(hit.sum())
+    if hit.any():
+        out["collided_max"] = {q: float(np.abs(a[sl][:, hit] - b[sl][:, hit]).max() /
+                                        np.abs(b[sl]).max()) for q, sl in QUANT}
+    out["collision_count_mismatch"] = int((ca != cb).sum())
     return out
 
 
@@ -291,7 +311,8 @@ def main():
         ms = 1e3 * dt / args.steps
         res[name] = dict(seconds=dt, ms_per_step=ms, stats=st, state=state,
                          updates_per_s=wl.np_ * args.steps / dt)
-        log(f"{name}: {ms:.3f} ms/step, {wl.np_ * args.steps / dt:.3e} updates/s, stats {st}")
+        shown = {k: v for k, v in st.items() if k != "collision_counts"}
+        log(f"{name}: {ms:.3f} ms/step, {wl.np_ * args.steps / dt:.3e} updates/s, stats {shown}")
 
     roof = roofline(wl, res, scheme, cfg)
     cpu = None
@@ -323,7 +344,8 @@ def main():
         "fp64": {"value": round(f["updates_per_s"], 1), "ms_per_step": round(f["ms_per_step"], 4),
                  "dtype": "f64", "roofline": roof["fp64"], "stage_ms": stage(f["stats"]),
                  "per_step": f["stats"]["per_step"]},
-        "mixed_vs_fp64_relerr": relerr(m["state"], f["state"]),
+        "mixed_vs_fp64_relerr": relerr(m["state"], f["state"], m["stats"]["collision_counts"],
+                                       f["stats"]["collision_counts"]),
         "speedup_mixed_over_fp64": round(m["updates_per_s"] / f["updates_per_s"], 3),
         "stage_ms": stage(m["stats"]),
         "per_step": m["stats"]["per_step"],
@@ -338,7 +360,9 @@ def main():
                                      "ms_per_step": round(c["ms_per_step"], 4),
                                      "stage_ms": stage(c["stats"]),
                                      "per_step": c["stats"]["per_step"]}
-        line["seeded_vs_centre_seed_relerr"] = relerr(m["state"], c["state"])
+        line["seeded_vs_centre_seed_relerr"] = relerr(m["state"], c["state"],
+                                                      m["stats"]["collision_counts"],
+                                                      c["stats"]["collision_counts"])
     line.update(extra)
     print(json.dumps(line))
 

@@ -45,10 +45,10 @@ EXPORTS = (
     "gp_set_species", "gp_set_particles", "gp_set_groups", "gp_get_particles", "gp_get_fluid_at_particles",
     "gp_interph", "gp_interph_field", "gp_adv_particle_ellp", "gp_collision_detection", "gp_contact_batch",
     "gp_collision", "gp_step", "gp_get_stats", "gp_reset_timers", "gp_collision_grid",
-    "gp_seed_net_set", "gp_seed_net_get", "gp_seed_net_forward", "gp_seed_net_loss_grad",
+    "gp_get_collision_counts", "gp_seed_net_set", "gp_seed_net_get", "gp_seed_net_forward", "gp_seed_net_loss_grad",
     "gp_seed_net_train", "gp_seed_net_pairs", "gp_seed_net_fit", "gp_seed_net_eval",
 )
-SEED_NET_NPARAM = 5187
+SEED_NET_NPARAM = 5252
 
 
 class GpError(RuntimeError):
@@ -65,7 +65,7 @@ class Params(C.Structure):
                 ("muw", C.c_double), ("dpmax", C.c_double), ("prefilter", C.c_int),
                 ("xcd_remap", C.c_int), ("keep_fluid", C.c_int), ("timing", C.c_int),
                 ("graph", C.c_int), ("brick", C.c_int), ("lm_seed", C.c_int),
-                ("lm_waves", C.c_int)]
+                ("lm_waves", C.c_int), ("overlap", C.c_int), ("reserved", C.c_int * 3)]
 
 
 class Stats(C.Structure):
@@ -127,6 +127,7 @@ def load_library(path: str = LIB_PATH):
     L.gp_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.gp_collision_grid.argtypes = [vp, ip, dp]
     L.gp_reset_timers.argtypes = [vp]
+    L.gp_get_collision_counts.argtypes = [vp, C.c_int, ip, C.c_int]
     L.gp_seed_net_set.argtypes = [vp, vp, C.c_int]
     L.gp_seed_net_get.argtypes = [vp, vp, C.c_int]
     L.gp_seed_net_forward.argtypes = [vp, C.c_int64, vp, vp]
@@ -152,7 +153,7 @@ class Context:
     def __init__(self, n, L, visc, rhof, dtime, *, scheme=3, precision=PREC_FP64, ecw=1.0,
                  muw=0.0, dpmax=0.0, prefilter=2, xcd_remap=1, renorm_quat=0,
                  keep_fluid=False, timing=False, graph=False, brick=0, lm_seed=0,
-                 lm_waves=0):
+                 lm_waves=0, overlap=1):
         self.lib = load_library()
         p = Params()
         p.n, p.interp_scheme, p.precision, p.renorm_quat = n, scheme, precision, renorm_quat
@@ -164,6 +165,7 @@ class Context:
         p.brick = brick
         p.lm_seed = lm_seed
         p.lm_waves = lm_waves
+        p.overlap = overlap
         self.params = p
         h = C.c_void_p()
         rc = self.lib.gp_create(C.byref(p), C.byref(h))
@@ -280,6 +282,13 @@ class Context:
         """COLLISION (collision.f90:7) at NPROC = 1."""
         self._chk(self.lib.gp_collision(self.h, ncycle, time))
 
+    def collision_counts(self, reset=False):
+        """Collisions each particle took part in (reference order) since
+        set_particles or the last reset."""
+        out = np.empty(self.np, dtype=np.int32)
+        self._chk(self.lib.gp_get_collision_counts(self.h, self.np, _ptr(out), 1 if reset else 0))
+        return out
+
     def step(self, nsteps=1):
         """nsteps x (ADV_PARTICLE_ELLP + COLLISION), asynchronous."""
         self._chk(self.lib.gp_step(self.h, nsteps))
@@ -325,7 +334,7 @@ class Context:
 
     def seed_net_forward(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)
-        y = np.empty((x.shape[0], 3), dtype=np.float32)
+        y = np.empty((x.shape[0], 4), dtype=np.float32)
         self._chk(self.lib.gp_seed_net_forward(self.h, x.shape[0], _ptr(x), _ptr(y)))
         return y
 
@@ -347,10 +356,11 @@ class Context:
         return loss.value
 
     def seed_net_pairs(self, n, seed):
-        """(features (n,12), targets (n,3), evaluations (n,), poses (n,20):
-        b1, q1, a1, b2, q2, a2) of synthetic solver-labelled pairs."""
+        """(features (n,12), targets (n,4): normal and flag, evaluations (n,),
+        poses (n,20): b1, q1, a1, b2, q2, a2) of synthetic solver-labelled
+        pairs."""
         x = np.empty((n, 12), dtype=np.float32)
-        t = np.empty((n, 3), dtype=np.float32)
+        t = np.empty((n, 4), dtype=np.float32)
         nf = np.empty(n, dtype=np.int32)
         poses = np.empty((n, 20))
         self._chk(self.lib.gp_seed_net_pairs(self.h, n, seed, _ptr(x), _ptr(t), _ptr(nf),

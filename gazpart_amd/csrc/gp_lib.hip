@@ -369,17 +369,16 @@ __device__ __forceinline__ int grouped_slot(int* cnt, int key) {
   return slot;
 }
 
-// fused gather + advance + wrap + cell binning; dst[i] = advance(src[perm[i]])
+// one particle: dst[i] = advance(src[s]), its cell and slot (and brick
+// when bcnt; grouped = every lane of the wavefront that is still here
+// calls together), error bits and wall count
 template <typename FT, typename VT, typename CT, int S>
-__global__ __launch_bounds__(BLOCK) void k_advance(
-    StepParams P, const FT* __restrict__ f, const Species* __restrict__ spec, PState<VT> src,
-    PState<VT> dst, const int* __restrict__ perm, int np, CellGrid g, int* __restrict__ cell,
-    int* __restrict__ slot, int* __restrict__ cnt, BrickGrid bg, int* __restrict__ brick,
-    int* __restrict__ bslot, int* __restrict__ bcnt, unsigned* err, unsigned long long* nwall,
-    double* __restrict__ flout, int xcd) {
-  const int i = block_index(xcd) * BLOCK + threadIdx.x;
-  if (i >= np) return;
-  const int s = perm ? perm[i] : i;
+__device__ __forceinline__ void advance_one(
+    const StepParams& P, const FT* __restrict__ f, const Species* __restrict__ spec,
+    const PState<VT>& src, const PState<VT>& dst, int i, int s, int np, const CellGrid& g,
+    int* __restrict__ cell, int* __restrict__ slot, int* __restrict__ cnt, const BrickGrid& bg,
+    int* __restrict__ brick, int* __restrict__ bslot, int* __restrict__ bcnt, bool grouped,
+    unsigned* err, unsigned long long* nwall, double* __restrict__ flout) {
   double st[13];
   load_state(src, s, st);
   const int sid = src.sp[s];
@@ -409,13 +408,58 @@ __global__ __launch_bounds__(BLOCK) void k_advance(
     slot[i] = atomicAdd(&cnt[c], 1);
   }
   if (bcnt) {
-    // every lane of the wavefront that is still here takes part
     const int b = brick_of(g, bg, c);
     brick[i] = b;
-    bslot[i] = grouped_slot(bcnt, b);
+    bslot[i] = grouped ? grouped_slot(bcnt, b) : atomicAdd(&bcnt[b], 1);
   }
   if (wall) atomicAdd(nwall, 1ull);
   if (e) atomicOr(err, e);
+}
+
+// fused gather + advance + wrap + cell binning; dst[i] = advance(src[perm[i]]).
+// skip != nullptr: particles flagged there (those of this step's contact
+// solves, advanced later by k_advance_listed) are left out.
+template <typename FT, typename VT, typename CT, int S>
+__global__ __launch_bounds__(BLOCK) void k_advance(
+    StepParams P, const FT* __restrict__ f, const Species* __restrict__ spec, PState<VT> src,
+    PState<VT> dst, const int* __restrict__ perm, int np, CellGrid g, int* __restrict__ cell,
+    int* __restrict__ slot, int* __restrict__ cnt, BrickGrid bg, int* __restrict__ brick,
+    int* __restrict__ bslot, int* __restrict__ bcnt, unsigned* err, unsigned long long* nwall,
+    double* __restrict__ flout, int xcd, const int* __restrict__ skip) {
+  const int i = block_index(xcd) * BLOCK + threadIdx.x;
+  if (i >= np) return;
+  const int s = perm ? perm[i] : i;
+  if (skip && skip[s]) return;
+  advance_one<FT, VT, CT, S>(P, f, spec, src, dst, i, s, np, g, cell, slot, cnt, bg, brick, bslot,
+                             bcnt, true, err, nwall, flout);
+}
+
+// the particles left out above: storage indices in list[0, *nlist), each
+// to its visiting position inv[s]
+template <typename FT, typename VT, typename CT, int S>
+__global__ __launch_bounds__(BLOCK) void k_advance_listed(
+    StepParams P, const FT* __restrict__ f, const Species* __restrict__ spec, PState<VT> src,
+    PState<VT> dst, const int* __restrict__ list, const unsigned long long* nlist,
+    const int* __restrict__ inv, int np, CellGrid g,
+    int* __restrict__ cell, int* __restrict__ slot, int* __restrict__ cnt, BrickGrid bg,
+    int* __restrict__ brick, int* __restrict__ bslot, int* __restrict__ bcnt, unsigned* err,
+    unsigned long long* nwall, double* __restrict__ flout) {
+  const long long nl = (long long)min(*nlist, (unsigned long long)np);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nl;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int s = list[t];
+    advance_one<FT, VT, CT, S>(P, f, spec, src, dst, inv[s], s, np, g, cell, slot, cnt, bg, brick,
+                               bslot, bcnt, false, err, nwall, flout);
+  }
+}
+
+// flags of the listed particles back to 0 (after both advances are done)
+__global__ void k_clear_listed(const int* __restrict__ list, const unsigned long long* nlist,
+                               int* __restrict__ flag, int np) {
+  const long long nl = (long long)min(*nlist, (unsigned long long)np);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nl;
+       t += (long long)gridDim.x * blockDim.x)
+    flag[list[t]] = 0;
 }
 
 // cell binning of the current positions (when no advance preceded)
@@ -444,10 +488,13 @@ __global__ __launch_bounds__(BLOCK) void k_cells(PState<VT> p, int np, CellGrid 
 __global__ __launch_bounds__(BLOCK) void k_brick_scatter(const int* __restrict__ brick,
                                                          const int* __restrict__ bslot,
                                                          const int* __restrict__ boff,
-                                                         int* __restrict__ trav, int np) {
+                                                         int* __restrict__ trav,
+                                                         int* __restrict__ inv, int np) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  trav[boff[brick[i]] + bslot[i]] = i;
+  const int pos = boff[brick[i]] + bslot[i];
+  trav[pos] = i;
+  inv[i] = pos;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_scatter(const int* __restrict__ cell,
@@ -473,7 +520,8 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
                                                 const double* __restrict__ pz,
                                                 const float* __restrict__ rm,
                                                 int* __restrict__ sorted,
-                                                double4* __restrict__ pos4, int np) {
+                                                double4* __restrict__ pos4,
+                                                int* __restrict__ inv, int np) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   int c = cell[i];
@@ -489,6 +537,7 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
     for (int t = 0; t < m; ++t) r += (id[tmp[o + t]] < me) ? 1 : 0;
   }
   sorted[o + r] = i;
+  if (inv) inv[i] = o + r;
   // cell-ordered copy of what the bounding-sphere test reads: a cell's
   // members are contiguous here
   pos4[o + r] = make_double4(px[i], py[i], pz[i], (double)rm[i]);
@@ -660,7 +709,8 @@ template <typename VT>
 __global__ __launch_bounds__(BLOCK) void k_pw_filter(
     PState<VT> p, const Species* __restrict__ spec, const WorkRec* __restrict__ work,
     const unsigned long long* nwork, unsigned long long cap, double L, int use_pw,
-    WorkRec* __restrict__ lm, unsigned long long* nlm) {
+    WorkRec* __restrict__ lm, unsigned long long* nlm, int* __restrict__ flag,
+    int* __restrict__ plist, unsigned long long* nplist) {
   const unsigned long long nw = min(*nwork, cap);
   const int lane = threadIdx.x & 63;
   // the loop bound is uniform over the workgroup, so every lane of a
@@ -697,6 +747,23 @@ __global__ __launch_bounds__(BLOCK) void k_pw_filter(
       if (lane == leader) base = atomicAdd(nlm, (unsigned long long)__popcll(m));
       base = __shfl(base, leader);
       if (pass) lm[base + __popcll(m & ((1ull << lane) - 1ull))] = r;
+    }
+    // flag (once) and list the particles of the solves, whose advance of the
+    // next step waits for this step's collisions
+    if (flag) {
+      const bool ci = pass && atomicExch(&flag[r.i], 1) == 0;
+      const bool cj = pass && atomicExch(&flag[r.j], 1) == 0;
+      const unsigned long long mi = __ballot(ci), mj = __ballot(cj);
+      const int ni = __popcll(mi), nt = ni + __popcll(mj);
+      if (nt) {
+        const int leader = __ffsll((long long)(mi | mj)) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(nplist, (unsigned long long)nt);
+        base = __shfl(base, leader);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (ci) plist[base + __popcll(mi & below)] = r.i;
+        if (cj) plist[base + ni + __popcll(mj & below)] = r.j;
+      }
     }
   }
 }
@@ -808,10 +875,11 @@ __device__ __forceinline__ bool wave_seed(const MlpDev& W, bool valid, const flo
     MlpAct a;
     const f32x4 y = mlp_forward_tile(W, to_bf(xv), a, lane);
     if (lane < 16)
-      for (int c = 0; c < 3; ++c) sY[16 * tile + lane][c] = y[c];
+      for (int c = 0; c < 4; ++c) sY[16 * tile + lane][c] = y[c];
   }
   __syncthreads();
-  const bool ok = valid && seed_from_output(q1, sY[lane][0], sY[lane][1], sY[lane][2], x);
+  const bool ok =
+      valid && seed_from_output(q1, sY[lane][0], sY[lane][1], sY[lane][2], sY[lane][3], x);
   __syncthreads();
   return ok;
 }
@@ -1040,7 +1108,8 @@ template <typename VT>
 __global__ __launch_bounds__(BLOCK) void k_greedy_impulse(
     PState<VT> p, const Species* __restrict__ spec, const ActiveRec* __restrict__ act,
     const unsigned long long* nact, unsigned long long acap, int* __restrict__ used,
-    const unsigned char* __restrict__ status, unsigned long long* ncoll, unsigned* err) {
+    const unsigned char* __restrict__ status, unsigned long long* ncoll, unsigned* err,
+    int* __restrict__ ccount) {
   const long long na = (long long)min(*nact, acap);
   for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < na;
        b0 += (long long)gridDim.x * blockDim.x) {
@@ -1065,6 +1134,9 @@ __global__ __launch_bounds__(BLOCK) void k_greedy_impulse(
         p.ox[r.i] = (VT)o1[0]; p.oy[r.i] = (VT)o1[1]; p.oz[r.i] = (VT)o1[2];
         p.u[r.j] = (VT)u2[0]; p.v[r.j] = (VT)u2[1]; p.w[r.j] = (VT)u2[2];
         p.ox[r.j] = (VT)o2[0]; p.oy[r.j] = (VT)o2[1]; p.oz[r.j] = (VT)o2[2];
+        // accepted pairs are disjoint: plain increments
+        ccount[p.id[r.i]] += 1;
+        ccount[p.id[r.j]] += 1;
       }
     }
     const unsigned long long m = __ballot(acc);
@@ -1188,6 +1260,13 @@ struct gp_ctx {
   int* boff = nullptr;
   int* trav = nullptr;
   double4* pos4 = nullptr;      // (x, y, z, rmax) in cell (sorted) order
+  // pipelined steps (gp_params.overlap)
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int* inlm = nullptr;          // per particle: in a contact solve of this step
+  int* plist = nullptr;         // those particles (storage indices)
+  int* inv = nullptr;           // storage index -> visiting position of the next advance
+  int* ccount = nullptr;        // collisions per particle (reference index)
   // contact-seed network (fp32 masters, Adam moments, bf16 copies)
   float* mlp_th = nullptr;
   float* mlp_m = nullptr;
@@ -1226,7 +1305,7 @@ struct gp_ctx {
   // counters (device)
   unsigned* d_err = nullptr;
   unsigned long long* d_cnt = nullptr;   // 0 nwall 1 nwork 2 nact 3 nfev 4 ncontact 5 ncoll 6 rounds 7 nlm 8 undecided
-                                         // 9 seed fallbacks
+                                         // 9 seed fallbacks 10 particles in solves
   unsigned long long* h_cnt = nullptr;   // pinned mirror
   long long* h_total = nullptr;
   double* flout = nullptr;
@@ -1429,7 +1508,12 @@ int gp_create(const gp_params* p, gp_ctx** out) {
   s.dtime = p->dtime;
   s.ecw = p->ecw;
   s.muw = p->muw;
-  if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess) {
+  // default-priority streams: the main one and the one a pipelined advance
+  // runs on beside the contact solves
+  if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->st2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
     delete ctx;
     return GP_ERR_HIP;
   }
@@ -1461,7 +1545,7 @@ int gp_destroy(gp_ctx* ctx) {
                   ctx->slot, ctx->cnt, ctx->off, ctx->tmp, ctx->sorted, ctx->pcount,
                   ctx->poff, ctx->scan_scr, ctx->mark, ctx->used, ctx->pos_of_id, ctx->work,
                   ctx->lmwork, ctx->brick, ctx->bslot, ctx->bcnt, ctx->boff, ctx->trav,
-                  ctx->pos4, ctx->mlp_th, ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, ctx->mlp_wb,
+                  ctx->pos4, ctx->inlm, ctx->plist, ctx->inv, ctx->ccount, ctx->mlp_th, ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, ctx->mlp_wb,
                   ctx->mlp_bias, ctx->mlp_part, ctx->mlp_lpart, ctx->mlp_loss, ctx->seeds,
                   ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout};
   for (void* b : bufs)
@@ -1471,6 +1555,10 @@ int gp_destroy(gp_ctx* ctx) {
   drop_graph(ctx);
   if (ctx->h_cnt) hipHostFree(ctx->h_cnt);
   if (ctx->h_total) hipHostFree(ctx->h_total);
+  hipStreamSynchronize(ctx->st2);
+  hipEventDestroy(ctx->ev_fork);
+  hipEventDestroy(ctx->ev_join);
+  hipStreamDestroy(ctx->st2);
   hipStreamDestroy(ctx->st);
   delete ctx;
   return GP_OK;
@@ -1599,6 +1687,12 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   if ((rc = dalloc(ctx, &ctx->bslot, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->trav, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->pos4, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->inlm, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->plist, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->inv, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->ccount, (size_t)np))) return rc;
+  HIPCHK(hipMemset(ctx->inlm, 0, (size_t)np * sizeof(int)));
+  HIPCHK(hipMemset(ctx->ccount, 0, (size_t)np * sizeof(int)));
   if (ctx->prm.keep_fluid) {
     if ((rc = dalloc(ctx, &ctx->flout, (size_t)np * 6))) return rc;
   }
@@ -1782,53 +1876,74 @@ static void collect_timing(gp_ctx* ctx) {
 static void mark(gp_ctx* ctx, int k) {
   if (ctx->timing) hipEventRecord(ctx->ev[k], ctx->st);
 }
+static void zero_bins(gp_ctx* ctx, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->g.nbox)), dim3(BLOCK), 0, st, ctx->cnt, 0,
+                     (long long)ctx->g.nbox);
+  if (ctx->use_bricks)
+    hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->bg.n)), dim3(BLOCK), 0, st, ctx->bcnt, 0,
+                       (long long)ctx->bg.n);
+}
+
+// the advance kernel from buffer cur to 1 - cur in the visiting order perm
+// (nullptr: storage order); skip: particles left for k_advance_listed
 template <typename VT>
-static int launch_advance(gp_ctx* ctx, bool bin) {
+static void launch_advance_kernel(gp_ctx* ctx, hipStream_t st, const int* perm, const int* skip,
+                                  bool bin) {
   const int np = ctx->np;
-  const int src = ctx->cur, dst = 1 - ctx->cur;
-  if (bin) {
-    hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->g.nbox)), dim3(BLOCK), 0, ctx->st, ctx->cnt, 0,
-                       (long long)ctx->g.nbox);
-    if (ctx->use_bricks)
-      hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->bg.n)), dim3(BLOCK), 0, ctx->st, ctx->bcnt,
-                         0, (long long)ctx->bg.n);
-  }
-  const int* perm = ctx->perm_identity ? nullptr : (ctx->use_bricks ? ctx->trav : ctx->perm);
+  auto ps = pstate<VT>(ctx, ctx->cur), pd = pstate<VT>(ctx, 1 - ctx->cur);
   int* bcnt = bin && ctx->use_bricks ? ctx->bcnt : nullptr;
-  const int xcd = ctx->prm.xcd_remap;
-  auto ps = pstate<VT>(ctx, src), pd = pstate<VT>(ctx, dst);
   int* cell = bin ? ctx->cell : nullptr;
-  collect_timing(ctx);
-  mark(ctx, 0);
+  const int xcd = ctx->prm.xcd_remap;
   unsigned nb = nblk(np);
   if (xcd) nb = (nb + 7) / 8 * 8;
   const dim3 gr(nb), bl(BLOCK);
+#define GP_ADV(FT, CT, S)                                                                        \
+  hipLaunchKernelGGL((k_advance<FT, VT, CT, S>), gr, bl, 0, st, ctx->sp, (const FT*)ctx->field, \
+                     ctx->spec, ps, pd, perm, np, ctx->g, cell, ctx->slot, ctx->cnt, ctx->bg,    \
+                     ctx->brick, ctx->bslot, bcnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd,  \
+                     skip)
   if (ctx->mixed) {
-    if (ctx->prm.interp_scheme == 3)
-      hipLaunchKernelGGL((k_advance<float, VT, float, 4>), gr, bl, 0, ctx->st, ctx->sp,
-                         (const float*)ctx->field, ctx->spec, ps, pd, perm, np, ctx->g, cell,
-                         ctx->slot, ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,
-                         ctx->d_cnt + 0, ctx->flout, xcd);
-    else
-      hipLaunchKernelGGL((k_advance<float, VT, float, 5>), gr, bl, 0, ctx->st, ctx->sp,
-                         (const float*)ctx->field, ctx->spec, ps, pd, perm, np, ctx->g, cell,
-                         ctx->slot, ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,
-                         ctx->d_cnt + 0, ctx->flout, xcd);
+    if (ctx->prm.interp_scheme == 3) GP_ADV(float, float, 4);
+    else GP_ADV(float, float, 5);
   } else {
-    if (ctx->prm.interp_scheme == 3)
-      hipLaunchKernelGGL((k_advance<double, VT, double, 4>), gr, bl, 0, ctx->st, ctx->sp,
-                         (const double*)ctx->field, ctx->spec, ps, pd, perm, np, ctx->g, cell,
-                         ctx->slot, ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,
-                         ctx->d_cnt + 0, ctx->flout, xcd);
-    else
-      hipLaunchKernelGGL((k_advance<double, VT, double, 5>), gr, bl, 0, ctx->st, ctx->sp,
-                         (const double*)ctx->field, ctx->spec, ps, pd, perm, np, ctx->g, cell,
-                         ctx->slot, ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,
-                         ctx->d_cnt + 0, ctx->flout, xcd);
+    if (ctx->prm.interp_scheme == 3) GP_ADV(double, double, 4);
+    else GP_ADV(double, double, 5);
   }
+#undef GP_ADV
+}
+
+// the advance of the particles listed by k_pw_filter (after the collisions)
+template <typename VT>
+static void launch_advance_listed(gp_ctx* ctx, hipStream_t st) {
+  const int np = ctx->np;
+  auto ps = pstate<VT>(ctx, ctx->cur), pd = pstate<VT>(ctx, 1 - ctx->cur);
+  int* bcnt = ctx->use_bricks ? ctx->bcnt : nullptr;
+  const dim3 gr(512), bl(BLOCK);
+#define GP_ADVL(FT, CT, S)                                                                       \
+  hipLaunchKernelGGL((k_advance_listed<FT, VT, CT, S>), gr, bl, 0, st, ctx->sp,                 \
+                     (const FT*)ctx->field, ctx->spec, ps, pd, ctx->plist, ctx->d_cnt + 10,     \
+                     ctx->inv, np, ctx->g, ctx->cell, ctx->slot, ctx->cnt, ctx->bg,              \
+                     ctx->brick, ctx->bslot, bcnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout)
+  if (ctx->mixed) {
+    if (ctx->prm.interp_scheme == 3) GP_ADVL(float, float, 4);
+    else GP_ADVL(float, float, 5);
+  } else {
+    if (ctx->prm.interp_scheme == 3) GP_ADVL(double, double, 4);
+    else GP_ADVL(double, double, 5);
+  }
+#undef GP_ADVL
+}
+
+template <typename VT>
+static int launch_advance(gp_ctx* ctx, bool bin) {
+  if (bin) zero_bins(ctx, ctx->st);
+  const int* perm = ctx->perm_identity ? nullptr : (ctx->use_bricks ? ctx->trav : ctx->perm);
+  collect_timing(ctx);
+  mark(ctx, 0);
+  launch_advance_kernel<VT>(ctx, ctx->st, perm, nullptr, bin);
   HIPCHK(hipGetLastError());
   mark(ctx, 1);
-  ctx->cur = dst;
+  ctx->cur = 1 - ctx->cur;
   ctx->perm_identity = true;
   ctx->cells_valid = bin;
   return GP_OK;
@@ -1866,14 +1981,14 @@ static int launch_sort(gp_ctx* ctx) {
   const int* grp = ctx->ngroups > 1 ? ps.grp : nullptr;
   hipLaunchKernelGGL(k_rank, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->off,
                      ctx->cnt, ctx->tmp, ps.id, grp, ps.x, ps.y, ps.z, ps.rm, ctx->sorted,
-                     ctx->pos4, np);
+                     ctx->pos4, ctx->use_bricks ? nullptr : ctx->inv, np);
   hipLaunchKernelGGL(k_pair_count, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, g, ctx->sorted,
                      ctx->cell, ctx->off, ctx->cnt, grp, ctx->pcount, np);
   scan_exclusive<int, long long>(ctx->pcount, ctx->poff, np, (long long*)ctx->scan_scr, ctx->st);
   if (ctx->use_bricks) {
     scan_exclusive<int, int>(ctx->bcnt, ctx->boff, ctx->bg.n, (int*)ctx->scan_scr, ctx->st);
     hipLaunchKernelGGL(k_brick_scatter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->brick,
-                       ctx->bslot, ctx->boff, ctx->trav, np);
+                       ctx->bslot, ctx->boff, ctx->trav, ctx->inv, np);
   }
   HIPCHK(hipGetLastError());
   mark(ctx, 2);
@@ -1896,9 +2011,17 @@ static int launch_sort_count(gp_ctx* ctx) {
   return read_nclose(ctx);
 }
 
-// work / active lists able to hold `need` prefilter survivors
+// work / active lists able to hold `need` prefilter survivors.  Kernels
+// already queued may still use the current lists (the eager step that sizes
+// them): both streams are drained before anything is freed.
 static int ensure_lists(gp_ctx* ctx, unsigned long long need) {
   int rc;
+  const unsigned long long acap_need =
+      std::max<unsigned long long>(std::max(need, ctx->work_cap) / 4, 4096);
+  if (need > ctx->work_cap || acap_need > ctx->act_cap) {
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st2));
+  }
   if (need > ctx->work_cap) {
     drop_graph(ctx);
     const unsigned long long cap = need + need / 4 + 1024;
@@ -1917,12 +2040,13 @@ static int ensure_lists(gp_ctx* ctx, unsigned long long need) {
   return GP_OK;
 }
 
-// prefilter + LM + greedy pass on the sorted lists (asynchronous)
+// prefilters on the sorted lists (asynchronous); list_lm: also flag and
+// list the particles of the contact solves (for a pipelined next advance)
 template <typename VT>
-static int launch_contacts(gp_ctx* ctx) {
+static int launch_contacts_filter(gp_ctx* ctx, bool list_lm) {
   const int np = ctx->np;
   unsigned long long* c = ctx->d_cnt;
-  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(16), 0, ctx->st, c + 1, 0ull, 9ll);
+  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(16), 0, ctx->st, c + 1, 0ull, 10ll);
   auto ps = pstate<VT>(ctx, ctx->cur);
   hipLaunchKernelGGL(k_prefilter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->pos4,
                      ctx->sorted, ctx->cell, ctx->off, ctx->cnt, ctx->poff,
@@ -1933,7 +2057,16 @@ static int launch_contacts(gp_ctx* ctx) {
                                                                  2048ull);
   hipLaunchKernelGGL(k_pw_filter<VT>, dim3(fgrid), dim3(BLOCK), 0, ctx->st, ps, ctx->spec,
                      ctx->work, c + 1, ctx->work_cap, ctx->prm.L, ctx->prm.prefilter >= 2 ? 1 : 0,
-                     ctx->lmwork, c + 7);
+                     ctx->lmwork, c + 7, list_lm ? ctx->inlm : nullptr, ctx->plist, c + 10);
+  HIPCHK(hipGetLastError());
+  return GP_OK;
+}
+
+// contact solves + one-collision pass (asynchronous)
+template <typename VT>
+static int launch_contacts_solve(gp_ctx* ctx) {
+  unsigned long long* c = ctx->d_cnt;
+  auto ps = pstate<VT>(ctx, ctx->cur);
   // one wavefront per workgroup: the LM is latency-bound (one pair per lane),
   // so the pairs are spread over as many CUs as possible
   const bool net = ctx->prm.lm_seed == 1;
@@ -1954,7 +2087,7 @@ static int launch_contacts(gp_ctx* ctx) {
                      ctx->act_cap, ctx->mark, ctx->used, ctx->act_status, c + 8, c + 6);
   hipLaunchKernelGGL(k_greedy_impulse<VT>, dim3(GREEDY_GRID), dim3(BLOCK), 0, ctx->st, ps,
                      ctx->spec, ctx->act, c + 2, ctx->act_cap, ctx->used, ctx->act_status, c + 5,
-                     ctx->d_err);
+                     ctx->d_err, ctx->ccount);
   HIPCHK(hipGetLastError());
   mark(ctx, 4);
   ctx->ev_pending = ctx->timing;
@@ -1963,6 +2096,13 @@ static int launch_contacts(gp_ctx* ctx) {
   ctx->perm_identity = false;
   ctx->cells_valid = false;
   return GP_OK;
+}
+
+template <typename VT>
+static int launch_contacts(gp_ctx* ctx) {
+  int rc;
+  if ((rc = launch_contacts_filter<VT>(ctx, false))) return rc;
+  return launch_contacts_solve<VT>(ctx);
 }
 
 // COLLISION with the list sizes read back from the device
@@ -1976,18 +2116,63 @@ static int launch_collide(gp_ctx* ctx) {
   return launch_contacts<VT>(ctx);
 }
 
-// one step without any host synchronisation (lists already sized)
+// k full steps without host synchronisation (lists already sized).  With
+// gp_params.overlap (and no stage timers) the advance of step s + 1 of every
+// particle outside step s's contact solves runs on a second stream beside
+// those solves (it reads and writes none of their particles); the solves'
+// particles are advanced after the one-collision pass, then the next sort
+// waits for both.
 template <typename VT>
-static int launch_step_async(gp_ctx* ctx) {
+static int launch_steps(gp_ctx* ctx, int k) {
   int rc;
-  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, ctx->d_cnt, 0ull,
-                     1ll);
+  const bool pipe = ctx->prm.overlap && !ctx->timing;
+  unsigned long long* c = ctx->d_cnt;
+  if (pipe)
+    hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st, ctx->inlm, 0,
+                       (long long)ctx->np);
+  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, c, 0ull, 1ll);
   if ((rc = launch_advance<VT>(ctx, true))) return rc;
-  if ((rc = launch_sort<VT>(ctx))) return rc;
-  return launch_contacts<VT>(ctx);
+  for (int s = 0; s < k; ++s) {
+    const bool next = s + 1 < k;
+    if ((rc = launch_sort<VT>(ctx))) return rc;
+    if (!(pipe && next)) {
+      if ((rc = launch_contacts<VT>(ctx))) return rc;
+      if (next) {
+        hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, c, 0ull, 1ll);
+        if ((rc = launch_advance<VT>(ctx, true))) return rc;
+      }
+      continue;
+    }
+    if ((rc = launch_contacts_filter<VT>(ctx, true))) return rc;
+    // the prefilter was the last reader of this step's bins: clear them (and
+    // the wall count) on the main stream, before both advances
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, c, 0ull, 1ll);
+    zero_bins(ctx, ctx->st);
+    // fork: the visiting order of the next advance is this sort's order
+    const int* order = ctx->use_bricks ? ctx->trav : ctx->sorted;
+    HIPCHK(hipEventRecord(ctx->ev_fork, ctx->st));
+    HIPCHK(hipStreamWaitEvent(ctx->st2, ctx->ev_fork, 0));
+    launch_advance_kernel<VT>(ctx, ctx->st2, order, ctx->inlm, true);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev_join, ctx->st2));
+    if ((rc = launch_contacts_solve<VT>(ctx))) return rc;
+    launch_advance_listed<VT>(ctx, ctx->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamWaitEvent(ctx->st, ctx->ev_join, 0));
+    // the flags were read by the advance beside the solves: clear them now
+    hipLaunchKernelGGL(k_clear_listed, dim3(64), dim3(BLOCK), 0, ctx->st, ctx->plist, c + 10,
+                       ctx->inlm, ctx->np);
+    ctx->cur = 1 - ctx->cur;
+    ctx->perm_identity = true;
+    ctx->cells_valid = true;
+  }
+  return GP_OK;
 }
 
-// captures two steps (the buffer flips of two steps cancel) into ctx->gexec
+// steps per captured graph (even: the buffer flips cancel)
+static int graph_unit(const gp_ctx* ctx) { return ctx->prm.overlap ? 8 : 2; }
+
+// captures graph_unit() steps into ctx->gexec
 template <typename VT>
 static int capture_graph(gp_ctx* ctx) {
   drop_graph(ctx);
@@ -1995,8 +2180,7 @@ static int capture_graph(gp_ctx* ctx) {
   ctx->g_perm = ctx->perm;
   ctx->g_perm_identity = ctx->perm_identity;
   HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-  int rc = launch_step_async<VT>(ctx);
-  if (!rc) rc = launch_step_async<VT>(ctx);
+  int rc = launch_steps<VT>(ctx, graph_unit(ctx));
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(ctx->st, &g);
   if (rc) {
@@ -2033,9 +2217,10 @@ static int graph_steps(gp_ctx* ctx, int nsteps) {
     if ((rc = ensure_lists(ctx, need))) return rc;
     if ((rc = capture_graph<VT>(ctx))) return rc;
   }
-  for (; nsteps >= 2; nsteps -= 2) HIPCHK(hipGraphLaunch(ctx->gexec, ctx->st));
-  if (nsteps == 1) {
-    if ((rc = launch_step_async<VT>(ctx))) return rc;
+  const int unit = graph_unit(ctx);
+  for (; nsteps >= unit; nsteps -= unit) HIPCHK(hipGraphLaunch(ctx->gexec, ctx->st));
+  if (nsteps > 0) {
+    if ((rc = launch_steps<VT>(ctx, nsteps))) return rc;
   }
   return GP_OK;
 }
@@ -2220,6 +2405,17 @@ int gp_reset_timers(gp_ctx* ctx) {
   return GP_OK;
 }
 
+int gp_get_collision_counts(gp_ctx* ctx, int np, int* counts, int reset) {
+  if (!ctx || !counts || np != ctx->np || np < 1) return GP_ERR_ARG;
+  HIPCHK(hipMemcpyAsync(counts, ctx->ccount, (size_t)np * sizeof(int), hipMemcpyDeviceToHost,
+                        ctx->st));
+  if (reset)
+    hipLaunchKernelGGL(k_fill<int>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->ccount, 0,
+                       (long long)np);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return check_err(ctx);
+}
+
 int gp_collision_grid(gp_ctx* ctx, int* nbox_xyz, double* box_min_size) {
   if (!ctx || !nbox_xyz || !box_min_size) return GP_ERR_ARG;
   nbox_xyz[0] = ctx->g.nx; nbox_xyz[1] = ctx->g.ny; nbox_xyz[2] = ctx->g.nz;
@@ -2261,8 +2457,8 @@ static void mlp_pad(const float* c, std::vector<float>& pd) {
   for (int h = 0; h < 64; ++h) pd[MO_B1 + h] = c[o++];
   for (int k = 0; k < 64 * 64; ++k) pd[MO_W2 + k] = c[o++];
   for (int h = 0; h < 64; ++h) pd[MO_B2 + h] = c[o++];
-  for (int k = 0; k < 3 * 64; ++k) pd[MO_W3 + k] = c[o++];
-  for (int k = 0; k < 3; ++k) pd[MO_B3 + k] = c[o++];
+  for (int k = 0; k < 4 * 64; ++k) pd[MO_W3 + k] = c[o++];
+  for (int k = 0; k < 4; ++k) pd[MO_B3 + k] = c[o++];
 }
 static void mlp_unpad(const float* pd, float* c) {
   int o = 0;
@@ -2271,8 +2467,8 @@ static void mlp_unpad(const float* pd, float* c) {
   for (int h = 0; h < 64; ++h) c[o++] = pd[MO_B1 + h];
   for (int k = 0; k < 64 * 64; ++k) c[o++] = pd[MO_W2 + k];
   for (int h = 0; h < 64; ++h) c[o++] = pd[MO_B2 + h];
-  for (int k = 0; k < 3 * 64; ++k) c[o++] = pd[MO_W3 + k];
-  for (int k = 0; k < 3; ++k) c[o++] = pd[MO_B3 + k];
+  for (int k = 0; k < 4 * 64; ++k) c[o++] = pd[MO_W3 + k];
+  for (int k = 0; k < 4; ++k) c[o++] = pd[MO_B3 + k];
 }
 
 static void mlp_launch_adam(gp_ctx* ctx, int update, float lr) {
@@ -2300,7 +2496,7 @@ static void mlp_launch_grad(gp_ctx* ctx, const float* dx, const float* dt, long 
                      parts, ctx->mlp_grad, ctx->mlp_lpart, ctx->mlp_loss);
 }
 
-// host rows (12 features, 3 targets) -> device rows (16, 4)
+// host rows (12 features; 4 targets: normal, flag) -> device rows (16, 4)
 static int mlp_upload(gp_ctx* ctx, long long n, const float* x, const float* t, float** dx,
                       float** dt) {
   std::vector<float> hx((size_t)n * 16, 0.f), ht;
@@ -2311,7 +2507,7 @@ static int mlp_upload(gp_ctx* ctx, long long n, const float* x, const float* t, 
   if (t) {
     ht.assign((size_t)n * 4, 0.f);
     for (long long k = 0; k < n; ++k)
-      for (int c = 0; c < 3; ++c) ht[(size_t)k * 4 + c] = t[k * 3 + c];
+      for (int c = 0; c < 4; ++c) ht[(size_t)k * 4 + c] = t[k * 4 + c];
     HIPCHK(hipMalloc((void**)dt, (size_t)n * 4 * sizeof(float)));
     HIPCHK(hipMemcpy(*dt, ht.data(), ht.size() * sizeof(float), hipMemcpyHostToDevice));
   }
@@ -2388,12 +2584,12 @@ int gp_seed_net_forward(gp_ctx* ctx, int64_t n, const float* x, float* y) {
   float *dx = nullptr, *dy = nullptr;
   int rc = mlp_upload(ctx, n, x, nullptr, &dx, nullptr);
   if (!rc) {
-    HIPCHK(hipMalloc((void**)&dy, (size_t)n * 3 * sizeof(float)));
+    HIPCHK(hipMalloc((void**)&dy, (size_t)n * 4 * sizeof(float)));
     const long long ntile = (n + 15) / 16;
     hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)std::min<long long>(ntile, 4096)), dim3(64),
                        0, ctx->st, mlp_dev(ctx), dx, (long long)n, dy);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(y, dy, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(y, dy, (size_t)n * 4 * sizeof(float), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
   }
   if (dx) hipFree(dx);
@@ -2476,7 +2672,7 @@ int gp_seed_net_pairs(gp_ctx* ctx, int64_t n, uint64_t seed, float* x, float* t,
         if (x)
           for (int c = 0; c < 12; ++c) x[k * 12 + c] = hx[(size_t)k * 16 + c];
         if (t)
-          for (int c = 0; c < 3; ++c) t[k * 3 + c] = ht[(size_t)k * 4 + c];
+          for (int c = 0; c < 4; ++c) t[k * 4 + c] = ht[(size_t)k * 4 + c];
       }
     }
   }
@@ -2566,6 +2762,9 @@ int gp_seed_net_eval(gp_ctx* ctx, int64_t npairs, uint64_t seed, gp_seed_report*
     hipLaunchKernelGGL(k_mlp_eval, dim3((unsigned)std::min<long long>((npairs + 63) / 64, 16384)),
                        dim3(64), 0, ctx->st, (unsigned long long)seed, (long long)npairs,
                        mlp_dev(ctx), o);
+    // the copies below run on the null stream: wait for the context's stream
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
     std::vector<int> hr(n), hn(n);
     std::vector<unsigned char> hfr(n), hfn(n), hfb(n);
     std::vector<float> hcd(n);

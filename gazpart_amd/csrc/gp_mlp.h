@@ -1,10 +1,12 @@
-// gp_mlp.h -- the contact-seed network: 12 -> 64 -> 64 -> 3 with ReLU,
+// gp_mlp.h -- the contact-seed network: 12 -> 64 -> 64 -> 4 with ReLU,
 // bf16 weights and activations, fp32 accumulate on the CDNA matrix cores
-// (v_mfma_f32_16x16x16_bf16).  It maps the relative pose and semi-axes of a
-// candidate pair (body frame of the first particle) to the common normal
-// the FP64 contact solve converges to; the solve starts from it instead of
-// the centre line and still decides every contact.  No reference
-// counterpart (the reference seeds from the centre line,
+// (v_mfma_f32_16x16x16_bf16).  From the relative pose and semi-axes of a
+// candidate pair (body frame of the first particle) it predicts the common
+// normal the FP64 contact solve converges to when it finds a contact
+// (outputs 0..2) and the logit of the solve finding one (output 3).  When
+// that probability exceeds MLP_SEED_P the solve starts from the predicted
+// normal instead of the centre line; it still decides every contact.  No
+// reference counterpart (the reference seeds from the centre line,
 // ellipsoid_contact_detection.f90:53-62).
 //
 // Layout: every layer is computed transposed, activations as
@@ -25,13 +27,15 @@ namespace gp {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int MLP_NIN = 12, MLP_NH = 64, MLP_NOUT = 3;
+constexpr int MLP_NIN = 12, MLP_NH = 64, MLP_NOUT = 4;
+// seed only when the predicted contact probability exceeds 0.9
+constexpr float MLP_SEED_LOGIT = 2.1972245773362196f;   // ln(0.9 / 0.1)
 // padded device layout of the parameters (and of their gradients):
 //   W1 [64][16] (k >= 12 zero), b1 [64], W2 [64][64], b2 [64],
-//   W3 [16][64] (rows >= 3 zero), b3 [16] (>= 3 zero)
+//   W3 [16][64] (rows >= 4 zero), b3 [16] (>= 4 zero)
 constexpr int MO_W1 = 0, MO_B1 = MO_W1 + 64 * 16, MO_W2 = MO_B1 + 64, MO_B2 = MO_W2 + 64 * 64,
               MO_W3 = MO_B2 + 64, MO_B3 = MO_W3 + 16 * 64, MLP_NPAD = MO_B3 + 16;
-constexpr int MLP_NPARAM = 64 * 12 + 64 + 64 * 64 + 64 + 3 * 64 + 3;   // compact vector
+constexpr int MLP_NPARAM = 64 * 12 + 64 + 64 * 64 + 64 + 4 * 64 + 4;   // compact vector
 
 __device__ __forceinline__ short f2bf(float x) {
   unsigned u = __float_as_uint(x);
@@ -79,7 +83,7 @@ struct MlpAct {
 
 // forward of one tile of 16 pairs; xb = B operand of the inputs (features
 // 4 (l / 16) + i of pair l % 16).  Returns the output tile (outputs
-// 4 (l / 16) + i of pair l % 16: lanes 0..15 hold outputs 0..2).
+// 4 (l / 16) + i of pair l % 16: lanes 0..15 hold outputs 0..3).
 __device__ __forceinline__ f32x4 mlp_forward_tile(const MlpDev& W, bf16x4 xb, MlpAct& a,
                                                   int lane) {
   const int rq = 4 * (lane >> 4);
@@ -143,9 +147,11 @@ __device__ __forceinline__ void pair_features(const double b1[3], const double q
   f[9] = (float)(rx * fac); f[10] = (float)(ry * fac); f[11] = (float)(rz * fac);
 }
 
-// solve angles (azimuth, elevation) of the lab-frame normal R1 y / |y|
+// solve angles (azimuth, elevation) of the lab-frame normal R1 y / |y|,
+// when the contact logit clears the seeding threshold
 __device__ __forceinline__ bool seed_from_output(const double q1[4], float y0, float y1,
-                                                 float y2, double x[2]) {
+                                                 float y2, float logit, double x[2]) {
+  if (!(logit > MLP_SEED_LOGIT)) return false;
   const double n = sqrt((double)y0 * y0 + (double)y1 * y1 + (double)y2 * y2);
   if (!(n > 1e-20) || !(n < 1e20)) return false;
   Rot<double> R(q1[0], q1[1], q1[2], q1[3]);
@@ -241,13 +247,14 @@ __global__ __launch_bounds__(64) void k_mlp_forward(MlpDev W, const float* __res
     MlpAct a;
     const f32x4 out = mlp_forward_tile(W, to_bf(xv), a, lane);
     if (lane < 16 && p < n)
-      for (int i = 0; i < 3; ++i) y[p * 3 + i] = out[i];
+      for (int i = 0; i < 4; ++i) y[p * 4 + i] = out[i];
   }
 }
 
-// forward + backward of the mean squared error over n rows (targets
-// t [n][4], output 3 zero), scaled by inv_n; each wavefront (workgroup)
-// writes its partial gradient (padded layout) and partial loss.
+// forward + backward of the mean pair loss over n rows (targets t [n][4]:
+// unit normal and contact flag; loss flag |y0..2 - t|^2 + softplus(y3) -
+// flag y3), scaled by inv_n; each wavefront (workgroup) writes its partial
+// gradient (padded layout) and partial loss.
 __global__ __launch_bounds__(64) void k_mlp_train_part(MlpDev W, const float* __restrict__ x,
                                                        const float* __restrict__ t, long long n,
                                                        float inv_n, float* __restrict__ part,
@@ -274,11 +281,15 @@ __global__ __launch_bounds__(64) void k_mlp_train_part(MlpDev W, const float* __
     const f32x4 y = mlp_forward_tile(W, xb, a, lane);
     f32x4 dy = zero4();
     if (lane < 16 && valid) {
+      const float flag = t[p * 4 + 3];
       for (int i = 0; i < 3; ++i) {
         const float e = y[i] - t[p * 4 + i];
-        lsum += e * e;
-        dy[i] = 2.f * e * inv_n;
+        lsum += flag * e * e;
+        dy[i] = 2.f * flag * e * inv_n;
       }
+      const float z = y[3];
+      lsum += fmaxf(z, 0.f) + log1pf(expf(-fabsf(z))) - flag * z;
+      dy[3] = (1.f / (1.f + expf(-z)) - flag) * inv_n;
     }
     gb3 += dy;
     const bf16x4 dyb = to_bf(dy);
@@ -429,7 +440,7 @@ __global__ __launch_bounds__(64) void k_mlp_pairs(unsigned long long seed, long 
     double n0, n1, n2;
     R.mulT(o.cn[0], o.cn[1], o.cn[2], n0, n1, n2);
     t[k * 4 + 0] = (float)n0; t[k * 4 + 1] = (float)n1; t[k * 4 + 2] = (float)n2;
-    t[k * 4 + 3] = 0.f;
+    t[k * 4 + 3] = (float)o.flag;
     if (nfev) nfev[k] = o.nfev;
     if (poses) {
       double* q = poses + k * 20;

@@ -12,7 +12,8 @@ module gazpart_gpu
     integer(c_int) :: n, interp_scheme, precision, renorm_quat
     real(c_double) :: L, visc, rhof, dtime, ecw, muw, dpmax
     integer(c_int) :: prefilter, xcd_remap
-    integer(c_int) :: keep_fluid, timing, graph, brick, lm_seed, lm_waves
+    integer(c_int) :: keep_fluid, timing, graph, brick, lm_seed, lm_waves, overlap
+    integer(c_int) :: reserved(3)
   end type gp_params
 
   type, bind(C) :: gp_stats
@@ -22,6 +23,15 @@ module gazpart_gpu
     integer(c_int64_t) :: ntimed, nlm, seed_fallbacks
     integer(c_int64_t) :: reserved(2)
   end type gp_stats
+
+  type, bind(C) :: gp_seed_report
+    integer(c_int64_t) :: npairs
+    real(c_double) :: loss, label_ms, train_ms, nfev_ref_mean, nfev_net_mean
+    integer(c_int64_t) :: nfev_ref_max, nfev_net_max, flags_ref, flags_net, flag_agree
+    integer(c_int64_t) :: fallbacks, no_seed
+    real(c_double) :: cn_min_dot
+    real(c_double) :: reserved(4)
+  end type gp_seed_report
 
   interface
     integer(c_int) function gp_create(p, ctx) bind(C, name='gp_create')
@@ -143,6 +153,74 @@ module gazpart_gpu
       type(c_ptr), value :: ctx
       integer(c_int), intent(out) :: nbox(4)
       real(c_double), intent(out) :: boxr(6)
+    end function
+    integer(c_int) function gp_get_collision_counts(ctx, np, counts, reset) &
+        bind(C, name='gp_get_collision_counts')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: np, reset
+      integer(c_int), intent(out) :: counts(np)
+    end function
+    ! contact-seed network (theta: GP_SEED_NET_NPARAM = 5252 floats)
+    integer(c_int) function gp_seed_net_set(ctx, theta, nparam) bind(C, name='gp_seed_net_set')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nparam
+      real(c_float), intent(in) :: theta(nparam)
+    end function
+    integer(c_int) function gp_seed_net_get(ctx, theta, nparam) bind(C, name='gp_seed_net_get')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nparam
+      real(c_float), intent(out) :: theta(nparam)
+    end function
+    integer(c_int) function gp_seed_net_forward(ctx, n, x, y) bind(C, name='gp_seed_net_forward')
+      import :: c_int, c_ptr, c_float, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_float), intent(in) :: x(12, n)
+      real(c_float), intent(out) :: y(4, n)
+    end function
+    integer(c_int) function gp_seed_net_loss_grad(ctx, n, x, t, grad, loss) &
+        bind(C, name='gp_seed_net_loss_grad')
+      import :: c_int, c_ptr, c_float, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_float), intent(in) :: x(12, n), t(4, n)
+      real(c_float), intent(out) :: grad(*)
+      real(c_double), intent(out) :: loss
+    end function
+    integer(c_int) function gp_seed_net_train(ctx, n, x, t, epochs, batch, lr, loss) &
+        bind(C, name='gp_seed_net_train')
+      import :: c_int, c_ptr, c_float, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_float), intent(in) :: x(12, n), t(4, n)
+      integer(c_int), value :: epochs, batch
+      real(c_float), value :: lr
+      real(c_double), intent(out) :: loss
+    end function
+    integer(c_int) function gp_seed_net_pairs(ctx, n, seed, x, t, nfev, poses) &
+        bind(C, name='gp_seed_net_pairs')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n, seed
+      type(c_ptr), value :: x, t, nfev, poses
+    end function
+    integer(c_int) function gp_seed_net_fit(ctx, npairs, seed, epochs, batch, lr, rep) &
+        bind(C, name='gp_seed_net_fit')
+      import :: c_int, c_ptr, c_float, c_int64_t, gp_seed_report
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: npairs, seed
+      integer(c_int), value :: epochs, batch
+      real(c_float), value :: lr
+      type(gp_seed_report), intent(out) :: rep
+    end function
+    integer(c_int) function gp_seed_net_eval(ctx, npairs, seed, rep) bind(C, name='gp_seed_net_eval')
+      import :: c_int, c_ptr, c_int64_t, gp_seed_report
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: npairs, seed
+      type(gp_seed_report), intent(out) :: rep
     end function
   end interface
 

@@ -79,6 +79,8 @@ contains
       p%brick = 0
       p%lm_seed = 0
       p%lm_waves = 0
+      p%overlap = 0
+      p%reserved = 0
       GPU_CTX(J) = c_null_ptr
       rc = gp_create(p, GPU_CTX(J))
       call GPU_CHECK(rc, GPU_CTX(J), 'gp_create')

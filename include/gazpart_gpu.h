@@ -95,6 +95,12 @@ typedef struct {
                           unconverged.  The solve decides every contact */
   int lm_waves;        /* wavefronts of the contact-solve launch (one pair per
                           lane, pairs dealt lane-major); 0: one per SIMD */
+  int overlap;         /* 1: in graph-mode gp_step, the next step's advance
+                          of the particles outside the contact solves runs on
+                          a second stream beside the solves (8 steps per
+                          captured graph); 0: steps run one after another.
+                          No effect on results */
+  int reserved[3];
 } gp_params;
 
 typedef struct {
@@ -178,21 +184,29 @@ int gp_get_stats(gp_ctx *ctx, gp_stats *out);
 int gp_reset_timers(gp_ctx *ctx);
 /* introspection for benchmarks: collision grid of the context           */
 int gp_collision_grid(gp_ctx *ctx, int *nbox_xyz, double *box_min_size);
+/* collisions each particle took part in since gp_set_particles (or the
+ * last reset), in reference order (the per-particle collision statistics
+ * of collision_interaction.f90:330-345); reset != 0 zeroes them */
+int gp_get_collision_counts(gp_ctx *ctx, int np, int *counts, int reset);
 
 /* contact-seed network -------------------------------------------------
- * No reference counterpart: a 12 -> 64 -> 64 -> 3 ReLU network (bf16
- * weights, fp32 accumulate on the matrix cores) mapping a candidate pair's
- * relative pose and semi-axes (body frame of the first particle) to the
- * common normal the contact solve converges to.  It only seeds the solve
- * (gp_params.lm_seed = 1).  Parameters travel as one float vector:
- * W1 (64x12), b1 (64), W2 (64x64), b2 (64), W3 (3x64), b3 (3), row-major.
- * Feature rows: R1^T (b2 - b1)/s, a1/s, a2/s, rotation vector of q1* q2,
- * s = max(a1) + max(a2).  Targets: the unit normal in body frame 1.      */
-#define GP_SEED_NET_NPARAM 5187
+ * No reference counterpart: a 12 -> 64 -> 64 -> 4 ReLU network (bf16
+ * weights, fp32 accumulate on the matrix cores).  From a candidate pair's
+ * relative pose and semi-axes (body frame of the first particle) it
+ * predicts the common normal the contact solve converges to when it finds
+ * a contact (outputs 0..2) and the logit of finding one (output 3).  It
+ * only seeds the solve (gp_params.lm_seed = 1), and only where the
+ * predicted contact probability exceeds 0.9.  Parameters travel as one
+ * float vector: W1 (64x12), b1 (64), W2 (64x64), b2 (64), W3 (4x64),
+ * b3 (4), row-major.  Feature rows: R1^T (b2 - b1)/s, a1/s, a2/s, rotation
+ * vector of q1* q2, s = max(a1) + max(a2).  Target rows: the converged
+ * unit normal in body frame 1 and the contact flag (0 / 1); loss per row
+ * flag |y0..2 - t0..2|^2 + softplus(y3) - flag y3.                       */
+#define GP_SEED_NET_NPARAM 5252
 
 typedef struct {
   int64_t npairs;
-  double loss;           /* mean over pairs of |y - t|^2 (last epoch) */
+  double loss;           /* mean row loss over the last epoch */
   double label_ms;       /* solver labelling of the training pairs */
   double train_ms;       /* forward + backward + Adam on the GPU */
   double nfev_ref_mean;  /* evaluations per solve from the centre line */
@@ -207,12 +221,13 @@ typedef struct {
 
 int gp_seed_net_set(gp_ctx *ctx, const float *theta, int nparam);
 int gp_seed_net_get(gp_ctx *ctx, float *theta, int nparam);
-/* y[n][3] = network(x[n][12]) */
+/* y[n][4] = network(x[n][12]) */
 int gp_seed_net_forward(gp_ctx *ctx, int64_t n, const float *x, float *y);
-/* gradient (same layout as theta) of mean over rows of |y - t|^2 */
+/* gradient (same layout as theta) of the mean row loss; t[n][4] */
 int gp_seed_net_loss_grad(gp_ctx *ctx, int64_t n, const float *x, const float *t,
                           float *grad, double *loss);
-/* Adam (lr, beta 0.9 / 0.999, eps 1e-8) over caller rows, batches in order */
+/* Adam (lr, beta 0.9 / 0.999, eps 1e-8) over caller rows (t[n][4]),
+ * batches in order */
 int gp_seed_net_train(gp_ctx *ctx, int64_t n, const float *x, const float *t, int epochs,
                       int batch, float lr, double *loss);
 /* npairs synthetic tri-axial pairs (semi-axis ratios U[1,4], uniform

@@ -3,10 +3,13 @@ tests, smoke() and bench.py's baseline leg may use it; the library never
 does).  No reference counterpart: GAZPART seeds its contact solve from the
 centre line (ellipsoid_contact_detection.f90:53-62).
 
-Network 12 -> 64 -> 64 -> 3 (ReLU), predicting the converged common normal
-of the contact solve in the body frame of the first particle.  The
-parameters travel as one float32 vector in this order:
-    W1 (64 x 12), b1 (64), W2 (64 x 64), b2 (64), W3 (3 x 64), b3 (3).
+Network 12 -> 64 -> 64 -> 4 (ReLU): outputs 0..2 predict the converged
+common normal of the contact solve in the body frame of the first particle,
+output 3 is the logit of the solve reporting a contact.  Loss per pair:
+flag * |y[0:3] - t|^2 + softplus(y3) - flag * y3 (the normal is learned on
+contacts only; the solve's converged normal elsewhere depends on its path,
+not on the pose).  The parameters travel as one float32 vector:
+    W1 (64 x 12), b1 (64), W2 (64 x 64), b2 (64), W3 (4 x 64), b3 (4).
 
 Two modes:
   exact  - float64 throughout (pinned by finite differences in the tests);
@@ -20,10 +23,10 @@ from __future__ import annotations
 
 import numpy as np
 
-NIN, NH, NOUT = 12, 64, 3
+NIN, NH, NOUT = 12, 64, 4
 SHAPES = (("W1", (NH, NIN)), ("b1", (NH,)), ("W2", (NH, NH)), ("b2", (NH,)),
           ("W3", (NOUT, NH)), ("b3", (NOUT,)))
-NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)   # 5187
+NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)   # 5252
 
 
 def unpack(theta):
@@ -120,8 +123,14 @@ def target(q1, cv):
     return np.einsum("nji,nj->ni", rotm(q1), cv)
 
 
+def seed_ok(y, p_min=0.9):
+    """The seed is used when the contact probability exceeds p_min."""
+    return 1.0 / (1.0 + np.exp(-y[:, 3])) > p_min
+
+
 def seed_angles(q1, y):
     """Solve angles (azimuth, elevation) of the normal R1 y / |y|."""
+    y = y[:, :3]
     n = y / np.linalg.norm(y, axis=-1, keepdims=True)
     cv = np.einsum("nij,nj->ni", rotm(q1), n)
     return np.stack([np.arctan2(cv[:, 1], cv[:, 0]), np.arcsin(np.clip(cv[:, 2], -1, 1))], axis=1)
@@ -130,7 +139,7 @@ def seed_angles(q1, y):
 # ---------------------------------------------------------------------------
 # network
 def forward(theta, X, mode="bf16"):
-    """Returns y (n, 3) and the cache for backward."""
+    """Returns y (n, 4) and the cache for backward."""
     p = unpack(theta)
     rnd = bf16 if mode == "bf16" else (lambda v: np.asarray(v, dtype=np.float64))
     fl = f32 if mode == "bf16" else (lambda v: np.asarray(v, dtype=np.float64))
@@ -146,18 +155,32 @@ def forward(theta, X, mode="bf16"):
     return y, dict(Xb=Xb, W1=W1, W2=W2, W3=W3, z1=z1, H1=H1, z2=z2, H2=H2)
 
 
+def softplus(z):
+    return np.maximum(z, 0.0) + np.log1p(np.exp(-np.abs(z)))
+
+
+def pair_loss(y, T):
+    """T (n, 4): unit normal (3) and contact flag (0 / 1)."""
+    f = T[:, 3]
+    return f * ((y[:, :3] - T[:, :3]) ** 2).sum(axis=1) + softplus(y[:, 3]) - f * y[:, 3]
+
+
 def loss(theta, X, T, mode="exact"):
     y, _ = forward(theta, X, mode)
-    return float(((y - T) ** 2).sum() / X.shape[0])
+    return float(pair_loss(y, T).sum() / X.shape[0])
 
 
 def backward(theta, X, T, mode="bf16"):
-    """Gradient of mean over pairs of sum over outputs (y - t)^2."""
+    """Gradient of the mean pair loss (see the module docstring)."""
     y, c = forward(theta, X, mode)
     rnd = bf16 if mode == "bf16" else (lambda v: np.asarray(v, dtype=np.float64))
     fl = f32 if mode == "bf16" else (lambda v: np.asarray(v, dtype=np.float64))
     n = X.shape[0]
-    dy = fl(2.0 * (y - T) / n)
+    f = T[:, 3:4]
+    dy = np.empty_like(y)
+    dy[:, :3] = 2.0 * f * (y[:, :3] - T[:, :3]) / n
+    dy[:, 3] = (1.0 / (1.0 + np.exp(-y[:, 3])) - T[:, 3]) / n
+    dy = fl(dy)
     dYb = rnd(dy)
     g = {}
     g["W3"] = dYb.T @ c["H2"]

@@ -206,6 +206,8 @@ def test_trajectory_against_reference_fixture(name):
                 e = np.abs(cur[sl][:, ok] - ref[sl][:, ok]).max() / np.abs(ref[sl]).max()
                 assert e <= 1e-9, (s, q, e, "excluded:", bad)
     assert coll.sum() > 100
+    # the per-particle collision counters count the same collisions
+    np.testing.assert_array_equal(ctx.collision_counts(), coll.sum(axis=0))
 
 
 @pytest.mark.parametrize("prec,tol", [(G.PREC_FP64, 1e-9), (G.PREC_MIXED, 1e-5)])
@@ -291,28 +293,38 @@ def test_step_equals_adv_plus_collision_and_is_deterministic(c1):
     np.testing.assert_array_equal(sb, sc)
 
 
+@pytest.mark.parametrize("overlap", [0, 1])
 @pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
-def test_graph_replays_equal_eager_steps(c1, prec):
-    """The captured two-step hipGraph replayed several times, with eager
+def test_graph_replays_equal_eager_steps(c1, prec, overlap):
+    """The captured hipGraph (2 steps, or 8 pipelined steps with the next
+    advance beside the contact solves) replayed several times, with eager
     work and synchronisations in between, gives the eager result bit for
-    bit after every replay."""
-    eager = G.Context.from_workload(c1, precision=prec)
-    graph = G.Context.from_workload(c1, precision=prec, graph=True)
-    for n in (3, 2, 4, 1, 2):
-        eager.step(n)
-        graph.step(n)
-        a, b = eager.get_particles(), graph.get_particles()
-        np.testing.assert_array_equal(a, b)
-        # eager calls on the graph context between replays
-        graph.collision_detection_connectivity()
-        graph.sync()
-    graph.adv_particle_ellp(1)
-    graph.collision(1)
-    eager.adv_particle_ellp(1)
-    eager.collision(1)
-    graph.step(4)
-    eager.step(4)
-    np.testing.assert_array_equal(eager.get_particles(), graph.get_particles())
+    bit after every call; on C1 and on the dense reference ensemble (many
+    contact solves per step)."""
+    d = np.load(GOLD + "golden_trajectory.npz")
+    # the dense ensemble stops at step 53 in the reference (NaN drag, see
+    # test_error_at_the_reference_step): keep it below that
+    for wl, counts in ((c1, (9, 16, 3, 17, 2)), (golden_workload(d), (9, 16, 3, 9, 2))):
+        eager = G.Context.from_workload(wl, precision=prec)
+        graph = G.Context.from_workload(wl, precision=prec, graph=True, overlap=overlap)
+        for n in counts:
+            eager.step(n)
+            graph.step(n)
+            a, b = eager.get_particles(), graph.get_particles()
+            np.testing.assert_array_equal(a, b)
+            assert eager.stats()["ncollision"] == graph.stats()["ncollision"]
+            # eager calls on the graph context between replays
+            graph.collision_detection_connectivity()
+            graph.sync()
+        graph.adv_particle_ellp(1)
+        graph.collision(1)
+        eager.adv_particle_ellp(1)
+        eager.collision(1)
+        graph.step(8)
+        eager.step(8)
+        np.testing.assert_array_equal(eager.get_particles(), graph.get_particles())
+        eager.close()
+        graph.close()
 
 
 @pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
@@ -345,6 +357,32 @@ def test_ensembles_share_a_field_and_never_collide(prec):
             for name, sl in QUANT.items():
                 assert relerr(got[:, a:b], so, sl) <= 1e-9, (e, name)
     assert ncoll > 0
+
+
+def test_error_at_the_reference_step():
+    """The dense reference ensemble runs into a NaN drag at step 53 in the
+    reference's own algorithm (the FP64 oracle, bit-exact with the compiled
+    reference over the fixture's 30 steps, stops there with error 1): the
+    FP64 GPU path stops at the same step with the same error."""
+    d = np.load(GOLD + "golden_trajectory.npz")
+    wl = golden_workload(d)
+    stp = O.Stepper(wl)
+    s = wl.state.copy()
+    ref_step = None
+    for k in range(1, 60):
+        s, err, _, _ = stp.step(s)
+        if err:
+            ref_step = (k, err)
+            break
+    assert ref_step == (53, 1)
+    ctx = G.Context.from_workload(wl, precision=G.PREC_FP64)
+    for k in range(1, 53):
+        ctx.step(1)
+        ctx.sync()
+    ctx.step(1)
+    with pytest.raises(G.GpError) as e:
+        ctx.sync()
+    assert e.value.code == 1
 
 
 def test_error_nan_drag(c1):

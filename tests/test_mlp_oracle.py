@@ -22,15 +22,31 @@ def test_bf16_rounding_matches_torch():
     assert M.bf16(np.float32(1.0 + 3 * 2 ** -9)) == 1.0 + 2 ** -7
 
 
+def _targets(g, n):
+    t = g.normal(size=(n, 3))
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    return np.concatenate([t, (g.random((n, 1)) < 0.6).astype(float)], axis=1)
+
+
+def test_loss_closed_forms():
+    """Contact rows: squared distance of the normal plus the logistic loss;
+    other rows: the logistic loss only (softplus(z) = log(1 + e^z))."""
+    y = np.array([[0.0, 1.0, 0.0, 2.0], [5.0, 5.0, 5.0, -1.0]])
+    T = np.array([[1.0, 0.0, 0.0, 1.0], [0.0, 0.0, 1.0, 0.0]])
+    np.testing.assert_allclose(M.pair_loss(y, T),
+                               [2.0 + np.log1p(np.exp(2.0)) - 2.0, np.log1p(np.exp(-1.0))],
+                               rtol=1e-14)
+
+
 def test_exact_gradient_matches_finite_differences():
     g = np.random.default_rng(1)
     theta = M.init_params(3).astype(np.float64)
     theta += 0.05 * g.normal(size=theta.size)                # non-zero biases too
     X = g.normal(size=(40, M.NIN))
-    T = g.normal(size=(40, M.NOUT))
+    T = _targets(g, 40)
     grad, _ = M.backward(theta, X, T, mode="exact")
     idx = np.concatenate([g.choice(theta.size, 200, replace=False),
-                          np.arange(M.NPARAM - 3, M.NPARAM)])  # b3 included
+                          np.arange(M.NPARAM - 4, M.NPARAM)])  # b3 included
     h = 1e-6
     for k in idx:
         tp, tm = theta.copy(), theta.copy()
@@ -44,7 +60,7 @@ def test_bf16_mode_is_close_to_exact():
     g = np.random.default_rng(2)
     theta = M.init_params(4)
     X = g.normal(size=(256, M.NIN))
-    T = g.normal(size=(256, M.NOUT))
+    T = _targets(g, 256)
     ge, ye = M.backward(theta, X, T, mode="exact")
     gb, yb = M.backward(theta, X, T, mode="bf16")
     assert np.abs(yb - ye).max() <= 0.05 * np.abs(ye).max()
@@ -86,11 +102,11 @@ def test_features_frame_invariance_and_closed_forms():
 def test_seed_angles_reproduce_the_normal():
     g = np.random.default_rng(6)
     q1 = _rand_q(g, 100)
-    y = g.normal(size=(100, 3))
+    y = g.normal(size=(100, 4))
     x = M.seed_angles(q1, y)
     cv = np.stack([np.cos(x[:, 0]) * np.cos(x[:, 1]), np.sin(x[:, 0]) * np.cos(x[:, 1]),
                    np.sin(x[:, 1])], 1)
-    n = y / np.linalg.norm(y, axis=1, keepdims=True)
+    n = y[:, :3] / np.linalg.norm(y[:, :3], axis=1, keepdims=True)
     np.testing.assert_allclose(M.target(q1, cv), n, atol=1e-12)
 
 
