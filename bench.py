@@ -249,6 +249,9 @@ def main():
     ap.add_argument("--brick", type=int, default=0,
                     help="advance visiting order: bricks of BRICK^3 collision cells "
                          "(0: the library default)")
+    ap.add_argument("--lm-waves", type=int, default=0,
+                    help="wavefronts of the contact-solve launch (0: the library "
+                         "default, one per SIMD)")
     ap.add_argument("--pairs", type=int, default=10_000_000,
                     help="C5: solver-labelled pairs the seed network is trained on")
     ap.add_argument("--epochs", type=int, default=3, help="C5: training epochs")
@@ -282,7 +285,7 @@ def main():
         f"dt={wl.dtime:.3e} dpmax={wl.dpmax:.3e}")
     groups = wl.meta.get("group") if cfg == "C4" else None
 
-    kw = {"brick": args.brick}
+    kw = {"brick": args.brick, "lm_waves": args.lm_waves}
     extra = {}
     theta = None
     if cfg == "C4":

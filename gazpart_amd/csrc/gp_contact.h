@@ -101,69 +101,95 @@ struct ContactPair {
   double b1[3], b2[3];
 };
 
-// support points along the normal of angles x (ellipsoid_pair_depth.f90)
-__device__ __forceinline__ void support(const ContactPair& cp, const double x[2], double cv[3],
-                                        double p1[3], double p2[3], double t1[3], double t2[3],
-                                        double& l1, double& l2) {
-  double s0, c0, s1, c1;
-  sincos(x[0], &s0, &c0);
-  sincos(x[1], &s1, &c1);
-  cv[0] = c0 * c1; cv[1] = s0 * c1; cv[2] = s1;
-  symv(cp.E1, cv, t1);
-  symv(cp.E2, cv, t2);   // E2 (-cv) = -t2
-  l1 = sqrt(0.25 * (cv[0] * t1[0] + cv[1] * t1[1] + cv[2] * t1[2]));
-  l2 = sqrt(0.25 * (cv[0] * t2[0] + cv[1] * t2[1] + cv[2] * t2[2]));
-  double h1 = 1.0 / (2.0 * l1), h2 = 1.0 / (2.0 * l2);
+// The support points along the normal of angles x (ellipsoid_pair_depth.f90)
+// and what the Jacobian at the same x needs of them.  lmder evaluates the
+// Jacobian only at accepted points, whose residuals it has already
+// evaluated, so the solve keeps the Jet of its current point and the
+// Jacobian starts from it instead of recomputing the support (the same
+// arithmetic on the same x, hence the same values).
+struct Jet {
+  double s0, c0, s1, c1;   // sin and cos of x[0], x[1]
+  double t1[3], t2[3];     // E1 cv, E2 cv
+  double l1, l2;           // sqrt(cv^T E cv / 4)
+  double h1, h2;           // 1 / (2 l)
+  double d[3];             // p1 - p2
+};
+
+// support of the normal cv(x) = (cos x0 cos x1, sin x0 cos x1, sin x1):
+// p1 = b1 + E1 cv / (2 l1), p2 = b2 - E2 cv / (2 l2); residuals
+// f_i = (p1 - p2)_i^2
+__device__ __forceinline__ void support(const ContactPair& cp, const double x[2], Jet& s,
+                                        double f[3]) {
+  sincos(x[0], &s.s0, &s.c0);
+  sincos(x[1], &s.s1, &s.c1);
+  const double cv[3] = {s.c0 * s.c1, s.s0 * s.c1, s.s1};
+  symv(cp.E1, cv, s.t1);
+  symv(cp.E2, cv, s.t2);   // E2 (-cv) = -t2
+  s.l1 = sqrt(0.25 * (cv[0] * s.t1[0] + cv[1] * s.t1[1] + cv[2] * s.t1[2]));
+  s.l2 = sqrt(0.25 * (cv[0] * s.t2[0] + cv[1] * s.t2[1] + cv[2] * s.t2[2]));
+  s.h1 = 1.0 / (2.0 * s.l1);
+  s.h2 = 1.0 / (2.0 * s.l2);
   for (int i = 0; i < 3; ++i) {
-    p1[i] = h1 * t1[i] + cp.b1[i];
-    p2[i] = cp.b2[i] - h2 * t2[i];
+    const double p1 = s.h1 * s.t1[i] + cp.b1[i];
+    const double p2 = cp.b2[i] - s.h2 * s.t2[i];
+    s.d[i] = p1 - p2;
+    f[i] = s.d[i] * s.d[i];
   }
 }
 
-// residuals f_i = (p1 - p2)_i^2
-__device__ __forceinline__ void fcn_f(const ContactPair& cp, const double x[2], double f[3]) {
-  double cv[3], p1[3], p2[3], t1[3], t2[3], l1, l2;
-  support(cp, x, cv, p1, p2, t1, t2, l1, l2);
+// normal and support points of a Jet
+__device__ __forceinline__ void support_points(const ContactPair& cp, const Jet& s, double cv[3],
+                                               double p1[3], double p2[3]) {
+  cv[0] = s.c0 * s.c1; cv[1] = s.s0 * s.c1; cv[2] = s.s1;
   for (int i = 0; i < 3; ++i) {
-    double d = p1[i] - p2[i];
-    f[i] = d * d;
+    p1[i] = s.h1 * s.t1[i] + cp.b1[i];
+    p2[i] = cp.b2[i] - s.h2 * s.t2[i];
   }
 }
 
 // Jacobian (ellipsoid_contact_detection.f90:290-330), column-major 3x2:
 // d(depth)/dx = (C1 + C2) dcv/dx with Ck = Ek/(2 lk) - tk tk^T / (8 lk^3)
-__device__ __forceinline__ void fcn_j(const ContactPair& cp, const double x[2], double J[6]) {
-  double cv[3], p1[3], p2[3], t1[3], t2[3], l1, l2;
-  support(cp, x, cv, p1, p2, t1, t2, l1, l2);
-  double s0, c0, s1, c1;
-  sincos(x[0], &s0, &c0);
-  sincos(x[1], &s1, &c1);
-  const double g0[3] = {-s0 * c1, c0 * c1, 0.0};
-  const double g1[3] = {-c0 * s1, -s0 * s1, c1};
-  const double h1 = 1.0 / (2.0 * l1), h2 = 1.0 / (2.0 * l2);
-  const double k1 = 1.0 / (8.0 * l1 * l1 * l1), k2 = 1.0 / (8.0 * l2 * l2 * l2);
+__device__ __forceinline__ void fcn_j(const ContactPair& cp, const Jet& s, double J[6]) {
+  const double g0[3] = {-s.s0 * s.c1, s.c0 * s.c1, 0.0};
+  const double g1[3] = {-s.c0 * s.s1, -s.s0 * s.s1, s.c1};
+  const double k1 = 1.0 / (8.0 * s.l1 * s.l1 * s.l1), k2 = 1.0 / (8.0 * s.l2 * s.l2 * s.l2);
   double e1g0[3], e1g1[3], e2g0[3], e2g1[3];
   symv(cp.E1, g0, e1g0); symv(cp.E1, g1, e1g1);
   symv(cp.E2, g0, e2g0); symv(cp.E2, g1, e2g1);
-  double t1g0 = t1[0] * g0[0] + t1[1] * g0[1] + t1[2] * g0[2];
-  double t1g1 = t1[0] * g1[0] + t1[1] * g1[1] + t1[2] * g1[2];
-  double t2g0 = t2[0] * g0[0] + t2[1] * g0[1] + t2[2] * g0[2];
-  double t2g1 = t2[0] * g1[0] + t2[1] * g1[1] + t2[2] * g1[2];
+  const double t1g0 = s.t1[0] * g0[0] + s.t1[1] * g0[1] + s.t1[2] * g0[2];
+  const double t1g1 = s.t1[0] * g1[0] + s.t1[1] * g1[1] + s.t1[2] * g1[2];
+  const double t2g0 = s.t2[0] * g0[0] + s.t2[1] * g0[1] + s.t2[2] * g0[2];
+  const double t2g1 = s.t2[0] * g1[0] + s.t2[1] * g1[1] + s.t2[2] * g1[2];
   for (int i = 0; i < 3; ++i) {
-    double d = p1[i] - p2[i];
-    double gd0 = h1 * e1g0[i] - k1 * t1[i] * t1g0 + h2 * e2g0[i] - k2 * t2[i] * t2g0;
-    double gd1 = h1 * e1g1[i] - k1 * t1[i] * t1g1 + h2 * e2g1[i] - k2 * t2[i] * t2g1;
-    J[i] = 2.0 * (d * gd0);
-    J[3 + i] = 2.0 * (d * gd1);
+    const double gd0 = s.h1 * e1g0[i] - k1 * s.t1[i] * t1g0 + s.h2 * e2g0[i] - k2 * s.t2[i] * t2g0;
+    const double gd1 = s.h1 * e1g1[i] - k1 * s.t1[i] * t1g1 + s.h2 * e2g1[i] - k2 * s.t2[i] * t2g1;
+    J[i] = 2.0 * (s.d[i] * gd0);
+    J[3 + i] = 2.0 * (s.d[i] * gd1);
   }
 }
 
-// enorm (minpack_reduc.f90:180-232)
+// a[l] for l in {0, 1} (MINPACK's permutation indices), as a select: a
+// dynamic index into a register array would go through scratch memory
+__device__ __forceinline__ double pick(const double a[2], int l) { return l ? a[1] : a[0]; }
+
+// enorm (minpack_reduc.f90:180-232).  When every component is zero or of
+// intermediate magnitude (rdwarf < |x| < rgiant / N) MINPACK's sums reduce
+// to s1 = s3 = x3max = 0 and the result to sqrt(s2), s2 the plain sum of
+// squares in index order; that case is taken without the scaled branches.
 template <int N>
 __device__ __forceinline__ double enorm(const double* x) {
   const double rdwarf = 3.834e-20, rgiant = 1.304e19;
-  double s1 = 0.0, s2 = 0.0, s3 = 0.0, x1max = 0.0, x3max = 0.0;
   const double agiant = rgiant / double(N);
+  bool mid = true;
+  double sq = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double xabs = fabs(x[i]);
+    mid = mid && (xabs == 0.0 || (xabs > rdwarf && xabs < agiant));
+    sq += xabs * xabs;
+  }
+  if (mid) return sqrt(sq);
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0, x1max = 0.0, x3max = 0.0;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     double xabs = fabs(x[i]);
@@ -250,10 +276,10 @@ __device__ __forceinline__ void qrsolv2(double r[6], const int ipvt[2], const do
   x[0] = r[0]; x[1] = r[4];
   wa[0] = qtb[0]; wa[1] = qtb[1];
   for (int j = 0; j < 2; ++j) {
-    int l = ipvt[j];
-    if (diag[l] != 0.0) {
+    const double dl = pick(diag, ipvt[j]);
+    if (dl != 0.0) {
       for (int k = j; k < 2; ++k) sdiag[k] = 0.0;
-      sdiag[j] = diag[l];
+      sdiag[j] = dl;
       double qtbpj = 0.0;
       for (int k = j; k < 2; ++k) {
         if (sdiag[k] != 0.0) {
@@ -293,8 +319,8 @@ __device__ __forceinline__ void qrsolv2(double r[6], const int ipvt[2], const do
     double sum = (nsing >= 2) ? r[1] * wa[1] : 0.0;
     wa[0] = (wa[0] - sum) / sdiag[0];
   }
-  x[ipvt[0]] = wa[0];
-  x[ipvt[1]] = wa[1];
+  x[0] = ipvt[0] ? wa[1] : wa[0];   // x[ipvt[j]] = wa[j]
+  x[1] = ipvt[0] ? wa[0] : wa[1];
 }
 
 // lmpar (minpack_reduc.f90:798-987), n = 2
@@ -311,22 +337,22 @@ __device__ __forceinline__ void lmpar2(double r[6], const int ipvt[2], const dou
   }
   if (nsing >= 2) { wa1[1] = wa1[1] / r[4]; wa1[0] = wa1[0] - r[3] * wa1[1]; }
   if (nsing >= 1) wa1[0] = wa1[0] / r[0];
-  x[ipvt[0]] = wa1[0];
-  x[ipvt[1]] = wa1[1];
+  x[0] = ipvt[0] ? wa1[1] : wa1[0];   // x[ipvt[j]] = wa1[j]
+  x[1] = ipvt[0] ? wa1[0] : wa1[1];
   wa2[0] = diag[0] * x[0]; wa2[1] = diag[1] * x[1];
   double dxnorm = enorm<2>(wa2);
   double fp = dxnorm - delta;
   if (fp <= p1 * delta) { par = 0.0; return; }
   double parl = 0.0;
   if (nsing >= 2) {
-    for (int j = 0; j < 2; ++j) { int l = ipvt[j]; wa1[j] = diag[l] * (wa2[l] / dxnorm); }
+    for (int j = 0; j < 2; ++j) wa1[j] = pick(diag, ipvt[j]) * (pick(wa2, ipvt[j]) / dxnorm);
     wa1[0] = wa1[0] / r[0];
     wa1[1] = (wa1[1] - r[3] * wa1[0]) / r[4];
     double temp = enorm<2>(wa1);
     parl = ((fp / delta) / temp) / temp;
   }
-  wa1[0] = (r[0] * qtb[0]) / diag[ipvt[0]];
-  wa1[1] = (r[3] * qtb[0] + r[4] * qtb[1]) / diag[ipvt[1]];
+  wa1[0] = (r[0] * qtb[0]) / pick(diag, ipvt[0]);
+  wa1[1] = (r[3] * qtb[0] + r[4] * qtb[1]) / pick(diag, ipvt[1]);
   double gnorm = enorm<2>(wa1);
   double paru = gnorm / delta;
   if (paru == 0.0) paru = DWARF / fmin(delta, p1);
@@ -343,7 +369,7 @@ __device__ __forceinline__ void lmpar2(double r[6], const int ipvt[2], const dou
     temp = fp;
     fp = dxnorm - delta;
     if (fabs(fp) <= p1 * delta || (parl == 0.0 && fp <= temp && temp < 0.0) || iter == 10) break;
-    for (int j = 0; j < 2; ++j) { int l = ipvt[j]; wa1[j] = diag[l] * (wa2[l] / dxnorm); }
+    for (int j = 0; j < 2; ++j) wa1[j] = pick(diag, ipvt[j]) * (pick(wa2, ipvt[j]) / dxnorm);
     wa1[0] = wa1[0] / sdiag[0];
     wa1[1] = wa1[1] - r[1] * wa1[0];
     wa1[1] = wa1[1] / sdiag[1];
@@ -358,8 +384,9 @@ __device__ __forceinline__ void lmpar2(double r[6], const int ipvt[2], const dou
 // lmder with lmder1's settings (minpack_reduc.f90:243-640, :732-745):
 // mode 1, factor 100, maxfev 300, ftol = xtol = sqrt(eps), gtol = 0.
 // Returns the number of function evaluations; *info_out = MINPACK's info
-// (1 converged, 4 zero gradient, 5 maxfev or tolerances too small).
-__device__ int lm_solve(const ContactPair& cp, double x[2], int* info_out) {
+// (1 converged, 4 zero gradient, 5 maxfev or tolerances too small);
+// cur = the Jet of the returned x.
+__device__ int lm_solve(const ContactPair& cp, double x[2], int* info_out, Jet& cur) {
   const double tol = 1.4901161193847656e-08;
   const double p1 = 0.1, p5 = 0.5, p25 = 0.25, p75 = 0.75, p0001 = 1.0e-4;
   const int maxfev = 300;
@@ -367,10 +394,11 @@ __device__ int lm_solve(const ContactPair& cp, double x[2], int* info_out) {
   int ipvt[2];
   int info = 0, nfev = 1, iter = 1;
   double delta = 0.0, xnorm = 0.0, par = 0.0;
-  fcn_f(cp, x, fvec);
+  Jet trial;
+  support(cp, x, cur, fvec);
   double fnorm = enorm<3>(fvec);
   for (;;) {
-    fcn_j(cp, x, fjac);
+    fcn_j(cp, cur, fjac);
     qrfac32(fjac, ipvt, wa1, wa2, wa3);
     if (iter == 1) {
       for (int j = 0; j < 2; ++j) diag[j] = (wa2[j] == 0.0) ? 1.0 : wa2[j];
@@ -399,11 +427,11 @@ __device__ int lm_solve(const ContactPair& cp, double x[2], int* info_out) {
     double gnorm = 0.0;
     if (fnorm != 0.0) {
       for (int j = 0; j < 2; ++j) {
-        int l = ipvt[j];
-        if (wa2[l] != 0.0) {
+        const double wl = pick(wa2, ipvt[j]);
+        if (wl != 0.0) {
           double sum = (j == 0) ? fjac[0] * (qtf[0] / fnorm)
                                 : fjac[3] * (qtf[0] / fnorm) + fjac[4] * (qtf[1] / fnorm);
-          gnorm = fmax(gnorm, fabs(sum / wa2[l]));
+          gnorm = fmax(gnorm, fabs(sum / wl));
         }
       }
     }
@@ -417,13 +445,13 @@ __device__ int lm_solve(const ContactPair& cp, double x[2], int* info_out) {
       wa3[0] = diag[0] * wa1[0]; wa3[1] = diag[1] * wa1[1];
       double pnorm = enorm<2>(wa3);
       if (iter == 1) delta = fmin(delta, pnorm);
-      fcn_f(cp, wa2, wa4);
+      support(cp, wa2, trial, wa4);
       nfev++;
       double fnorm1 = enorm<3>(wa4);
       double actred = -1.0;
       if (p1 * fnorm1 < fnorm) { double t = fnorm1 / fnorm; actred = 1.0 - t * t; }
       {
-        double t0 = wa1[ipvt[0]], t1 = wa1[ipvt[1]];
+        double t0 = pick(wa1, ipvt[0]), t1 = pick(wa1, ipvt[1]);
         wa3[0] = fjac[0] * t0 + fjac[3] * t1;
         wa3[1] = fjac[4] * t1;
       }
@@ -446,6 +474,7 @@ __device__ int lm_solve(const ContactPair& cp, double x[2], int* info_out) {
         x[0] = wa2[0]; x[1] = wa2[1];
         wa2[0] = diag[0] * x[0]; wa2[1] = diag[1] * x[1];
         fvec[0] = wa4[0]; fvec[1] = wa4[1]; fvec[2] = wa4[2];
+        cur = trial;
         xnorm = enorm<2>(wa2);
         fnorm = fnorm1;
         iter++;
@@ -492,9 +521,10 @@ __device__ __forceinline__ ContactOut contact_detect(const double bi[3], const d
     x[0] = acos(c0 / cos(x[1]));
   }
   ContactOut o;
-  o.nfev = lm_solve(cp, x, &o.info);
-  double cv[3], t1[3], t2[3], l1, l2;
-  support(cp, x, cv, o.e1, o.e2, t1, t2, l1, l2);
+  Jet s;
+  o.nfev = lm_solve(cp, x, &o.info, s);
+  double cv[3];
+  support_points(cp, s, cv, o.e1, o.e2);
   double d0 = o.e1[0] - o.e2[0], d1 = o.e1[1] - o.e2[1], d2 = o.e1[2] - o.e2[2];
   double contact = d0 * cv[0] + d1 * cv[1] + d2 * cv[2];
   o.flag = (contact > 0.0) && (fmargin(qi, si.a, si.b, si.c, o.e2, bi) < 1.0) &&

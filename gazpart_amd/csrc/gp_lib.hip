@@ -775,10 +775,15 @@ __global__ __launch_bounds__(BLOCK) void k_pw_filter(
 // wave), so a short list leaves few active lanes in every wavefront rather
 // than a few full ones: a wavefront runs the union of its lanes' iteration
 // paths, and the kernel lasts as long as its slowest wavefront.
+// The solve is capped at 256 registers (amdgpu_waves_per_eu(2)): uncapped,
+// a one-wavefront-per-SIMD kernel gets up to 512 and the compiler takes
+// ~356, which leaves no SIMD room for a wavefront of the next step's
+// advance (~192) running beside it on the second stream; the ~100 values
+// it then spills to scratch cost the latency-bound solve little.
 constexpr int LM_BLOCK = 64;
 
 template <typename VT>
-__global__ __launch_bounds__(LM_BLOCK) void k_contact(
+__global__ __launch_bounds__(LM_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void k_contact(
     PState<VT> p, const Species* __restrict__ spec, const WorkRec* __restrict__ work,
     const unsigned long long* nwork, unsigned long long cap, double L,
     ActiveRec* __restrict__ act, unsigned long long* nact, unsigned long long acap,
