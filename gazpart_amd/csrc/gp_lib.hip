@@ -435,9 +435,14 @@ __global__ __launch_bounds__(BLOCK) void k_advance(
 }
 
 // the particles left out above: storage indices in list[0, *nlist), each
-// to its visiting position inv[s]
+// to its visiting position inv[s].  A few thousand particles: launched as
+// one-wavefront workgroups so they spread over as many CUs as possible (each
+// lane's stencil gather is dozens of scattered loads, and a CU's address
+// path, not HBM, limits a handful of full workgroups).
+constexpr int ADVL_BLOCK = 64;
+
 template <typename FT, typename VT, typename CT, int S>
-__global__ __launch_bounds__(BLOCK) void k_advance_listed(
+__global__ __launch_bounds__(ADVL_BLOCK) void k_advance_listed(
     StepParams P, const FT* __restrict__ f, const Species* __restrict__ spec, PState<VT> src,
     PState<VT> dst, const int* __restrict__ list, const unsigned long long* nlist,
     const int* __restrict__ inv, int np, CellGrid g,
@@ -520,7 +525,8 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
                                                 const double* __restrict__ pz,
                                                 const float* __restrict__ rm,
                                                 int* __restrict__ sorted,
-                                                double4* __restrict__ pos4,
+                                                float4* __restrict__ pos4,
+                                                int* __restrict__ csort,
                                                 int* __restrict__ inv, int np) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
@@ -540,7 +546,8 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
   if (inv) inv[i] = o + r;
   // cell-ordered copy of what the bounding-sphere test reads: a cell's
   // members are contiguous here
-  pos4[o + r] = make_double4(px[i], py[i], pz[i], (double)rm[i]);
+  pos4[o + r] = make_float4((float)px[i], (float)py[i], (float)pz[i], rm[i]);
+  csort[o + r] = c;
 }
 
 // members of cell range [o, o+m) (sorted by ensemble) in ensemble g
@@ -627,41 +634,51 @@ __device__ __forceinline__ void min_image(const PState<VT>& p, int i, int j, dou
   }
 }
 
-// bounding-sphere prefilter over each home's candidates (list order kept in
-// the index k); survivors go to the LM work list.  Exact: a contact needs
+// bounding-sphere prefilter over each home's candidates; survivors go to
+// the LM work list.  Their list order is kept in the key k = (q << 32) +
+// loc, q the home's sorted position and loc the candidate's index among
+// the home's candidates: the candidate list is ordered by (q, loc), so k
+// orders pairs as their list index poff[q] + loc would, without the pair
+// counts and their scan.  The candidates of the workgroup are summed into
+// *nclose (the list's length).  Exact: a contact needs
 // the support point of one ellipsoid inside the other, which is impossible
 // when the centres are farther apart than the sum of the largest semi-axes.
+// The test runs on float copies of the centres (rmax rounded up to float);
+// tol, 1e-5 of the largest coordinate magnitude, exceeds their rounding
+// (a few float ulps of that magnitude) by far, so every dropped pair is
+// farther apart than the sum; pairs it keeps beyond it go to the exact
+// disjointness test and the solve, which flag them 0.
 // Survivors are staged in LDS and appended with one global atomic per
 // workgroup (appends from every lane to one counter would serialise);
 // LDS overflow, in dense suspensions, appends directly.
 constexpr int PF_STAGE = 1024;
 
 __global__ __launch_bounds__(BLOCK) void k_prefilter(
-    CellGrid g, const double4* __restrict__ pos4, const int* __restrict__ sorted,
-    const int* __restrict__ cell, const int* __restrict__ off, const int* __restrict__ cnt,
-    const long long* __restrict__ poff, const int* __restrict__ grp, double L, int use_filter,
-    WorkRec* __restrict__ work, unsigned long long* nwork, unsigned long long cap, unsigned* err,
+    CellGrid g, const float4* __restrict__ pos4, const int* __restrict__ sorted,
+    const int* __restrict__ csort, const int* __restrict__ off, const int* __restrict__ cnt,
+    const int* __restrict__ grp, double L, float tol, int use_filter, WorkRec* __restrict__ work,
+    unsigned long long* nwork, unsigned long long cap, unsigned long long* nclose, unsigned* err,
     int nsorted) {
   __shared__ WorkRec stage[PF_STAGE];
   __shared__ int nstage;
-  __shared__ unsigned long long gbase;
-  if (threadIdx.x == 0) nstage = 0;
+  __shared__ unsigned long long gbase, ncand;
+  if (threadIdx.x == 0) { nstage = 0; ncand = 0; }
   __syncthreads();
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  long long k = (long long)q << 32;
   if (q < nsorted) {
     const int hi = sorted[q];
-    const int c = cell[hi];
-    long long k = poff[q];
-    const double4 h = pos4[q];
-    const double xi = h.x, yi = h.y, zi = h.z, ri = h.w;
+    const int c = csort[q];
+    const float4 h = pos4[q];
+    const float Lf = (float)L, Lh = 0.5f * Lf, ri = h.w + tol;
     auto test = [&](int t) {
-      const double4 o = pos4[t];
-      double dx = o.x - xi, dy = o.y - yi, dz = o.z - zi;
-      if (fabs(dx) > L / 2.0) dx -= copysign(L, dx);
-      if (fabs(dy) > L / 2.0) dy -= copysign(L, dy);
-      if (fabs(dz) > L / 2.0) dz -= copysign(L, dz);
-      const double rs = ri + o.w;
-      if (!use_filter || dx * dx + dy * dy + dz * dz < rs * rs * (1.0 + 1.0e-9)) {
+      const float4 o = pos4[t];
+      float dx = o.x - h.x, dy = o.y - h.y, dz = o.z - h.z;
+      if (fabsf(dx) > Lh) dx -= copysignf(Lf, dx);
+      if (fabsf(dy) > Lh) dy -= copysignf(Lf, dy);
+      if (fabsf(dz) > Lh) dz -= copysignf(Lf, dz);
+      const float rs = ri + o.w;
+      if (!use_filter || dx * dx + dy * dy + dz * dz <= rs * rs * 1.0001f) {
         WorkRec r;
         r.k = k; r.i = hi; r.j = sorted[t];
         const int sidx = atomicAdd(&nstage, 1);
@@ -688,7 +705,11 @@ __global__ __launch_bounds__(BLOCK) void k_prefilter(
       for (int t = lo; t < up; ++t) test(t);
     }
   }
+  unsigned long long nc = (unsigned long long)(k - ((long long)q << 32));
+  for (int o = 32; o > 0; o >>= 1) nc += __shfl_xor(nc, o);
+  if ((threadIdx.x & 63) == 0 && nc) atomicAdd(&ncand, nc);
   __syncthreads();
+  if (threadIdx.x == 0 && ncand) atomicAdd(nclose, ncand);
   const int ns = min(nstage, PF_STAGE);
   if (ns == 0) return;
   if (threadIdx.x == 0) gbase = atomicAdd(nwork, (unsigned long long)ns);
@@ -1264,7 +1285,8 @@ struct gp_ctx {
   int* bcnt = nullptr;
   int* boff = nullptr;
   int* trav = nullptr;
-  double4* pos4 = nullptr;      // (x, y, z, rmax) in cell (sorted) order
+  float4* pos4 = nullptr;       // (x, y, z, rmax) as float in cell (sorted) order
+  int* csort = nullptr;         // cell of each sorted position
   // pipelined steps (gp_params.overlap)
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -1310,7 +1332,7 @@ struct gp_ctx {
   // counters (device)
   unsigned* d_err = nullptr;
   unsigned long long* d_cnt = nullptr;   // 0 nwall 1 nwork 2 nact 3 nfev 4 ncontact 5 ncoll 6 rounds 7 nlm 8 undecided
-                                         // 9 seed fallbacks 10 particles in solves
+                                         // 9 seed fallbacks 10 particles in solves 11 candidates
   unsigned long long* h_cnt = nullptr;   // pinned mirror
   long long* h_total = nullptr;
   double* flout = nullptr;
@@ -1548,9 +1570,9 @@ int gp_destroy(gp_ctx* ctx) {
   hipStreamSynchronize(ctx->st);
   void* bufs[] = {ctx->field, ctx->spec, ctx->pbuf[0], ctx->pbuf[1], ctx->perm, ctx->cell,
                   ctx->slot, ctx->cnt, ctx->off, ctx->tmp, ctx->sorted, ctx->pcount,
-                  ctx->poff, ctx->scan_scr, ctx->mark, ctx->used, ctx->pos_of_id, ctx->work,
-                  ctx->lmwork, ctx->brick, ctx->bslot, ctx->bcnt, ctx->boff, ctx->trav,
-                  ctx->pos4, ctx->inlm, ctx->plist, ctx->inv, ctx->ccount, ctx->mlp_th, ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, ctx->mlp_wb,
+                  ctx->poff, ctx->scan_scr, ctx->mark, ctx->used, ctx->pos_of_id,
+                  ctx->work, ctx->lmwork, ctx->brick, ctx->bslot, ctx->bcnt, ctx->boff, ctx->trav,
+                  ctx->pos4, ctx->csort, ctx->inlm, ctx->plist, ctx->inv, ctx->ccount, ctx->mlp_th, ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, ctx->mlp_wb,
                   ctx->mlp_bias, ctx->mlp_part, ctx->mlp_lpart, ctx->mlp_loss, ctx->seeds,
                   ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout};
   for (void* b : bufs)
@@ -1692,6 +1714,7 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   if ((rc = dalloc(ctx, &ctx->bslot, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->trav, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->pos4, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->csort, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->inlm, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->plist, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->inv, (size_t)np))) return rc;
@@ -1923,7 +1946,7 @@ static void launch_advance_listed(gp_ctx* ctx, hipStream_t st) {
   const int np = ctx->np;
   auto ps = pstate<VT>(ctx, ctx->cur), pd = pstate<VT>(ctx, 1 - ctx->cur);
   int* bcnt = ctx->use_bricks ? ctx->bcnt : nullptr;
-  const dim3 gr(512), bl(BLOCK);
+  const dim3 gr(1024), bl(ADVL_BLOCK);
 #define GP_ADVL(FT, CT, S)                                                                       \
   hipLaunchKernelGGL((k_advance_listed<FT, VT, CT, S>), gr, bl, 0, st, ctx->sp,                 \
                      (const FT*)ctx->field, ctx->spec, ps, pd, ctx->plist, ctx->d_cnt + 10,     \
@@ -1971,27 +1994,31 @@ static int launch_bin(gp_ctx* ctx) {
   return GP_OK;
 }
 
-// counting sort + pair counts (asynchronous)
+// counting sort (asynchronous); offsets: also the candidate counts per home
+// and their offsets (the list length before the prefilter, and the
+// materialised lists), which the steps themselves do not need
 template <typename VT>
-static int launch_sort(gp_ctx* ctx) {
+static int launch_sort(gp_ctx* ctx, bool offsets = true) {
   const int np = ctx->np;
   const CellGrid& g = ctx->g;
   int rc;
   if (!ctx->cells_valid)
     if ((rc = launch_bin<VT>(ctx))) return rc;
-  scan_exclusive<int, int>(ctx->cnt, ctx->off, g.nbox, (int*)ctx->scan_scr, ctx->st);
+  scan_exclusive_int(ctx->cnt, ctx->off, g.nbox, (int*)ctx->scan_scr, ctx->st);
   auto ps = pstate<VT>(ctx, ctx->cur);
   hipLaunchKernelGGL(k_scatter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->slot,
                      ctx->off, ctx->tmp, np);
   const int* grp = ctx->ngroups > 1 ? ps.grp : nullptr;
   hipLaunchKernelGGL(k_rank, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->off,
                      ctx->cnt, ctx->tmp, ps.id, grp, ps.x, ps.y, ps.z, ps.rm, ctx->sorted,
-                     ctx->pos4, ctx->use_bricks ? nullptr : ctx->inv, np);
-  hipLaunchKernelGGL(k_pair_count, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, g, ctx->sorted,
-                     ctx->cell, ctx->off, ctx->cnt, grp, ctx->pcount, np);
-  scan_exclusive<int, long long>(ctx->pcount, ctx->poff, np, (long long*)ctx->scan_scr, ctx->st);
+                     ctx->pos4, ctx->csort, ctx->use_bricks ? nullptr : ctx->inv, np);
+  if (offsets) {
+    hipLaunchKernelGGL(k_pair_count, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, g, ctx->sorted,
+                       ctx->cell, ctx->off, ctx->cnt, grp, ctx->pcount, np);
+    scan_exclusive<int, long long>(ctx->pcount, ctx->poff, np, (long long*)ctx->scan_scr, ctx->st);
+  }
   if (ctx->use_bricks) {
-    scan_exclusive<int, int>(ctx->bcnt, ctx->boff, ctx->bg.n, (int*)ctx->scan_scr, ctx->st);
+    scan_exclusive_int(ctx->bcnt, ctx->boff, ctx->bg.n, (int*)ctx->scan_scr, ctx->st);
     hipLaunchKernelGGL(k_brick_scatter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->brick,
                        ctx->bslot, ctx->boff, ctx->trav, ctx->inv, np);
   }
@@ -2045,18 +2072,31 @@ static int ensure_lists(gp_ctx* ctx, unsigned long long need) {
   return GP_OK;
 }
 
+// slack of the float bounding-sphere test: 1e-5 of the largest coordinate
+// magnitude of the collision grid (the float rounding of a centre is 6e-8
+// of it)
+static float prefilter_tol(const gp_ctx* ctx) {
+  const CellGrid& g = ctx->g;
+  const double ext = std::max({1.0, std::fabs(ctx->prm.L), std::fabs(g.xmin),
+                               std::fabs(g.ymin), std::fabs(g.zmin),
+                               std::fabs(g.xmin + g.dx * g.nx), std::fabs(g.ymin + g.dy * g.ny),
+                               std::fabs(g.zmin + g.dz * g.nz)});
+  return (float)(1.0e-5 * ext);
+}
+
 // prefilters on the sorted lists (asynchronous); list_lm: also flag and
 // list the particles of the contact solves (for a pipelined next advance)
 template <typename VT>
 static int launch_contacts_filter(gp_ctx* ctx, bool list_lm) {
   const int np = ctx->np;
   unsigned long long* c = ctx->d_cnt;
-  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(16), 0, ctx->st, c + 1, 0ull, 10ll);
+  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(16), 0, ctx->st, c + 1, 0ull, 11ll);
   auto ps = pstate<VT>(ctx, ctx->cur);
   hipLaunchKernelGGL(k_prefilter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->pos4,
-                     ctx->sorted, ctx->cell, ctx->off, ctx->cnt, ctx->poff,
-                     ctx->ngroups > 1 ? ps.grp : nullptr, ctx->prm.L, ctx->prm.prefilter,
-                     ctx->work, c + 1, ctx->work_cap, ctx->d_err, np);
+                     ctx->sorted, ctx->csort, ctx->off, ctx->cnt,
+                     ctx->ngroups > 1 ? ps.grp : nullptr, ctx->prm.L, prefilter_tol(ctx),
+                     ctx->prm.prefilter,
+                     ctx->work, c + 1, ctx->work_cap, c + 11, ctx->d_err, np);
   // the survivor count stays on the device: a fixed grid, grid-stride inside
   const unsigned fgrid = (unsigned)std::min<unsigned long long>((ctx->work_cap + BLOCK - 1) / BLOCK,
                                                                  2048ull);
@@ -2139,7 +2179,7 @@ static int launch_steps(gp_ctx* ctx, int k) {
   if ((rc = launch_advance<VT>(ctx, true))) return rc;
   for (int s = 0; s < k; ++s) {
     const bool next = s + 1 < k;
-    if ((rc = launch_sort<VT>(ctx))) return rc;
+    if ((rc = launch_sort<VT>(ctx, false))) return rc;
     if (!(pipe && next)) {
       if ((rc = launch_contacts<VT>(ctx))) return rc;
       if (next) {
@@ -2241,7 +2281,8 @@ static void accumulate_stats(gp_ctx* ctx) {
   ctx->stats.ncollision = (int64_t)h[5];
   ctx->stats.lm_nfev = (int64_t)h[3];
   ctx->stats.greedy_rounds = (int64_t)h[6];
-  ctx->stats.nclose = ctx->last_nclose;
+  // the candidate count the last prefilter saw (the sizing sort's before any)
+  ctx->stats.nclose = h[11] ? (int64_t)h[11] : ctx->last_nclose;
 }
 
 extern "C" {
@@ -2263,7 +2304,7 @@ int gp_adv_particle_ellp(gp_ctx* ctx, int ncycle) {
                      1ll);
   rc = ctx->mixed ? launch_advance<float>(ctx, true) : launch_advance<double>(ctx, true);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 10 * sizeof(unsigned long long),
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, ctx->st));
   rc = check_err(ctx);
   ctx->stats.nwall = (int64_t)ctx->h_cnt[0];
@@ -2356,7 +2397,7 @@ int gp_collision(gp_ctx* ctx, int ncycle, double time) {
   if (rc) return rc;
   rc = ctx->mixed ? launch_collide<float>(ctx) : launch_collide<double>(ctx);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 10 * sizeof(unsigned long long),
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, ctx->st));
   rc = check_err(ctx);
   accumulate_stats(ctx);
@@ -2370,7 +2411,7 @@ int gp_step(gp_ctx* ctx, int nsteps) {
   if (ctx->prm.graph) {
     rc = ctx->mixed ? graph_steps<float>(ctx, nsteps) : graph_steps<double>(ctx, nsteps);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 10 * sizeof(unsigned long long),
+    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, ctx->st));
     return GP_OK;
   }
@@ -2382,7 +2423,7 @@ int gp_step(gp_ctx* ctx, int nsteps) {
     rc = ctx->mixed ? launch_collide<float>(ctx) : launch_collide<double>(ctx);
     if (rc) return rc;
   }
-  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 10 * sizeof(unsigned long long),
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, ctx->st));
   return GP_OK;
 }

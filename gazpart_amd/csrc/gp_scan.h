@@ -71,6 +71,97 @@ __global__ void k_copy1(T* dst, const T* src, int zero) {
   dst[0] = zero ? (T)0 : src[0];
 }
 
+// ---------------------------------------------------------------------------
+// Two-launch exclusive scan of int counts (reduce, then scan): the first
+// launch writes each tile's total; in the second every workgroup sums the
+// totals of the tiles before its own (at most a few thousand, read from L2)
+// and scans its tile from there.  No recursion and no separate add or copy
+// launches; the last tile writes the grand total to out[n].  The sums must
+// fit 31 bits (cell counts).  (A single-pass decoupled look-back was slower
+// here: its look-back reads must reach a point of coherence shared by all
+// XCDs, and the inclusive prefixes propagate 64 tiles per round trip.)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(SCAN_BLOCK) void k_tile_sums(const int* __restrict__ in,
+                                                          int* __restrict__ sums, int n) {
+  __shared__ int wsum[SCAN_BLOCK / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int tot = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t idx = base + k;
+    tot += (idx < n) ? in[idx] : 0;
+  }
+  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = tot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < SCAN_BLOCK / 64; ++w) t += wsum[w];
+    sums[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_from_sums(const int* __restrict__ in,
+                                                               int* __restrict__ out,
+                                                               const int* __restrict__ sums,
+                                                               int n, int ntiles) {
+  __shared__ int wsum[SCAN_BLOCK / 64];
+  __shared__ int s_excl;
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // exclusive prefix of this tile: the totals of tiles [0, blockIdx.x)
+  int pre = 0;
+  for (int t = threadIdx.x; t < (int)blockIdx.x; t += SCAN_BLOCK) pre += sums[t];
+  for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o);
+  if (lane == 0) wsum[wid] = pre;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int e = 0;
+    for (int w = 0; w < SCAN_BLOCK / 64; ++w) e += wsum[w];
+    s_excl = e;
+  }
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int v[SCAN_ITEMS];
+  int tot = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t idx = base + k;
+    v[k] = (idx < n) ? in[idx] : 0;
+    tot += v[k];
+  }
+  const int incl = wave_incl_scan(tot);
+  if (lane == 63) wsum[wid] = incl;   // wsum reuse: s_excl was read after the barrier above
+  __syncthreads();
+  int woff = 0, agg = 0;
+  for (int w = 0; w < SCAN_BLOCK / 64; ++w) {
+    if (w < wid) woff += wsum[w];
+    agg += wsum[w];
+  }
+  int run = s_excl + woff + incl - tot;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t idx = base + k;
+    if (idx < n) out[idx] = run;
+    run += v[k];
+  }
+  if ((int)blockIdx.x == ntiles - 1 && threadIdx.x == 0) out[n] = s_excl + agg;
+}
+
+// out[i] = sum_{k<i} in[k] for i < n, out[n] = total; sums holds one int
+// per tile of SCAN_TILE inputs
+inline int64_t scan2_sums(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+inline void scan_exclusive_int(const int* in, int* out, int n, int* sums, hipStream_t st) {
+  const int ntiles = (int)scan2_sums(n);
+  if (n <= 0) {
+    hipLaunchKernelGGL((k_copy1<int>), dim3(1), dim3(1), 0, st, out, (const int*)out, 1);
+    return;
+  }
+  hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, st, in, sums, n);
+  hipLaunchKernelGGL(k_scan_from_sums, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, st, in, out,
+                     sums, n, ntiles);
+}
+
 // scratch elements needed by scan_exclusive for n inputs
 inline int64_t scan_scratch(int64_t n) {
   const int64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
