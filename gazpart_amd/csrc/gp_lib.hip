@@ -527,8 +527,8 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
                                                 int* __restrict__ sorted,
                                                 float4* __restrict__ pos4,
                                                 int* __restrict__ csort,
-                                                int* __restrict__ inv, int np) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                int* __restrict__ inv, int np, int xcd) {
+  int i = block_index(xcd) * BLOCK + threadIdx.x;
   if (i >= np) return;
   int c = cell[i];
   const int o = off[c], m = cnt[c], me = id[i];
@@ -658,21 +658,20 @@ __global__ __launch_bounds__(BLOCK) void k_prefilter(
     const int* __restrict__ csort, const int* __restrict__ off, const int* __restrict__ cnt,
     const int* __restrict__ grp, double L, float tol, int use_filter, WorkRec* __restrict__ work,
     unsigned long long* nwork, unsigned long long cap, unsigned long long* nclose, unsigned* err,
-    int nsorted) {
+    int nsorted, int xcd) {
   __shared__ WorkRec stage[PF_STAGE];
   __shared__ int nstage;
   __shared__ unsigned long long gbase, ncand;
   if (threadIdx.x == 0) { nstage = 0; ncand = 0; }
   __syncthreads();
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = block_index(xcd) * BLOCK + threadIdx.x;
   long long k = (long long)q << 32;
   if (q < nsorted) {
     const int hi = sorted[q];
     const int c = csort[q];
     const float4 h = pos4[q];
     const float Lf = (float)L, Lh = 0.5f * Lf, ri = h.w + tol;
-    auto test = [&](int t) {
-      const float4 o = pos4[t];
+    auto test = [&](const float4 o, int t) {
       float dx = o.x - h.x, dy = o.y - h.y, dz = o.z - h.z;
       if (fabsf(dx) > Lh) dx -= copysignf(Lf, dx);
       if (fabsf(dy) > Lh) dy -= copysignf(Lf, dy);
@@ -692,17 +691,37 @@ __global__ __launch_bounds__(BLOCK) void k_prefilter(
       }
       ++k;
     };
-    int lo = off[c], up = off[c] + cnt[c];
+    // the 14 member ranges in list order (own cell after q, then the half
+    // shell); the first member of every range is fetched before any test
+    // (cells hold ~0-2 members, so this puts nearly every load of the
+    // thread in flight at once instead of one range after another)
+    int slo[14], sup[14];
     const int gr = grp ? grp[hi] : 0;
-    if (grp) group_range(sorted, grp, off[c], cnt[c], gr, &lo, &up);
-    for (int t = q + 1; t < up; ++t) test(t);
+    {
+      int lo = off[c], up = off[c] + cnt[c];
+      if (grp) group_range(sorted, grp, off[c], cnt[c], gr, &lo, &up);
+      slo[0] = q + 1;
+      sup[0] = up;
+    }
     int nb[13];
     half_shell(g, c, nb);
+#pragma unroll
     for (int v = 0; v < 13; ++v) {
-      lo = off[nb[v]];
-      up = lo + cnt[nb[v]];
+      int lo = off[nb[v]], up = lo + cnt[nb[v]];
       if (grp) group_range(sorted, grp, off[nb[v]], cnt[nb[v]], gr, &lo, &up);
-      for (int t = lo; t < up; ++t) test(t);
+      slo[v + 1] = lo;
+      sup[v + 1] = up;
+    }
+    float4 first[14];
+#pragma unroll
+    for (int r = 0; r < 14; ++r)
+      if (slo[r] < sup[r]) first[r] = pos4[slo[r]];
+#pragma unroll
+    for (int r = 0; r < 14; ++r) {
+      if (slo[r] < sup[r]) {
+        test(first[r], slo[r]);
+        for (int t = slo[r] + 1; t < sup[r]; ++t) test(pos4[t], t);
+      }
     }
   }
   unsigned long long nc = (unsigned long long)(k - ((long long)q << 32));
@@ -1914,6 +1933,13 @@ static void zero_bins(gp_ctx* ctx, hipStream_t st) {
 
 // the advance kernel from buffer cur to 1 - cur in the visiting order perm
 // (nullptr: storage order); skip: particles left for k_advance_listed
+// blocks of BLOCK over n items; a multiple of 8 when workgroups are remapped
+// to XCDs (block_index)
+static unsigned xcd_grid(const gp_ctx* ctx, int n) {
+  unsigned nb = nblk(n);
+  return ctx->prm.xcd_remap ? (nb + 7) / 8 * 8 : nb;
+}
+
 template <typename VT>
 static void launch_advance_kernel(gp_ctx* ctx, hipStream_t st, const int* perm, const int* skip,
                                   bool bin) {
@@ -2009,9 +2035,10 @@ static int launch_sort(gp_ctx* ctx, bool offsets = true) {
   hipLaunchKernelGGL(k_scatter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->slot,
                      ctx->off, ctx->tmp, np);
   const int* grp = ctx->ngroups > 1 ? ps.grp : nullptr;
-  hipLaunchKernelGGL(k_rank, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->off,
-                     ctx->cnt, ctx->tmp, ps.id, grp, ps.x, ps.y, ps.z, ps.rm, ctx->sorted,
-                     ctx->pos4, ctx->csort, ctx->use_bricks ? nullptr : ctx->inv, np);
+  hipLaunchKernelGGL(k_rank, dim3(xcd_grid(ctx, np)), dim3(BLOCK), 0, ctx->st, ctx->cell,
+                     ctx->off, ctx->cnt, ctx->tmp, ps.id, grp, ps.x, ps.y, ps.z, ps.rm,
+                     ctx->sorted, ctx->pos4, ctx->csort, ctx->use_bricks ? nullptr : ctx->inv, np,
+                     ctx->prm.xcd_remap);
   if (offsets) {
     hipLaunchKernelGGL(k_pair_count, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, g, ctx->sorted,
                        ctx->cell, ctx->off, ctx->cnt, grp, ctx->pcount, np);
@@ -2092,11 +2119,11 @@ static int launch_contacts_filter(gp_ctx* ctx, bool list_lm) {
   unsigned long long* c = ctx->d_cnt;
   hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(16), 0, ctx->st, c + 1, 0ull, 11ll);
   auto ps = pstate<VT>(ctx, ctx->cur);
-  hipLaunchKernelGGL(k_prefilter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->pos4,
-                     ctx->sorted, ctx->csort, ctx->off, ctx->cnt,
+  hipLaunchKernelGGL(k_prefilter, dim3(xcd_grid(ctx, np)), dim3(BLOCK), 0, ctx->st, ctx->g,
+                     ctx->pos4, ctx->sorted, ctx->csort, ctx->off, ctx->cnt,
                      ctx->ngroups > 1 ? ps.grp : nullptr, ctx->prm.L, prefilter_tol(ctx),
-                     ctx->prm.prefilter,
-                     ctx->work, c + 1, ctx->work_cap, c + 11, ctx->d_err, np);
+                     ctx->prm.prefilter, ctx->work, c + 1, ctx->work_cap, c + 11, ctx->d_err, np,
+                     ctx->prm.xcd_remap);
   // the survivor count stays on the device: a fixed grid, grid-stride inside
   const unsigned fgrid = (unsigned)std::min<unsigned long long>((ctx->work_cap + BLOCK - 1) / BLOCK,
                                                                  2048ull);
