@@ -172,27 +172,17 @@ __device__ __forceinline__ void fcn_j(const ContactPair& cp, const Jet& s, doubl
 // dynamic index into a register array would go through scratch memory
 __device__ __forceinline__ double pick(const double a[2], int l) { return l ? a[1] : a[0]; }
 
-// enorm (minpack_reduc.f90:180-232).  When every component is zero or of
-// intermediate magnitude (rdwarf < |x| < rgiant / N) MINPACK's sums reduce
-// to s1 = s3 = x3max = 0 and the result to sqrt(s2), s2 the plain sum of
-// squares in index order; that case is taken without the scaled branches.
-template <int N>
-__device__ __forceinline__ double enorm(const double* x) {
+// enorm (minpack_reduc.f90:180-232), MINPACK's scaled sums for
+// components that are tiny (<= rdwarf) or huge (>= rgiant / n).  Out of
+// line: the solve reaches it only for such vectors, and keeping its code
+// out of the solve's loop keeps that loop compact in the instruction cache.
+__device__ __noinline__ double enorm_scaled(double x0, double x1, double x2, int n) {
   const double rdwarf = 3.834e-20, rgiant = 1.304e19;
-  const double agiant = rgiant / double(N);
-  bool mid = true;
-  double sq = 0.0;
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const double xabs = fabs(x[i]);
-    mid = mid && (xabs == 0.0 || (xabs > rdwarf && xabs < agiant));
-    sq += xabs * xabs;
-  }
-  if (mid) return sqrt(sq);
+  const double agiant = rgiant / double(n);
+  const double xs[3] = {x0, x1, x2};
   double s1 = 0.0, s2 = 0.0, s3 = 0.0, x1max = 0.0, x3max = 0.0;
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    double xabs = fabs(x[i]);
+  for (int i = 0; i < n; ++i) {
+    double xabs = fabs(xs[i]);
     if (xabs > rdwarf && xabs < agiant) {
       s2 += xabs * xabs;
     } else if (xabs <= rdwarf) {
@@ -211,6 +201,28 @@ __device__ __forceinline__ double enorm(const double* x) {
   if (s2 == 0.0) return x3max * sqrt(s3);
   if (s2 >= x3max) return sqrt(s2 * (1.0 + (x3max / s2) * (x3max * s3)));
   return sqrt(x3max * ((s2 / x3max) + (x3max * s3)));
+}
+
+// enorm.  When every component is zero or of intermediate magnitude
+// (rdwarf < |x| < rgiant / N) MINPACK's sums reduce to s1 = s3 = x3max = 0
+// and the result to sqrt(s2), s2 the plain sum of squares in index order;
+// that case is computed here, the others by enorm_scaled.
+template <int N>
+__device__ __forceinline__ double enorm(const double* x) {
+  const double rdwarf = 3.834e-20, rgiant = 1.304e19;
+  const double agiant = rgiant / double(N);
+  // non-short-circuit operators: the test is a few compares, not a branch
+  // per component
+  bool mid = true;
+  double sq = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double xabs = fabs(x[i]);
+    mid = mid & ((xabs == 0.0) | ((xabs > rdwarf) & (xabs < agiant)));
+    sq += xabs * xabs;
+  }
+  if (mid) return sqrt(sq);
+  return enorm_scaled(x[0], x[1], N > 2 ? x[2] : 0.0, N);
 }
 
 constexpr double EPSMCH = 2.220446049250313e-16;
