@@ -99,3 +99,49 @@ def test_disjoint_pairs_have_no_contact_flag():
             unflagged += 1
     assert flagged > 300
     assert proven > 0.5 * unflagged
+
+
+def test_prefilter_float_slack():
+    """The bounding-sphere prefilter runs on float copies of the centres
+    (gp_lib.hip, k_prefilter) with radii rounded up to float, a slack of
+    1e-5 of the largest coordinate magnitude (prefilter_tol) and a 1e-4
+    relative factor.  With float32 arithmetic on pairs spread over the whole
+    box (the periodic wrap included) at distances from 0.999 to 1.01 times
+    the sum of radii, it must keep every pair whose exact centre distance is below the
+    sum: a dropped pair has to be disjoint for the reference's flag to be
+    0 without a solve."""
+    g = np.random.default_rng(11)
+    L = 2.0 * np.pi
+    ext = max(1.0, L + 0.05)               # largest coordinate magnitude of the grid
+    tol = np.float32(1.0e-5 * ext)
+    n = 200_000
+    ri = g.uniform(0.001, 0.05, n)
+    rj = g.uniform(0.001, 0.05, n)
+    rs = ri + rj
+    d = rs * (1.0 + np.where(g.random(n) < 0.5, g.uniform(-1e-3, 1e-3, n), g.uniform(3e-3, 1e-2, n)))
+    u = g.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    xi = g.uniform(-0.03, L + 0.03, (n, 3))
+    xj = xi + d[:, None] * u
+    xj = np.where(xj >= L, xj - L, np.where(xj < 0.0, xj + L, xj))   # stored wrapped
+    # exact (float64) minimum-image distance
+    dv = xj - xi
+    dv -= L * np.round(dv / L)
+    dex = np.sqrt((dv * dv).sum(axis=1))
+    # the kernel's float test
+    f32 = np.float32
+    rmi = np.nextafter(ri.astype(f32), f32(np.inf))   # float_up
+    rmj = np.nextafter(rj.astype(f32), f32(np.inf))
+    Lf = f32(L)
+    Lh = f32(0.5) * Lf
+    df = xj.astype(f32) - xi.astype(f32)
+    df = np.where(np.abs(df) > Lh, df - np.copysign(Lf, df), df).astype(f32)
+    rsf = (rmi + tol) + rmj
+    keep = (df[:, 0] * df[:, 0] + df[:, 1] * df[:, 1] + df[:, 2] * df[:, 2]) <= rsf * rsf * f32(1.0001)
+    must = dex < rs
+    assert must.sum() > n // 5
+    assert np.all(keep[must]), int((~keep & must).sum())
+    # the slack is small: far pairs are still dropped
+    far = dex > rs * (1.0 + 2e-3) + 2e-4
+    assert far.sum() > n // 5
+    assert not np.any(keep[far])
