@@ -697,9 +697,10 @@ __global__ __launch_bounds__(BLOCK) void k_prefilter(
     // thread in flight at once instead of one range after another)
     int slo[14], sup[14];
     const int gr = grp ? grp[hi] : 0;
+    // a cell's range is [off[c], off[c + 1]): two words of one line
     {
-      int lo = off[c], up = off[c] + cnt[c];
-      if (grp) group_range(sorted, grp, off[c], cnt[c], gr, &lo, &up);
+      int lo = off[c], up = off[c + 1];
+      if (grp) group_range(sorted, grp, lo, up - lo, gr, &lo, &up);
       slo[0] = q + 1;
       sup[0] = up;
     }
@@ -707,8 +708,8 @@ __global__ __launch_bounds__(BLOCK) void k_prefilter(
     half_shell(g, c, nb);
 #pragma unroll
     for (int v = 0; v < 13; ++v) {
-      int lo = off[nb[v]], up = lo + cnt[nb[v]];
-      if (grp) group_range(sorted, grp, off[nb[v]], cnt[nb[v]], gr, &lo, &up);
+      int lo = off[nb[v]], up = off[nb[v] + 1];
+      if (grp) group_range(sorted, grp, lo, up - lo, gr, &lo, &up);
       slo[v + 1] = lo;
       sup[v + 1] = up;
     }
