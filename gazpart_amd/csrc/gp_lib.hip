@@ -533,7 +533,9 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
   int c = cell[i];
   const int o = off[c], m = cnt[c], me = id[i];
   int r = 0;
-  if (grp) {
+  if (m == 1) {
+    // alone in its cell (the common case): rank 0, no member reads
+  } else if (grp) {
     const int g = grp[i];
     for (int t = 0; t < m; ++t) {
       const int j = tmp[o + t], gj = grp[j];

@@ -108,6 +108,17 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_from_sums(const int* __rest
   __shared__ int wsum[SCAN_BLOCK / 64];
   __shared__ int s_excl;
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the tile's own inputs are requested first, so their loads overlap the
+  // walk over the preceding totals
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int v[SCAN_ITEMS];
+  int tot = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t idx = base + k;
+    v[k] = (idx < n) ? in[idx] : 0;
+    tot += v[k];
+  }
   // exclusive prefix of this tile: the totals of tiles [0, blockIdx.x)
   int pre = 0;
   for (int t = threadIdx.x; t < (int)blockIdx.x; t += SCAN_BLOCK) pre += sums[t];
@@ -120,15 +131,6 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_from_sums(const int* __rest
     s_excl = e;
   }
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-  int v[SCAN_ITEMS];
-  int tot = 0;
-#pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    const int64_t idx = base + k;
-    v[k] = (idx < n) ? in[idx] : 0;
-    tot += v[k];
-  }
   const int incl = wave_incl_scan(tot);
   if (lane == 63) wsum[wid] = incl;   // wsum reuse: s_excl was read after the barrier above
   __syncthreads();
