@@ -35,7 +35,7 @@ ERRORS = {
     1: "GP_ERR_NAN_DRAG", 2: "GP_ERR_NAN_OMEGA", 3: "GP_ERR_KE_COLLISION",
     4: "GP_ERR_KE_WALL", 5: "GP_ERR_OUT_OF_GRID", 6: "GP_ERR_NCLOSE_OVERFLOW",
     7: "GP_ERR_INTERP_RANGE", 10: "GP_ERR_ARG", 11: "GP_ERR_STATE", 12: "GP_ERR_HIP",
-    13: "GP_ERR_NOMEM",
+    13: "GP_ERR_NOMEM", 14: "GP_ERR_IO",
 }
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -47,6 +47,8 @@ EXPORTS = (
     "gp_collision", "gp_step", "gp_get_stats", "gp_reset_timers", "gp_collision_grid",
     "gp_get_collision_counts", "gp_seed_net_set", "gp_seed_net_get", "gp_seed_net_forward", "gp_seed_net_loss_grad",
     "gp_seed_net_train", "gp_seed_net_pairs", "gp_seed_net_fit", "gp_seed_net_eval",
+    "gp_part_file_write", "gp_part_file_info", "gp_part_file_read", "gp_field_file_write",
+    "gp_field_file_read",
 )
 SEED_NET_NPARAM = 5252
 
@@ -128,6 +130,13 @@ def load_library(path: str = LIB_PATH):
     L.gp_collision_grid.argtypes = [vp, ip, dp]
     L.gp_reset_timers.argtypes = [vp]
     L.gp_get_collision_counts.argtypes = [vp, C.c_int, ip, C.c_int]
+    cp = C.c_char_p
+    L.gp_part_file_write.argtypes = [cp, C.c_int64, C.c_int, ip, dp, dp, C.c_double, dp]
+    L.gp_part_file_info.argtypes = [cp, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+    L.gp_part_file_read.argtypes = [cp, C.c_int64, C.c_int, ip, dp, dp, C.POINTER(C.c_double),
+                                    dp]
+    L.gp_field_file_write.argtypes = [cp, C.c_int, dp]
+    L.gp_field_file_read.argtypes = [cp, C.c_int, dp]
     L.gp_seed_net_set.argtypes = [vp, vp, C.c_int]
     L.gp_seed_net_get.argtypes = [vp, vp, C.c_int]
     L.gp_seed_net_forward.argtypes = [vp, C.c_int64, vp, vp]

@@ -32,6 +32,7 @@
 #include "../../include/gazpart_gpu.h"
 #include "gp_contact.h"
 #include "gp_device.h"
+#include "gp_io.h"
 #include "gp_mlp.h"
 #include "gp_scan.h"
 
@@ -2876,6 +2877,80 @@ int gp_seed_net_eval(gp_ctx* ctx, int64_t npairs, uint64_t seed, gp_seed_report*
   for (void* b : {(void*)nr, (void*)nn, (void*)fr, (void*)fn, (void*)fb, (void*)cd})
     if (b) hipFree(b);
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// on-disk formats (host only; no context, no GPU)
+// ---------------------------------------------------------------------------
+int gp_part_file_write(const char* path, int64_t npart, int npoly, const int* npclass,
+                       const double* rhop, const double* emaj, double apr, const double* cols) {
+  if (!path || npart < 0 || npart > INT_MAX || npoly < 1 || !npclass || !rhop || !emaj ||
+      (npart > 0 && !cols) || npart * 8 > gpio::kMaxRecord)
+    return GP_ERR_ARG;
+  gpio::File f(path, "wb");
+  if (!f.f) return GP_ERR_IO;
+  const int32_t n32 = (int32_t)npart;
+  bool ok = gpio::rec_write(f.f, &n32, 4) &&
+            gpio::rec_write(f.f, npclass, 4 * (int64_t)npoly) &&
+            gpio::rec_write(f.f, rhop, 8 * (int64_t)npoly) &&
+            gpio::rec_write(f.f, emaj, 8 * (int64_t)npoly) && gpio::rec_write(f.f, &apr, 8);
+  for (int c = 0; ok && c < 14; ++c) ok = gpio::rec_write(f.f, cols + (size_t)c * npart, 8 * npart);
+  ok = f.close() && ok;
+  return ok ? GP_OK : GP_ERR_IO;
+}
+
+int gp_part_file_info(const char* path, int64_t* npart, int* npoly) {
+  if (!path || !npart || !npoly) return GP_ERR_ARG;
+  gpio::File f(path, "rb");
+  if (!f.f) return GP_ERR_IO;
+  int32_t n32 = 0;
+  int64_t len = 0;
+  if (!gpio::rec_read(f.f, &n32, 4) || n32 < 0 || !gpio::rec_peek(f.f, &len) || len < 4 ||
+      len % 4)
+    return GP_ERR_IO;
+  *npart = n32;
+  *npoly = (int)(len / 4);
+  return GP_OK;
+}
+
+int gp_part_file_read(const char* path, int64_t npart, int npoly, int* npclass, double* rhop,
+                      double* emaj, double* apr, double* cols) {
+  if (!path || npart < 0 || npoly < 1 || !npclass || !rhop || !emaj || !apr ||
+      (npart > 0 && !cols))
+    return GP_ERR_ARG;
+  gpio::File f(path, "rb");
+  if (!f.f) return GP_ERR_IO;
+  int32_t n32 = 0;
+  bool ok = gpio::rec_read(f.f, &n32, 4) && n32 == npart &&
+            gpio::rec_read(f.f, npclass, 4 * (int64_t)npoly) &&
+            gpio::rec_read(f.f, rhop, 8 * (int64_t)npoly) &&
+            gpio::rec_read(f.f, emaj, 8 * (int64_t)npoly) && gpio::rec_read(f.f, apr, 8);
+  for (int c = 0; ok && c < 14; ++c) ok = gpio::rec_read(f.f, cols + (size_t)c * npart, 8 * npart);
+  return ok ? GP_OK : GP_ERR_IO;
+}
+
+int gp_field_file_write(const char* path, int n, const double* f3) {
+  if (!path || n < 1 || !f3) return GP_ERR_ARG;
+  gpio::File f(path, "wb");
+  if (!f.f) return GP_ERR_IO;
+  const int32_t hdr[3] = {n, n, n};
+  const size_t cnt = (size_t)n * n * n;
+  bool ok = std::fwrite(hdr, 4, 3, f.f) == 3 && std::fwrite(f3, 8, cnt, f.f) == cnt;
+  ok = f.close() && ok;
+  return ok ? GP_OK : GP_ERR_IO;
+}
+
+int gp_field_file_read(const char* path, int n, double* f3) {
+  if (!path || n < 1 || !f3) return GP_ERR_ARG;
+  gpio::File f(path, "rb");
+  if (!f.f) return GP_ERR_IO;
+  int32_t hdr[3] = {0, 0, 0};
+  const size_t cnt = (size_t)n * n * n;
+  if (std::fread(hdr, 4, 3, f.f) != 3 || hdr[0] != n || hdr[1] != n || hdr[2] != n)
+    return GP_ERR_IO;
+  if (std::fread(f3, 8, cnt, f.f) != cnt) return GP_ERR_IO;
+  char extra;
+  return std::fread(&extra, 1, 1, f.f) == 0 ? GP_OK : GP_ERR_IO;   // nothing after the data
 }
 
 }  // extern "C"

@@ -44,7 +44,9 @@ enum {
   GP_ERR_ARG = 10,
   GP_ERR_STATE = 11,
   GP_ERR_HIP = 12,
-  GP_ERR_NOMEM = 13
+  GP_ERR_NOMEM = 13,
+  GP_ERR_IO = 14              /* a file could not be opened, read or written,
+                                 or its framing / header does not match */
 };
 
 enum { GP_PREC_FP64 = 0, GP_PREC_MIXED = 1 };
@@ -241,6 +243,34 @@ int gp_seed_net_fit(gp_ctx *ctx, int64_t npairs, uint64_t seed, int epochs, int 
                     float lr, gp_seed_report *rep);
 /* solve fresh pairs from both seeds: evaluation counts, flag agreement */
 int gp_seed_net_eval(gp_ctx *ctx, int64_t npairs, uint64_t seed, gp_seed_report *rep);
+
+/* on-disk formats ------------------------------------------------------
+ * Host only (no context, no GPU): the files either side of the step, so
+ * that the path reads and writes what existing runs hold.
+ *
+ * Per-class particle file, ISAVEPART = 2: written by SAVE_PARTICLE as
+ * part_pJJ.end / part_pJJ_tNNNNNNNN.bin (save_particle.f90:116-185), read
+ * back as part_pJJ.ini by INITIATION_PARTICLE_POSITION
+ * (initiation_particle_position.f90:430-514).  Fortran unformatted
+ * sequential records (4-byte length before and after each): NPART_FULL;
+ * NPCLASS(J,1:npoly); RHOP(J,1:npoly); EMAJ_PART(J,1:npoly); APR_PART(J);
+ * then 14 records of npart doubles.  cols is double[14][npart]: the state
+ * components (layout above) followed by the size index IDP as a double.
+ * Records over 2^31 - 1 bytes (npart > 268,435,455) are refused
+ * (GP_ERR_ARG).                                                          */
+int gp_part_file_write(const char *path, int64_t npart, int npoly, const int *npclass,
+                       const double *rhop, const double *emaj, double apr, const double *cols);
+/* npart and npoly (length of the NPCLASS record) of a particle file      */
+int gp_part_file_info(const char *path, int64_t *npart, int *npoly);
+/* reads a file whose npart and npoly match (GP_ERR_IO otherwise)        */
+int gp_part_file_read(const char *path, int64_t npart, int npoly, int *npclass, double *rhop,
+                      double *emaj, double *apr, double *cols);
+/* Velocity-component file of SAVE_MPIIO / READ_MPIIO (uf.end, vf.end,
+ * wf.end, uf_tNNNNNNNN.bin ...; MPI_structures.f90:434-493, 571-621, names
+ * at save_fluid.f90:141-186 and initiation_fluid.f90:275-300): three int32
+ * NX NX NX, then NX^3 doubles in Fortran order (x fastest, no ghosts).   */
+int gp_field_file_write(const char *path, int n, const double *f);
+int gp_field_file_read(const char *path, int n, double *f);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,136 @@
+!! part_io_ref.f90 -- runs the reference's own writers and readers of its
+!! on-disk formats (SAVE_PARTICLE with ISAVEPART = 2, the ISAVEPART = 2
+!! branch of INITIATION_PARTICLE_POSITION, SAVE_MPIIO, READ_MPIIO; compiled
+!! unmodified from the reference's source/ directory by oracle/ref/Makefile)
+!! on one MPI rank.  TEST INFRASTRUCTURE: it checks the library's readers
+!! and writers of the same formats (tests/test_file_formats.py) and is never
+!! part of the product.
+!!
+!! Usage (all files in the current directory):
+!!   part_io_ref psave <in.raw> <ncycle>   classes from in.raw -> SAVE_PARTICLE
+!!   part_io_ref pread <in.raw> <out.raw>  part_pJJ.ini -> out.raw (shapes from in.raw)
+!!   part_io_ref fsave <in.raw> <name>     one field component -> SAVE_MPIIO
+!!   part_io_ref fread <name> <n> <out.raw> READ_MPIIO -> out.raw
+!!
+!! in.raw for psave / pread (stream, native endianness): int32 nig, npoly,
+!! npart, n; double L; int32 npclass(npoly, nig); double rhop(npoly, nig),
+!! emaj(npoly, nig), apr(nig); per class: double state(npart, 13), int32
+!! idp(npart).  out.raw of pread: per class: int32 nloc; int32
+!! npclass(npoly); double rhop(npoly), emaj(npoly), apr; double
+!! state(nloc, 13); int32 idp(nloc).  Field raw files: int32 n, then
+!! double f(n, n, n).
+program part_io_ref
+  use mpi
+  use iso_c_binding
+  use DNS_DIM
+  use PARAM_PHYS
+  use PARTICLE_PARALLEL
+  use MPI_STRUCTURES
+  use gpref_driver
+  implicit none
+  character(len=256) :: mode, a1, a2
+  character(len=40) :: fname
+  integer :: mpierr, ncls, npoly, npart, n, j, i, c, ncycle
+  real(8) :: L
+  integer, allocatable :: ncl(:, :), idp(:)
+  real(8), allocatable :: rh(:, :), em(:, :), apr(:), st(:, :), grav(:), ec(:), mu(:)
+  real(8), allocatable :: f3(:, :, :)
+
+  call get_command_argument(1, mode)
+  call get_command_argument(2, a1)
+  call get_command_argument(3, a2)
+  call MPI_INIT(mpierr)
+  DEBUG = .false.
+
+  select case (trim(mode))
+  case ('psave', 'pread')
+    open(unit=40, file=trim(a1), access='stream', form='unformatted', status='old')
+    read(40) ncls, npoly, npart, n
+    read(40) L
+    allocate(ncl(npoly, ncls), rh(npoly, ncls), em(npoly, ncls), apr(ncls))
+    allocate(grav(ncls), ec(ncls), mu(ncls), st(npart, 13), idp(npart))
+    read(40) ncl, rh, em, apr
+    grav = 0.0d0; ec = 1.0d0; mu = 0.0d0
+    call ref_setup(n, L, 1.0d-2, 1.0d0, 1.0d-3, 1.0d0, 0.0d0, 3, ncls, npoly, &
+                   transpose(em), apr, transpose(rh), grav, ec, mu)
+    if (allocated(NPCLASS)) deallocate(NPCLASS)
+    allocate(NPCLASS(NIG, POLYDISPMAX))
+    NPCLASS = transpose(ncl)
+    NPART_FULL = npart
+    ISAVEPART = 2
+    if (trim(mode) == 'psave') then
+      do j = 1, ncls
+        read(40) st, idp
+        call ref_set_particles(j, npart, st, idp)
+      end do
+      close(40)
+      read(a2, *) ncycle
+      call SAVE_PARTICLE(ncycle)
+    else
+      close(40)
+      !! room for every particle of the file in each class
+      do j = 1, ncls
+        st = 0.0d0; idp = 1
+        call ref_set_particles(j, npart, st, idp)
+      end do
+      INIT_PART_POSITION = 3
+      call INITIATION_PARTICLE_POSITION
+      open(unit=41, file=trim(a2), access='stream', form='unformatted', status='replace')
+      do j = 1, ncls
+        write(41) NPART_LOC(j)
+        write(41) (NPCLASS(j, i), i = 1, npoly)
+        write(41) (RHOP(j, i), i = 1, npoly), (EMAJ_PART(j, i), i = 1, npoly), APR_PART(j)
+        write(41) (PART(i, j)%XP, i = 1, NPART_LOC(j)), (PART(i, j)%YP, i = 1, NPART_LOC(j)), &
+                  (PART(i, j)%ZP, i = 1, NPART_LOC(j)), (PART(i, j)%UP, i = 1, NPART_LOC(j)), &
+                  (PART(i, j)%VP, i = 1, NPART_LOC(j)), (PART(i, j)%WP, i = 1, NPART_LOC(j)), &
+                  (PART(i, j)%ELLQUAT%a, i = 1, NPART_LOC(j)), &
+                  (PART(i, j)%ELLQUAT%b, i = 1, NPART_LOC(j)), &
+                  (PART(i, j)%ELLQUAT%c, i = 1, NPART_LOC(j)), &
+                  (PART(i, j)%ELLQUAT%d, i = 1, NPART_LOC(j)), &
+                  (PART(i, j)%OMEGAX, i = 1, NPART_LOC(j)), &
+                  (PART(i, j)%OMEGAY, i = 1, NPART_LOC(j)), &
+                  (PART(i, j)%OMEGAZ, i = 1, NPART_LOC(j))
+        write(41) (PART(i, j)%IDP, i = 1, NPART_LOC(j))
+      end do
+      close(41)
+    end if
+  case ('fsave', 'fread')
+    if (trim(mode) == 'fsave') then
+      open(unit=40, file=trim(a1), access='stream', form='unformatted', status='old')
+      read(40) n
+    else
+      read(a2, *) n
+    end if
+    allocate(apr(1), rh(1, 1), em(1, 1), grav(1), ec(1), mu(1))
+    apr = 2.0d0; rh = 1.0d3; em = 1.0d-2; grav = 0.0d0; ec = 1.0d0; mu = 0.0d0
+    call ref_setup(n, 1.0d0, 1.0d-2, 1.0d0, 1.0d-3, 1.0d0, 0.0d0, 3, 1, 1, &
+                   em, apr, rh, grav, ec, mu)
+    !! rank 0 of one: SAVE_MPIIO writes the header on rank 0 only
+    !! (ref_setup's MYID = 1 only skips the hot path's rank-0 timers)
+    MYID = 0
+    allocate(f3(n, n, n))
+    if (trim(mode) == 'fsave') then
+      read(40) f3
+      close(40)
+      UFLU = 0.0d0
+      UFLU(1:n, 1:n, 1:n) = f3
+      fname = trim(a2)
+      call SAVE_MPIIO(UFLU, fname)
+    else
+      fname = trim(a1)
+      UFLU = 0.0d0
+      call READ_MPIIO(UFLU, fname)
+      call get_command_argument(4, a2)
+      f3 = UFLU(1:n, 1:n, 1:n)
+      open(unit=41, file=trim(a2), access='stream', form='unformatted', status='replace')
+      write(41) n
+      write(41) f3
+      close(41)
+    end if
+  case default
+    print *, 'usage: part_io_ref psave|pread|fsave|fread ...'
+    call MPI_FINALIZE(mpierr)
+    stop 2
+  end select
+  call MPI_FINALIZE(mpierr)
+end program part_io_ref
