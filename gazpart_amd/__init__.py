@@ -48,7 +48,7 @@ EXPORTS = (
     "gp_get_collision_counts", "gp_seed_net_set", "gp_seed_net_get", "gp_seed_net_forward", "gp_seed_net_loss_grad",
     "gp_seed_net_train", "gp_seed_net_pairs", "gp_seed_net_fit", "gp_seed_net_eval",
     "gp_part_file_write", "gp_part_file_info", "gp_part_file_read", "gp_field_file_write",
-    "gp_field_file_read",
+    "gp_field_file_read", "gp_particle_stats",
 )
 SEED_NET_NPARAM = 5252
 
@@ -130,6 +130,7 @@ def load_library(path: str = LIB_PATH):
     L.gp_collision_grid.argtypes = [vp, ip, dp]
     L.gp_reset_timers.argtypes = [vp]
     L.gp_get_collision_counts.argtypes = [vp, C.c_int, ip, C.c_int]
+    L.gp_particle_stats.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
     cp = C.c_char_p
     L.gp_part_file_write.argtypes = [cp, C.c_int64, C.c_int, ip, dp, dp, C.c_double, dp]
     L.gp_part_file_info.argtypes = [cp, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
@@ -218,6 +219,7 @@ class Context:
     def set_species(self, rows):
         r = np.ascontiguousarray(np.atleast_2d(rows), dtype=np.float64)
         self._chk(self.lib.gp_set_species(self.h, r.shape[0], _ptr(r)))
+        self.nspec = r.shape[0]
 
     def set_particles(self, state, species):
         st = np.ascontiguousarray(state, dtype=np.float64)
@@ -253,6 +255,16 @@ class Context:
 
     def reset_timers(self):
         self._chk(self.lib.gp_reset_timers(self.h))
+
+    def particle_stats(self, npoly=None, ref_norm=True):
+        """STAT_PARTICLE level 1: (nspec, 41), row (J-1) npoly + IDP-1 =
+        MEAN_PART(1:41, J, IDP); npoly defaults to one class of nspec sizes.
+        ref_norm: the reference's normalisation by the last class's counts."""
+        npoly = npoly or self.nspec
+        out = np.zeros((self.nspec, 41))
+        self._chk(self.lib.gp_particle_stats(self.h, self.nspec, npoly, int(bool(ref_norm)),
+                                             _ptr(out)))
+        return out
 
     def collision_grid(self):
         nb = np.zeros(4, dtype=np.int32)

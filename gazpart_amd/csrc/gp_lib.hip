@@ -1473,6 +1473,121 @@ static int check_err(gp_ctx* ctx) {
   return map_err(ctx, e);
 }
 
+// ---------------------------------------------------------------------------
+// STAT_PARTICLE, level 1 (stat_particle.f90:169-450): per species row (one
+// per class J and size index IDP) the 41 entries of MEAN_PART(:, J, IDP).
+// Pass 1 sums the counts, velocities and angular velocities (body frame and
+// rotated to the lab frame as transform_basis.f90:43-77 does); pass 2 the
+// products of the fluctuations about those means, with the body-frame and
+// lab-frame inertia; a last pass forms entries 10, 39, 40, 41 from the sums
+// and divides everything by a count (entry 23 is that count).  Species rows
+// are class-major, npoly size indices per class.  The reference divides
+// every class's entries by NPM(IDP), which after its class loop holds the
+// counts of the last class (stat_particle.f90:436-449); ref_norm = 1 keeps
+// that, ref_norm = 0 divides each row by its own count.  Sums over
+// at most PST_LDS species go through LDS first, one global atomic per entry
+// and workgroup; larger tables take global atomics directly.  The order of
+// the floating-point sums differs from the reference's particle order.
+// ---------------------------------------------------------------------------
+constexpr int NSTP = 41;
+constexpr int PST_LDS = 8;
+
+template <typename VT>
+__device__ __forceinline__ void pst_load(const PState<VT>& p, int i, double u[3], double w[3],
+                                         double g[3], Rot<double>& R) {
+  u[0] = (double)p.u[i]; u[1] = (double)p.v[i]; u[2] = (double)p.w[i];
+  w[0] = (double)p.ox[i]; w[1] = (double)p.oy[i]; w[2] = (double)p.oz[i];
+  R = Rot<double>(p.qa[i], p.qb[i], p.qc[i], p.qd[i]);
+  R.mul(w[0], w[1], w[2], g[0], g[1], g[2]);
+}
+
+template <typename VT, int PASS>
+__global__ __launch_bounds__(BLOCK) void k_pstat(PState<VT> p, const Species* __restrict__ spec,
+                                                 int np, int nspec, double* acc,
+                                                 const double* __restrict__ mean) {
+  __shared__ double s_acc[PST_LDS * NSTP];
+  const bool lds = nspec <= PST_LDS;
+  if (lds) {
+    for (int t = threadIdx.x; t < nspec * NSTP; t += blockDim.x) s_acc[t] = 0.0;
+    __syncthreads();
+  }
+  auto add = [&](int sp, int k, double v) {
+    if (lds) atomicAdd(&s_acc[sp * NSTP + k], v);
+    else atomicAdd(&acc[(size_t)sp * NSTP + k], v);
+  };
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+    const int sp = p.sp[i];
+    double u[3], w[3], g[3];
+    Rot<double> R(1.0, 0.0, 0.0, 0.0);
+    pst_load(p, i, u, w, g, R);
+    if (PASS == 1) {
+      add(sp, 22, 1.0);
+      for (int c = 0; c < 3; ++c) {
+        add(sp, c, u[c]);
+        add(sp, 10 + c, w[c]);
+        add(sp, 13 + c, g[c]);
+      }
+    } else {
+      const double* m = mean + (size_t)sp * 10;
+      const double uf[3] = {u[0] - m[0], u[1] - m[1], u[2] - m[2]};
+      const double of[3] = {w[0] - m[3], w[1] - m[4], w[2] - m[5]};
+      const double gf[3] = {g[0] - m[6], g[1] - m[7], g[2] - m[8]};
+      const Species S = spec[sp];
+      // R diag(I) R^T as matmul(R, matmul(diag(I), transpose(R)))
+      const double ib[3] = {S.ixx, S.iyy, S.izz};
+      const double r[3][3] = {{R.r00, R.r01, R.r02}, {R.r10, R.r11, R.r12}, {R.r20, R.r21, R.r22}};
+      auto il = [&](int a, int b) {
+        return r[a][0] * (ib[0] * r[b][0]) + r[a][1] * (ib[1] * r[b][1]) + r[a][2] * (ib[2] * r[b][2]);
+      };
+      add(sp, 3, uf[0] * uf[0]); add(sp, 4, uf[1] * uf[1]); add(sp, 5, uf[2] * uf[2]);
+      add(sp, 6, uf[0] * uf[1]); add(sp, 7, uf[0] * uf[2]); add(sp, 8, uf[1] * uf[2]);
+      add(sp, 16, of[0] * of[0]); add(sp, 17, of[1] * of[1]); add(sp, 18, of[2] * of[2]);
+      add(sp, 19, of[0] * of[1]); add(sp, 20, of[0] * of[2]); add(sp, 21, of[1] * of[2]);
+      add(sp, 23, S.ixx * of[0] * of[0]); add(sp, 24, S.iyy * of[1] * of[1]);
+      add(sp, 25, S.izz * of[2] * of[2]);
+      add(sp, 26, gf[0] * gf[0]); add(sp, 27, gf[1] * gf[1]); add(sp, 28, gf[2] * gf[2]);
+      add(sp, 29, gf[0] * gf[1]); add(sp, 30, gf[0] * gf[2]); add(sp, 31, gf[1] * gf[2]);
+      add(sp, 32, il(0, 0) * gf[0] * gf[0]); add(sp, 33, il(1, 1) * gf[1] * gf[1]);
+      add(sp, 34, il(2, 2) * gf[2] * gf[2]); add(sp, 35, il(0, 1) * gf[0] * gf[1]);
+      add(sp, 36, il(0, 2) * gf[0] * gf[2]); add(sp, 37, il(1, 2) * gf[1] * gf[2]);
+    }
+  }
+  if (lds) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < nspec * NSTP; t += blockDim.x)
+      if (s_acc[t] != 0.0) atomicAdd(&acc[t], s_acc[t]);
+  }
+}
+
+// per row: the means for pass 2 (u v w, body omega, lab omega) and the count
+__global__ void k_pstat_mean(const double* __restrict__ acc, double* __restrict__ mean,
+                             int nspec) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nspec) return;
+  const double* a = acc + (size_t)s * NSTP;
+  const int src[9] = {0, 1, 2, 10, 11, 12, 13, 14, 15};
+  for (int c = 0; c < 9; ++c) mean[(size_t)s * 10 + c] = a[src[c]] / a[22];
+  mean[(size_t)s * 10 + 9] = a[22];
+}
+
+// entries 10, 39, 40, 41 from the sums, then the division by the count
+__global__ void k_pstat_final(const Species* __restrict__ spec, double* acc,
+                              const double* __restrict__ mean, int nspec, int npoly,
+                              int ref_norm) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nspec) return;
+  double* a = acc + (size_t)s * NSTP;
+  const Species S = spec[s];
+  a[9] = 0.5 * (a[3] + a[4] + a[5]);
+  a[38] = 0.5 * (S.ixx * a[16] + S.iyy * a[17] + S.izz * a[18]);
+  a[39] = 0.5 * (a[23] + a[24] + a[25]);
+  a[40] = a[9] + a[39];
+  const int nrow = ref_norm ? nspec - npoly + s % npoly : s;
+  const double n = mean[(size_t)nrow * 10 + 9];
+  for (int k = 0; k < NSTP; ++k) a[k] = a[k] / n;
+  a[22] = n;
+}
+
 static void grid_init(gp_ctx* ctx) {
   // COLLISION_INITIATION (collision_initiation.f90:50-102), INIT_GRID = 2,
   // NPROC = 1: DELTA_COLL = ZERO; the y/z range spans L + DY.
@@ -2490,6 +2605,51 @@ int gp_get_collision_counts(gp_ctx* ctx, int np, int* counts, int reset) {
     hipLaunchKernelGGL(k_fill<int>, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->ccount, 0,
                        (long long)np);
   HIPCHK(hipStreamSynchronize(ctx->st));
+  return check_err(ctx);
+}
+
+int gp_particle_stats(gp_ctx* ctx, int nspec, int npoly, int ref_norm, double* out) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (!out || nspec != ctx->nspec || npoly < 1 || nspec % npoly) return GP_ERR_ARG;
+  double *acc = nullptr, *mean = nullptr;
+  const size_t na = (size_t)nspec * NSTP, nm = (size_t)nspec * 10;
+  if (hipMalloc((void**)&acc, na * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&mean, nm * sizeof(double)) != hipSuccess) {
+    if (acc) hipFree(acc);
+    ctx->err = "particle statistics: out of device memory";
+    return GP_ERR_NOMEM;
+  }
+  const unsigned grid = std::max(1u, std::min(nblk(ctx->np), 2048u));
+  const unsigned sgrid = nblk(nspec);
+  hipMemsetAsync(acc, 0, na * sizeof(double), ctx->st);
+  if (ctx->mixed) {
+    auto ps = pstate<float>(ctx, ctx->cur);
+    hipLaunchKernelGGL((k_pstat<float, 1>), dim3(grid), dim3(BLOCK), 0, ctx->st, ps, ctx->spec,
+                       ctx->np, nspec, acc, (const double*)nullptr);
+    hipLaunchKernelGGL(k_pstat_mean, dim3(sgrid), dim3(BLOCK), 0, ctx->st, acc, mean, nspec);
+    hipLaunchKernelGGL((k_pstat<float, 2>), dim3(grid), dim3(BLOCK), 0, ctx->st, ps, ctx->spec,
+                       ctx->np, nspec, acc, (const double*)mean);
+  } else {
+    auto ps = pstate<double>(ctx, ctx->cur);
+    hipLaunchKernelGGL((k_pstat<double, 1>), dim3(grid), dim3(BLOCK), 0, ctx->st, ps,
+                       ctx->spec, ctx->np, nspec, acc, (const double*)nullptr);
+    hipLaunchKernelGGL(k_pstat_mean, dim3(sgrid), dim3(BLOCK), 0, ctx->st, acc, mean, nspec);
+    hipLaunchKernelGGL((k_pstat<double, 2>), dim3(grid), dim3(BLOCK), 0, ctx->st, ps,
+                       ctx->spec, ctx->np, nspec, acc, (const double*)mean);
+  }
+  hipLaunchKernelGGL(k_pstat_final, dim3(sgrid), dim3(BLOCK), 0, ctx->st, ctx->spec, acc,
+                     (const double*)mean, nspec, npoly, ref_norm ? 1 : 0);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, acc, na * sizeof(double), hipMemcpyDeviceToHost,
+                                          ctx->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
+  hipFree(acc);
+  hipFree(mean);
+  if (e != hipSuccess) {
+    ctx->err = std::string("particle statistics: ") + hipGetErrorString(e);
+    return GP_ERR_HIP;
+  }
   return check_err(ctx);
 }
 

@@ -182,6 +182,15 @@ int gp_collision(gp_ctx *ctx, int ncycle, double time);
 int gp_step(gp_ctx *ctx, int nsteps);
 int gp_get_stats(gp_ctx *ctx, gp_stats *out);
 
+/* STAT_PARTICLE, level 1 (stat_particle.f90:169-450) over the resident
+ * particles: out[s][k-1] = MEAN_PART(k, J, IDP), k = 1..41, species rows
+ * class-major with npoly size indices per class (s = (J-1) npoly + IDP-1,
+ * nspec = the species count, a multiple of npoly).  ref_norm = 1 divides as
+ * the reference does, every class by the last class's counts per size
+ * index (stat_particle.f90:436-449: NPM(IDP) keeps the last class loop's
+ * values); ref_norm = 0 divides each row by its own count.  The sums run in
+ * another order than the reference's particle loop.                      */
+int gp_particle_stats(gp_ctx *ctx, int nspec, int npoly, int ref_norm, double *out);
 /* zero the accumulated stage times                                       */
 int gp_reset_timers(gp_ctx *ctx);
 /* introspection for benchmarks: collision grid of the context           */

@@ -11,6 +11,8 @@
 !!   part_io_ref pread <in.raw> <out.raw>  part_pJJ.ini -> out.raw (shapes from in.raw)
 !!   part_io_ref fsave <in.raw> <name>     one field component -> SAVE_MPIIO
 !!   part_io_ref fread <name> <n> <out.raw> READ_MPIIO -> out.raw
+!!   part_io_ref pstat <in.raw> <out.raw>  classes from in.raw -> STAT_PARTICLE
+!!                                         (level 1) -> MEAN_PART(1:41, J, IDP)
 !!
 !! in.raw for psave / pread (stream, native endianness): int32 nig, npoly,
 !! npart, n; double L; int32 npclass(npoly, nig); double rhop(npoly, nig),
@@ -26,6 +28,7 @@ program part_io_ref
   use PARAM_PHYS
   use PARTICLE_PARALLEL
   use MPI_STRUCTURES
+  use STATISTICS
   use gpref_driver
   implicit none
   character(len=256) :: mode, a1, a2
@@ -43,7 +46,7 @@ program part_io_ref
   DEBUG = .false.
 
   select case (trim(mode))
-  case ('psave', 'pread')
+  case ('psave', 'pread', 'pstat')
     open(unit=40, file=trim(a1), access='stream', form='unformatted', status='old')
     read(40) ncls, npoly, npart, n
     read(40) L
@@ -66,6 +69,28 @@ program part_io_ref
       close(40)
       read(a2, *) ncycle
       call SAVE_PARTICLE(ncycle)
+    else if (trim(mode) == 'pstat') then
+      do j = 1, ncls
+        read(40) st, idp
+        call ref_set_particles(j, npart, st, idp)
+      end do
+      close(40)
+      !! one level-1 pass; with STAT_TIME the routine adds MEAN_PART to the
+      !! zeroed MEAN_TIME_PART, which is read back at full precision
+      NSTAT = 100
+      LEVEL1_STPAR = .true.; LEVEL2_STPAR = .false.; LEVEL3_STPAR = .false.
+      LEVEL1_STPDF = .false.; STAT_TIME = .true.
+      if (allocated(MEAN_TIME_PART)) deallocate(MEAN_TIME_PART)
+      allocate(MEAN_TIME_PART(NSTAT, NIG, POLYDISPMAX))
+      MEAN_TIME_PART = 0.0d0
+      call STAT_PARTICLE(1, 0.0d0)
+      open(unit=41, file=trim(a2), access='stream', form='unformatted', status='replace')
+      do j = 1, ncls
+        do i = 1, npoly
+          write(41) MEAN_TIME_PART(1:41, j, i)
+        end do
+      end do
+      close(41)
     else
       close(40)
       !! room for every particle of the file in each class
