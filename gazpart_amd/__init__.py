@@ -48,7 +48,7 @@ EXPORTS = (
     "gp_get_collision_counts", "gp_seed_net_set", "gp_seed_net_get", "gp_seed_net_forward", "gp_seed_net_loss_grad",
     "gp_seed_net_train", "gp_seed_net_pairs", "gp_seed_net_fit", "gp_seed_net_eval",
     "gp_part_file_write", "gp_part_file_info", "gp_part_file_read", "gp_field_file_write",
-    "gp_field_file_read", "gp_particle_stats",
+    "gp_field_file_read", "gp_particle_stats", "gp_set_collision_stats", "gp_get_collision_stats",
 )
 SEED_NET_NPARAM = 5252
 
@@ -131,6 +131,8 @@ def load_library(path: str = LIB_PATH):
     L.gp_reset_timers.argtypes = [vp]
     L.gp_get_collision_counts.argtypes = [vp, C.c_int, ip, C.c_int]
     L.gp_particle_stats.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
+    L.gp_set_collision_stats.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.gp_get_collision_stats.argtypes = [vp, dp, dp, C.POINTER(C.c_int64)]
     cp = C.c_char_p
     L.gp_part_file_write.argtypes = [cp, C.c_int64, C.c_int, ip, dp, dp, C.c_double, dp]
     L.gp_part_file_info.argtypes = [cp, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
@@ -309,6 +311,24 @@ class Context:
         out = np.empty(self.np, dtype=np.int32)
         self._chk(self.lib.gp_get_collision_counts(self.h, self.np, _ptr(out), 1 if reset else 0))
         return out
+
+    def set_collision_stats(self, nclass=1, npoly=None, fout2=1, ncycle=1):
+        """MEAN_PART_COL of collision_interaction.f90:330-453 on every
+        collision pass whose cycle number is a multiple of fout2 (0: off);
+        npoly defaults to one class of nspec sizes."""
+        npoly = npoly or self.nspec // nclass
+        self._cs_shape = (nclass, npoly, npoly, 8)
+        self._chk(self.lib.gp_set_collision_stats(self.h, nclass, npoly, fout2, ncycle))
+
+    def collision_stats(self):
+        """{"last": (nclass, npoly, npoly, 8) MEAN_PART_COL(1:8, IDP1, IDP2)
+        of each class at the last statistics pass, "time_sum": their sum
+        over statistics passes, "nstat": that number of passes}"""
+        last = np.zeros(self._cs_shape)
+        tsum = np.zeros(self._cs_shape)
+        n = C.c_int64(0)
+        self._chk(self.lib.gp_get_collision_stats(self.h, _ptr(last), _ptr(tsum), C.byref(n)))
+        return {"last": last, "time_sum": tsum, "nstat": n.value}
 
     def step(self, nsteps=1):
         """nsteps x (ADV_PARTICLE_ELLP + COLLISION), asynchronous."""

@@ -77,6 +77,34 @@ struct WorkRec {     // a candidate pair that passed the prefilter
   int i, j;
 };
 
+// a contact (flag true) of a statistics step, with the quantities
+// collision_interaction.f90:330-345 sums for it when the ordered pass
+// reaches it with both particles still free
+struct CollStatRec {
+  long long k;        // candidate-list order
+  int i, j;           // particle slots (HOME, NEAR)
+  int bin;            // (class, IDP1, IDP2)
+  int approach;       // NVREL < ZERO: a collision when reached
+  double depth, nvrel, nrm, theta;
+};
+
+// the collision-statistics buffers as the kernels see them; rec == nullptr
+// when the statistics are off
+struct CStat {
+  CollStatRec* rec;
+  unsigned long long* cnt;   // 0 records 1 statistics steps
+  unsigned long long cap;
+  long long* cycle;
+  double* acc;               // nbin x 6 sums of the current step
+  double* last;              // nbin x 8
+  double* tsum;              // nbin x 8
+  int fout2, npoly, nbin;
+};
+
+__device__ __forceinline__ bool cstat_step(const CStat& cs) {
+  return cs.rec && (*cs.cycle % (long long)cs.fout2) == 0;
+}
+
 __device__ __forceinline__ int block_index(int xcd_remap) {
   // AMD dispatches consecutive workgroups round-robin over the 8 XCDs;
   // remapping gives each XCD a contiguous range of (cell-sorted) particles
@@ -812,6 +840,28 @@ __global__ __launch_bounds__(BLOCK) void k_pw_filter(
   }
 }
 
+// one statistics record per contact (collision_interaction.f90:318-345):
+// relative velocity (vx, vy, vz) = VREL, (dx, dy, dz) = EPT1 - EPT2
+__device__ __noinline__ void cstat_record(const CStat* __restrict__ csp, long long k, int i, int j,
+                                          int s1, int s2, double nvrel, double vx, double vy,
+                                          double vz, double dx, double dy, double dz,
+                                          unsigned* err) {
+  const CStat& cs = *csp;
+  if (!cstat_step(cs)) return;
+  CollStatRec r;
+  r.k = k; r.i = i; r.j = j;
+  r.bin = s1 * cs.npoly + s2 % cs.npoly;   // (class, IDP1) row s1, then IDP2
+  if (r.bin >= cs.nbin) return;
+  r.approach = nvrel < ZERO_R;
+  r.depth = sqrt(dx * dx + dy * dy + dz * dz);
+  r.nvrel = nvrel;
+  r.nrm = sqrt(vx * vx + vy * vy + vz * vz);
+  r.theta = acos(nvrel / r.nrm);
+  const unsigned long long a = atomicAdd(cs.cnt, 1ull);
+  if (a < cs.cap) cs.rec[a] = r;
+  else atomicOr(err, EB_ACTIVE_OVERFLOW);
+}
+
 // LM contact solve on the work list + NVREL test
 // (collision_interaction.f90:221-348); approaching contacts are appended.
 // One pair per lane and one wavefront per workgroup, launched with one
@@ -832,7 +882,8 @@ __global__ __launch_bounds__(LM_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) v
     const unsigned long long* nwork, unsigned long long cap, double L,
     ActiveRec* __restrict__ act, unsigned long long* nact, unsigned long long acap,
     unsigned long long* nfev_sum, unsigned long long* ncontact, unsigned* err,
-    const double* __restrict__ seeds, unsigned long long* nfallback) {
+    const double* __restrict__ seeds, unsigned long long* nfallback,
+    const CStat* __restrict__ csp) {
   const unsigned long long nw = min(*nwork, cap);
   const unsigned long long waves = gridDim.x;
   const int lane = threadIdx.x;
@@ -882,6 +933,12 @@ __global__ __launch_bounds__(LM_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) v
           ar.k = r.k; ar.i = r.i; ar.j = r.j;
           for (int c = 0; c < 3; ++c) { ar.r1[c] = r1[c]; ar.r2[c] = r2[c]; ar.cn[c] = o.cn[c]; }
         }
+        // statistics records out of line: the solve's registers stay as
+        // they are (the stores run only on statistics steps)
+        if (csp)
+          cstat_record(csp, r.k, r.i, r.j, p.sp[r.i], p.sp[r.j], nvrel, v2[0] - v1[0],
+                       v2[1] - v1[1], v2[2] - v1[2], o.e1[0] - o.e2[0], o.e1[1] - o.e2[1],
+                       o.e1[2] - o.e2[2], err);
       }
     }
     const unsigned long long m = __ballot(approach);
@@ -1158,7 +1215,7 @@ __global__ __launch_bounds__(BLOCK) void k_greedy_impulse(
     PState<VT> p, const Species* __restrict__ spec, const ActiveRec* __restrict__ act,
     const unsigned long long* nact, unsigned long long acap, int* __restrict__ used,
     const unsigned char* __restrict__ status, unsigned long long* ncoll, unsigned* err,
-    int* __restrict__ ccount) {
+    int* __restrict__ ccount, long long* __restrict__ acc_key) {
   const long long na = (long long)min(*nact, acap);
   for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < na;
        b0 += (long long)gridDim.x * blockDim.x) {
@@ -1170,6 +1227,12 @@ __global__ __launch_bounds__(BLOCK) void k_greedy_impulse(
       used[r.j] = 0;
       if (status[a] == 1) {
         acc = true;
+        // for the collision statistics: the list position of the collision
+        // that used each particle (k_cstat_reset restores the idle value)
+        if (acc_key) {
+          acc_key[r.i] = r.k;
+          acc_key[r.j] = r.k;
+        }
         double u1[3] = {(double)p.u[r.i], (double)p.v[r.i], (double)p.w[r.i]};
         double o1[3] = {(double)p.ox[r.i], (double)p.oy[r.i], (double)p.oz[r.i]};
         double u2[3] = {(double)p.u[r.j], (double)p.v[r.j], (double)p.w[r.j]};
@@ -1191,6 +1254,84 @@ __global__ __launch_bounds__(BLOCK) void k_greedy_impulse(
     const unsigned long long m = __ballot(acc);
     if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1)
       atomicAdd(ncoll, (unsigned long long)__popcll(m));
+  }
+}
+
+// Level-1 collision statistics (collision_interaction.f90:330-345, 409,
+// 423-453).  The reference's ordered pass reaches a contact with both
+// particles free exactly when no collision earlier in the list used either
+// of them: with acc_key[p] the list position of the collision that used p
+// (LLONG_MAX for none), a contact k is reached iff acc_key >= k at both
+// (equal when it is that collision).  Reached contacts add to their bin
+// (IDP1, IDP2): count, overlap depth, NVREL, |VREL|, angle; reached
+// approaching ones are the collisions.  Sums go through LDS per workgroup,
+// then one global atomic per bin and quantity.
+constexpr int CS_NQ = 6;            // count, collisions, depth, nvrel, |vrel|, theta
+constexpr int CS_LDS_BINS = 512;    // bins summed through LDS (24 KiB)
+
+__global__ __launch_bounds__(BLOCK) void k_cstat_accum(CStat cs,
+                                                       const long long* __restrict__ acc_key) {
+  __shared__ double sacc[CS_LDS_BINS * CS_NQ];
+  const bool lds = cs.nbin <= CS_LDS_BINS;
+  const long long n = (long long)min(*cs.cnt, cs.cap);
+  if (lds) {
+    for (int t = threadIdx.x; t < cs.nbin * CS_NQ; t += blockDim.x) sacc[t] = 0.0;
+    __syncthreads();
+  }
+  for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < n;
+       a += (long long)gridDim.x * blockDim.x) {
+    const CollStatRec r = cs.rec[a];
+    if (acc_key[r.i] < r.k || acc_key[r.j] < r.k) continue;
+    const double q[CS_NQ] = {1.0, r.approach ? 1.0 : 0.0, r.depth, r.nvrel, r.nrm, r.theta};
+    double* dst = lds ? sacc + r.bin * CS_NQ : cs.acc + (size_t)r.bin * CS_NQ;
+    for (int c = 0; c < CS_NQ; ++c) atomicAdd(dst + c, q[c]);
+  }
+  if (lds) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < cs.nbin * CS_NQ; t += blockDim.x)
+      if (sacc[t] != 0.0) atomicAdd(cs.acc + t, sacc[t]);
+  }
+}
+
+// acc_key back to idle over the accepted collisions
+__global__ __launch_bounds__(BLOCK) void k_cstat_reset(const ActiveRec* __restrict__ act,
+                                                       const unsigned long long* nact,
+                                                       unsigned long long acap,
+                                                       const unsigned char* __restrict__ status,
+                                                       long long* __restrict__ acc_key) {
+  const long long na = (long long)min(*nact, acap);
+  for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na;
+       a += (long long)gridDim.x * blockDim.x) {
+    if (status[a] != 1) continue;
+    acc_key[act[a].i] = LLONG_MAX;
+    acc_key[act[a].j] = LLONG_MAX;
+  }
+}
+
+// MEAN_PART_COL of the step (collision_interaction.f90:423-441) into last[]
+// and added to tsum[] (:453, the STAT_TIME sum); the sums start again from
+// the reference's ZERO (1e-33, :171), so that a bin no contact reached reads
+// ZERO in 1, 2, 4, 6-8 and 1 in 5, as the reference's does.  The record
+// count is cleared and the cycle number advanced every step.  One workgroup.
+__global__ __launch_bounds__(BLOCK) void k_cstat_final(CStat cs, double dtime) {
+  const long long cyc = *cs.cycle;
+  const bool on = (cyc % (long long)cs.fout2) == 0;
+  __syncthreads();
+  if (on) {
+    for (int b = threadIdx.x; b < cs.nbin; b += blockDim.x) {
+      double* a = cs.acc + (size_t)b * CS_NQ;
+      const double m[8] = {a[0], a[1], 2.0 * a[1] / dtime, a[2], a[2] / a[0], a[3], a[4], a[5]};
+      for (int c = 0; c < 8; ++c) {
+        cs.last[(size_t)b * 8 + c] = m[c];
+        cs.tsum[(size_t)b * 8 + c] += m[c];
+      }
+      for (int c = 0; c < CS_NQ; ++c) a[c] = ZERO_R;   // MEAN_PART_COL = ZERO (:171)
+    }
+  }
+  if (threadIdx.x == 0) {
+    if (on) cs.cnt[1] += 1ull;
+    cs.cnt[0] = 0ull;
+    *cs.cycle = cyc + 1;
   }
 }
 
@@ -1374,6 +1515,16 @@ struct gp_ctx {
   int g_cur = -1;
   int* g_perm = nullptr;
   bool g_perm_identity = false;
+  // level-1 collision statistics (off while cs_fout2 == 0): records of the
+  // step's contacts, per-bin sums (6 per bin), the last statistics step's
+  // MEAN_PART_COL (8 per bin) and their running sum (8 per bin)
+  int cs_fout2 = 0, cs_nclass = 0, cs_npoly = 0, cs_nbin = 0;
+  CollStatRec* cs_rec = nullptr;
+  unsigned long long cs_cap = 0;
+  double* cs_buf = nullptr;              // acc | last | tsum
+  unsigned long long* cs_cnt = nullptr;  // 0 records 1 statistics steps
+  long long* cs_cycle = nullptr;         // cycle number of the next collision pass
+  CStat* cs_dev = nullptr;               // device copy of cstat() for k_contact
 };
 
 static void drop_graph(gp_ctx* ctx) {
@@ -1706,13 +1857,15 @@ int gp_create(const gp_params* p, gp_ctx** out) {
 int gp_destroy(gp_ctx* ctx) {
   if (!ctx) return GP_ERR_ARG;
   hipStreamSynchronize(ctx->st);
+  hipStreamSynchronize(ctx->st2);
   void* bufs[] = {ctx->field, ctx->spec, ctx->pbuf[0], ctx->pbuf[1], ctx->perm, ctx->cell,
                   ctx->slot, ctx->cnt, ctx->off, ctx->tmp, ctx->sorted, ctx->pcount,
                   ctx->poff, ctx->scan_scr, ctx->mark, ctx->used, ctx->pos_of_id,
                   ctx->work, ctx->lmwork, ctx->brick, ctx->bslot, ctx->bcnt, ctx->boff, ctx->trav,
                   ctx->pos4, ctx->csort, ctx->inlm, ctx->plist, ctx->inv, ctx->ccount, ctx->mlp_th, ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, ctx->mlp_wb,
                   ctx->mlp_bias, ctx->mlp_part, ctx->mlp_lpart, ctx->mlp_loss, ctx->seeds,
-                  ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout};
+                  ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout,
+                  ctx->cs_rec, ctx->cs_buf, ctx->cs_cnt, ctx->cs_cycle, ctx->cs_dev};
   for (void* b : bufs)
     if (b) hipFree(b);
   for (auto& e : ctx->ev)
@@ -2192,6 +2345,8 @@ static int launch_sort_count(gp_ctx* ctx) {
 // work / active lists able to hold `need` prefilter survivors.  Kernels
 // already queued may still use the current lists (the eager step that sizes
 // them): both streams are drained before anything is freed.
+static int sync_cstat(gp_ctx* ctx);
+
 static int ensure_lists(gp_ctx* ctx, unsigned long long need) {
   int rc;
   const unsigned long long acap_need =
@@ -2215,6 +2370,38 @@ static int ensure_lists(gp_ctx* ctx, unsigned long long need) {
     if ((rc = dalloc(ctx, &ctx->act_status, (size_t)acap))) return rc;
     ctx->act_cap = acap;
   }
+  // statistics records: at most one per contact solve
+  if (ctx->cs_fout2 > 0 && ctx->work_cap > ctx->cs_cap) {
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st2));
+    drop_graph(ctx);
+    if ((rc = dalloc(ctx, &ctx->cs_rec, (size_t)ctx->work_cap))) return rc;
+    ctx->cs_cap = ctx->work_cap;
+    return sync_cstat(ctx);
+  }
+  return GP_OK;
+}
+
+static CStat cstat(gp_ctx* ctx) {
+  CStat cs{};
+  if (ctx->cs_fout2 <= 0 || !ctx->cs_rec) return cs;
+  cs.rec = ctx->cs_rec;
+  cs.cnt = ctx->cs_cnt;
+  cs.cap = ctx->cs_cap;
+  cs.cycle = ctx->cs_cycle;
+  cs.acc = ctx->cs_buf;
+  cs.last = cs.acc + (size_t)ctx->cs_nbin * CS_NQ;
+  cs.tsum = cs.last + (size_t)ctx->cs_nbin * 8;
+  cs.fout2 = ctx->cs_fout2;
+  cs.npoly = ctx->cs_npoly;
+  cs.nbin = ctx->cs_nbin;
+  return cs;
+}
+
+// the device copy k_contact reads (both streams idle)
+static int sync_cstat(gp_ctx* ctx) {
+  const CStat cs = cstat(ctx);
+  HIPCHK(hipMemcpy(ctx->cs_dev, &cs, sizeof(cs), hipMemcpyHostToDevice));
   return GP_OK;
 }
 
@@ -2264,9 +2451,11 @@ static int launch_contacts_solve(gp_ctx* ctx) {
   if (net)
     hipLaunchKernelGGL(k_mlp_seed<VT>, dim3(ctx->lm_waves), dim3(64), 0, ctx->st, ps, ctx->spec,
                        ctx->lmwork, c + 7, ctx->work_cap, ctx->prm.L, mlp_dev(ctx), ctx->seeds);
+  const CStat cs = cstat(ctx);
   hipLaunchKernelGGL(k_contact<VT>, dim3(ctx->lm_waves), dim3(LM_BLOCK), 0, ctx->st, ps, ctx->spec,
                      ctx->lmwork, c + 7, ctx->work_cap, ctx->prm.L, ctx->act, c + 2, ctx->act_cap,
-                     c + 3, c + 4, ctx->d_err, net ? ctx->seeds : nullptr, c + 9);
+                     c + 3, c + 4, ctx->d_err, net ? ctx->seeds : nullptr, c + 9,
+                     cs.rec ? ctx->cs_dev : nullptr);
   mark(ctx, 3);
   hipLaunchKernelGGL(k_greedy_mark, dim3(GREEDY_GRID), dim3(BLOCK), 0, ctx->st, ctx->act, c + 2,
                      ctx->act_cap, ctx->mark);
@@ -2278,7 +2467,15 @@ static int launch_contacts_solve(gp_ctx* ctx) {
                      ctx->act_cap, ctx->mark, ctx->used, ctx->act_status, c + 8, c + 6);
   hipLaunchKernelGGL(k_greedy_impulse<VT>, dim3(GREEDY_GRID), dim3(BLOCK), 0, ctx->st, ps,
                      ctx->spec, ctx->act, c + 2, ctx->act_cap, ctx->used, ctx->act_status, c + 5,
-                     ctx->d_err, ctx->ccount);
+                     ctx->d_err, ctx->ccount, cs.rec ? ctx->mark : nullptr);
+  if (cs.rec) {
+    // the greedy pass leaves mark[] idle (LLONG_MAX): it carries acc_key here
+    const unsigned sgrid = (unsigned)std::min<unsigned long long>((cs.cap + BLOCK - 1) / BLOCK, 64ull);
+    hipLaunchKernelGGL(k_cstat_accum, dim3(sgrid), dim3(BLOCK), 0, ctx->st, cs, ctx->mark);
+    hipLaunchKernelGGL(k_cstat_reset, dim3(GREEDY_GRID), dim3(BLOCK), 0, ctx->st, ctx->act, c + 2,
+                       ctx->act_cap, ctx->act_status, ctx->mark);
+    hipLaunchKernelGGL(k_cstat_final, dim3(1), dim3(BLOCK), 0, ctx->st, cs, ctx->prm.dtime);
+  }
   HIPCHK(hipGetLastError());
   mark(ctx, 4);
   ctx->ev_pending = ctx->timing;
@@ -2537,10 +2734,13 @@ int gp_contact_batch(gp_ctx* ctx, int64_t npairs, const int* home, const int* ne
 }
 
 int gp_collision(gp_ctx* ctx, int ncycle, double time) {
-  (void)ncycle;
   (void)time;
   int rc = ready(ctx);
   if (rc) return rc;
+  // the statistics gate of collision_interaction.f90:330 on this cycle
+  if (ctx->cs_fout2 > 0)
+    hipLaunchKernelGGL(k_fill<long long>, dim3(1), dim3(1), 0, ctx->st, ctx->cs_cycle,
+                       (long long)ncycle, 1ll);
   rc = ctx->mixed ? launch_collide<float>(ctx) : launch_collide<double>(ctx);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
@@ -2548,6 +2748,69 @@ int gp_collision(gp_ctx* ctx, int ncycle, double time) {
   rc = check_err(ctx);
   accumulate_stats(ctx);
   return rc;
+}
+
+int gp_set_collision_stats(gp_ctx* ctx, int nclass, int npoly, int fout2, int ncycle) {
+  if (!ctx) return GP_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st2));
+  drop_graph(ctx);
+  void* bufs[] = {ctx->cs_rec, ctx->cs_buf, ctx->cs_cnt, ctx->cs_cycle, ctx->cs_dev};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  ctx->cs_rec = nullptr; ctx->cs_buf = nullptr; ctx->cs_cnt = nullptr; ctx->cs_cycle = nullptr;
+  ctx->cs_dev = nullptr;
+  ctx->cs_cap = 0;
+  ctx->cs_fout2 = 0;
+  if (fout2 <= 0) return GP_OK;
+  if (nclass < 1 || npoly < 1 || (long long)nclass * npoly * npoly > GP_CSTAT_MAX_BINS) {
+    ctx->err = "collision statistics: need nclass, npoly >= 1 and nclass * npoly^2 <= " +
+               std::to_string(GP_CSTAT_MAX_BINS) + " bins";
+    return GP_ERR_ARG;
+  }
+  const int nbin = nclass * npoly * npoly;
+  int rc;
+  if ((rc = dalloc(ctx, &ctx->cs_buf, (size_t)nbin * (CS_NQ + 16)))) return rc;
+  if ((rc = dalloc(ctx, &ctx->cs_cnt, 2))) return rc;
+  if ((rc = dalloc(ctx, &ctx->cs_cycle, 1))) return rc;
+  const unsigned long long cap = std::max<unsigned long long>(ctx->work_cap, 4096ull);
+  if ((rc = dalloc(ctx, &ctx->cs_rec, (size_t)cap))) return rc;
+  ctx->cs_cap = cap;
+  // sums and their running sum from ZERO (collision_interaction.f90:171,
+  // collision_initiation.f90:172), the last pass's values from 0
+  std::vector<double> init((size_t)nbin * (CS_NQ + 16), 0.0);
+  for (size_t t = 0; t < (size_t)nbin * CS_NQ; ++t) init[t] = ZERO_R;
+  for (size_t t = (size_t)nbin * (CS_NQ + 8); t < init.size(); ++t) init[t] = ZERO_R;
+  HIPCHK(hipMemcpy(ctx->cs_buf, init.data(), init.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(ctx->cs_cnt, 0, 2 * sizeof(unsigned long long)));
+  const long long cyc = ncycle;
+  HIPCHK(hipMemcpy(ctx->cs_cycle, &cyc, sizeof(cyc), hipMemcpyHostToDevice));
+  ctx->cs_nclass = nclass;
+  ctx->cs_npoly = npoly;
+  ctx->cs_nbin = nbin;
+  ctx->cs_fout2 = fout2;
+  if ((rc = dalloc(ctx, &ctx->cs_dev, 1))) return rc;
+  return sync_cstat(ctx);
+}
+
+int gp_get_collision_stats(gp_ctx* ctx, double* last, double* time_sum, int64_t* nstat) {
+  if (!ctx) return GP_ERR_ARG;
+  if (ctx->cs_fout2 <= 0) {
+    ctx->err = "collision statistics are off (gp_set_collision_stats)";
+    return GP_ERR_STATE;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  const size_t nb = (size_t)ctx->cs_nbin;
+  const double* dlast = ctx->cs_buf + nb * CS_NQ;
+  if (last) HIPCHK(hipMemcpy(last, dlast, nb * 8 * sizeof(double), hipMemcpyDeviceToHost));
+  if (time_sum)
+    HIPCHK(hipMemcpy(time_sum, dlast + nb * 8, nb * 8 * sizeof(double), hipMemcpyDeviceToHost));
+  if (nstat) {
+    unsigned long long h[2];
+    HIPCHK(hipMemcpy(h, ctx->cs_cnt, sizeof(h), hipMemcpyDeviceToHost));
+    *nstat = (int64_t)h[1];
+  }
+  return check_err(ctx);
 }
 
 int gp_step(gp_ctx* ctx, int nsteps) {

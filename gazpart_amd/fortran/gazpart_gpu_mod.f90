@@ -222,6 +222,68 @@ module gazpart_gpu
       integer(c_int64_t), value :: npairs, seed
       type(gp_seed_report), intent(out) :: rep
     end function
+    ! statistics: out(41, nspec) = MEAN_PART(1:41, J, IDP), column (J-1)*npoly + IDP
+    integer(c_int) function gp_particle_stats(ctx, nspec, npoly, ref_norm, out) &
+        bind(C, name='gp_particle_stats')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nspec, npoly, ref_norm
+      real(c_double), intent(out) :: out(41, nspec)
+    end function
+    integer(c_int) function gp_set_collision_stats(ctx, nclass, npoly, fout2, ncycle) &
+        bind(C, name='gp_set_collision_stats')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nclass, npoly, fout2, ncycle
+    end function
+    ! last(8, npoly, npoly, nclass) and tsum(...) as (k, IDP2, IDP1, J)
+    integer(c_int) function gp_get_collision_stats(ctx, last, tsum, npass) &
+        bind(C, name='gp_get_collision_stats')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: last(*), tsum(*)
+      integer(c_int64_t), intent(out) :: npass
+    end function
+    ! on-disk formats (host only); path: a NUL-terminated string
+    integer(c_int) function gp_part_file_write(path, npart, npoly, npclass, rhop, emaj, apr, &
+                                               cols) bind(C, name='gp_part_file_write')
+      import :: c_int, c_int64_t, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int64_t), value :: npart
+      integer(c_int), value :: npoly
+      integer(c_int), intent(in) :: npclass(npoly)
+      real(c_double), intent(in) :: rhop(npoly), emaj(npoly)
+      real(c_double), value :: apr
+      real(c_double), intent(in) :: cols(npart, 14)
+    end function
+    integer(c_int) function gp_part_file_info(path, npart, npoly) bind(C, name='gp_part_file_info')
+      import :: c_int, c_int64_t, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int64_t), intent(out) :: npart
+      integer(c_int), intent(out) :: npoly
+    end function
+    integer(c_int) function gp_part_file_read(path, npart, npoly, npclass, rhop, emaj, apr, &
+                                              cols) bind(C, name='gp_part_file_read')
+      import :: c_int, c_int64_t, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int64_t), value :: npart
+      integer(c_int), value :: npoly
+      integer(c_int), intent(out) :: npclass(npoly)
+      real(c_double), intent(out) :: rhop(npoly), emaj(npoly), apr
+      real(c_double), intent(out) :: cols(npart, 14)
+    end function
+    integer(c_int) function gp_field_file_write(path, n, f) bind(C, name='gp_field_file_write')
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: f(n, n, n)
+    end function
+    integer(c_int) function gp_field_file_read(path, n, f) bind(C, name='gp_field_file_read')
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: n
+      real(c_double), intent(out) :: f(n, n, n)
+    end function
   end interface
 
 contains

@@ -17,7 +17,10 @@
 !! other routine of the reference sees the usual PART; with .false. the
 !! device copy is authoritative and GPU_SYNC_PART refreshes PART on demand
 !! (before statistics or output).  A non-zero status stops the run with
-!! the library's message, as the reference's own `stop`s do.
+!! the library's message, as the reference's own `stop`s do.  With
+!! LEVEL1_STPAR (at GPU_INIT), GPU_COLLISION also writes the level-1
+!! collision statistics and adds them to MEAN_TIME_PART_COL as
+!! COLLISION_INTERACTION does, from sums formed on the device.
 module gp_dropin
   use iso_c_binding
   use gazpart_gpu
@@ -110,6 +113,12 @@ contains
       rc = gp_set_species(GPU_CTX(J), ns, rows)
       call GPU_CHECK(rc, GPU_CTX(J), 'gp_set_species')
       deallocate(rows)
+      ! level-1 collision statistics (MEAN_PART_COL) summed on the device on
+      ! the cycles COLLISION_INTERACTION sums them (flags read here, once)
+      if (LEVEL1_STPAR .and. FOUT2 > 0) then
+        rc = gp_set_collision_stats(GPU_CTX(J), 1_c_int, int(ns, c_int), int(FOUT2, c_int), 1_c_int)
+        call GPU_CHECK(rc, GPU_CTX(J), 'gp_set_collision_stats')
+      end if
       call GPU_PUT_PART(J)
     end do
     deallocate(fld)
@@ -207,9 +216,43 @@ contains
       rc = gp_collision(GPU_CTX(J), int(NCYCLE, c_int), real(TIME, c_double))
       call GPU_CHECK(rc, GPU_CTX(J), 'COLLISION')
       if (GPU_MIRROR) call GPU_GET_PART(J)
+      if (LEVEL1_STPAR .and. FOUT2 > 0) then
+        if (mod(NCYCLE, FOUT2) == 0) call GPU_COLLISION_STATS(J, TIME)
+      end if
     end do
     if (.not. GPU_MIRROR) GPU_DEVICE_AHEAD = .true.
   end subroutine GPU_COLLISION
+
+  !! what COLLISION_INTERACTION does with one pass's MEAN_PART_COL
+  !! (collision_interaction.f90:423-453): one line per size pair on the
+  !! reference's statistics units, and the STAT_TIME sum MEAN_TIME_PART_COL;
+  !! the entries the pass does not form keep the reference's ZERO
+  subroutine GPU_COLLISION_STATS(J, TIME)
+    integer, intent(in) :: J
+    real(kind=8), intent(in) :: TIME
+    integer :: ns, IDP1, IDP2, NUMFILE
+    integer(c_int) :: rc
+    integer(c_int64_t) :: npass     ! (not nstat: that name is DNS_DIM's NSTAT)
+    real(c_double), allocatable :: last(:,:,:), tsum(:,:,:)
+    real(kind=8), allocatable :: mpc(:,:,:)
+    ns = POLYDISP(J)
+    allocate(last(8, ns, ns), tsum(8, ns, ns), mpc(NSTAT, POLYDISPMAX, POLYDISPMAX))
+    rc = gp_get_collision_stats(GPU_CTX(J), last, tsum, npass)
+    call GPU_CHECK(rc, GPU_CTX(J), 'gp_get_collision_stats')
+    mpc = ZERO
+    do IDP1 = 1, ns
+      do IDP2 = 1, ns
+        mpc(1:8, IDP1, IDP2) = last(:, IDP2, IDP1)
+        NUMFILE = 560 + sum(POLYDISP(1:J)**2) - POLYDISP(J)**2 + (IDP1 - 1)*POLYDISP(J) + IDP2
+        write(NUMFILE, 10000) TIME, mpc(1, IDP1, IDP2), mpc(2, IDP1, IDP2), mpc(3, IDP1, IDP2), &
+                              mpc(4, IDP1, IDP2)/EMAJ_PART(J, IDP1), mpc(5, IDP1, IDP2), &
+                              mpc(6, IDP1, IDP2), mpc(7, IDP1, IDP2), mpc(8, IDP1, IDP2)
+      end do
+    end do
+    if (STAT_TIME) MEAN_TIME_PART_COL = MEAN_TIME_PART_COL + mpc
+    deallocate(last, tsum, mpc)
+10000 format (30(e17.7))
+  end subroutine GPU_COLLISION_STATS
 
   !! INTERPH (interpH.f90:26), same interface; UAIN is passed to the GPU
   subroutine GPU_INTERPH(INT_SCHEME, XAIN, YAIN, ZAIN, UAIN, NINT, XINT, YINT, ZINT, UINT)

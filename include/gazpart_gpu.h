@@ -200,6 +200,32 @@ int gp_collision_grid(gp_ctx *ctx, int *nbox_xyz, double *box_min_size);
  * of collision_interaction.f90:330-345); reset != 0 zeroes them */
 int gp_get_collision_counts(gp_ctx *ctx, int np, int *counts, int reset);
 
+/* Level-1 collision statistics of COLLISION_INTERACTION (MEAN_PART_COL,
+ * collision_interaction.f90:330-345, 409, 423-441; with LEVEL1_STPAR and
+ * STAT_TIME, :453).  fout2 > 0 turns them on (fout2 <= 0 off): every
+ * collision pass whose cycle number is a multiple of fout2 sums, over the
+ * contacts the ordered pass reaches with both particles free, per class J
+ * and size pair (IDP1 of HOME, IDP2 of NEAR) with species rows class-major
+ * (s = (J-1) npoly + IDP-1; rows beyond nclass * npoly are not counted):
+ *   1 contacts, 2 collisions (NVREL < 0), 3 2 x (2) / DTIME,
+ *   4 sum of |EPT1 - EPT2|, 5 (4) / (1), 6 sum of NVREL, 7 sum of |VREL|,
+ *   8 sum of acos(NVREL / |VREL|),
+ * every sum starting from the reference's ZERO = 1e-33 (so a bin with no
+ * contact reads 1e-33 in 1, 2, 4, 6-8 and 1 in 5, as the reference's).
+ * gp_collision(ncycle) tests its own ncycle; gp_step numbers its passes
+ * on from the last one (the first is ncycle here).  Turning them on (or
+ * off) clears the sums and re-captures the step graph.  The sums run in
+ * another order than the reference's list loop.                         */
+#define GP_CSTAT_MAX_BINS 65536
+int gp_set_collision_stats(gp_ctx *ctx, int nclass, int npoly, int fout2, int ncycle);
+/* last[((J-1) npoly + IDP1-1) npoly + IDP2-1][k-1] = MEAN_PART_COL(k,
+ * IDP1, IDP2) of class J at the last statistics pass; time_sum[...] the
+ * sum over statistics passes from ZERO (MEAN_TIME_PART_COL,
+ * collision_initiation.f90:172; kept per class: the reference adds all
+ * classes into one array); nstat the number of
+ * statistics passes.  Any output may be NULL.                           */
+int gp_get_collision_stats(gp_ctx *ctx, double *last, double *time_sum, int64_t *nstat);
+
 /* contact-seed network -------------------------------------------------
  * No reference counterpart: a 12 -> 64 -> 64 -> 4 ReLU network (bf16
  * weights, fp32 accumulate on the matrix cores).  From a candidate pair's

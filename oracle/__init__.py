@@ -85,6 +85,9 @@ def lib():
         L.oc_collision_interaction.argtypes = [C.POINTER(Params), _dp, C.c_int, _dp, _ip,
                                                C.c_int64, _ip, _ip, C.c_void_p, C.c_void_p,
                                                C.POINTER(C.c_int), C.c_int]
+        L.oc_collision_interaction_stats.argtypes = [
+            C.POINTER(Params), _dp, C.c_int, _dp, _ip, C.c_int64, _ip, _ip, C.c_void_p,
+            C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -208,6 +211,23 @@ def interaction(p, spec, state, sid, home, near, nthreads=1):
                                          _c(near, np.int32), flags.ctypes.data, coll.ctypes.data,
                                          C.byref(nc), nthreads)
     return st, flags[:nclose], coll[:nclose], nc.value, err
+
+
+def interaction_stats(p, spec, state, sid, home, near, npoly, nbin, nthreads=1):
+    """interaction() with the level-1 collision statistics: also returns
+    col (nbin, 8) = MEAN_PART_COL(1:8) per (species[HOME], IDP of NEAR)"""
+    st = _c(state).copy()
+    np_ = st.shape[1]
+    nclose = len(home)
+    flags = np.zeros(max(nclose, 1), dtype=np.int32)
+    coll = np.zeros(max(nclose, 1), dtype=np.int32)
+    col = np.zeros((nbin, 8))
+    nc = C.c_int(0)
+    err = lib().oc_collision_interaction_stats(
+        C.byref(p), _c(spec).reshape(-1), np_, st.reshape(-1), _c(sid, np.int32), nclose,
+        _c(home, np.int32), _c(near, np.int32), flags.ctypes.data, coll.ctypes.data,
+        C.byref(nc), nthreads, npoly, nbin, col.reshape(-1))
+    return st, flags[:nclose], coll[:nclose], nc.value, err, col
 
 
 class Stepper:
@@ -377,6 +397,17 @@ class Ref:
         nc = C.c_int(0)
         f(ig, nmax, home, near, C.byref(nc))
         return home[:nc.value], near[:nc.value]
+
+    def cstat_on(self, fout2):
+        self.L.ref_cstat_on.argtypes = [C.c_int]
+        self.L.ref_cstat_on(fout2)
+
+    def cstat_get(self):
+        """MEAN_TIME_PART_COL(1:8, :, :) as [IDP1-1, IDP2-1, k-1]"""
+        self.L.ref_cstat_get.argtypes = [_dp]
+        out = np.zeros(8 * self.nsize * self.nsize)
+        self.L.ref_cstat_get(out)
+        return out.reshape(self.nsize, self.nsize, 8).transpose(1, 0, 2).copy()
 
     def collision(self, ig, ncycle, time, nmax):
         f = self.L.ref_collision

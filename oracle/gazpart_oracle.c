@@ -1402,7 +1402,19 @@ int oc_collision_interaction(const oc_params *p, const double *spec, int np,
                              double *st, const int *species, int64_t nclose,
                              const int *home, const int *near_, int *flag_out,
                              int *collided_out, int *ncoll, int nthreads) {
+  return oc_collision_interaction_stats(p, spec, np, st, species, nclose, home, near_,
+                                        flag_out, collided_out, ncoll, nthreads, 1, 0, NULL);
+}
+
+int oc_collision_interaction_stats(const oc_params *p, const double *spec, int np,
+                                   double *st, const int *species, int64_t nclose,
+                                   const int *home, const int *near_, int *flag_out,
+                                   int *collided_out, int *ncoll, int nthreads,
+                                   int npoly, int nbin, double *col) {
   const double L = p->L;
+  /* MEAN_PART_COL(:,:,:) = ZERO (collision_interaction.f90:171) */
+  if (col)
+    for (int64_t b = 0; b < (int64_t)nbin * 8; ++b) col[b] = ZERO_R;
   pair_res *res = (pair_res *)malloc(sizeof(pair_res) * (size_t)(nclose > 0 ? nclose : 1));
   (void)nthreads;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
@@ -1456,8 +1468,22 @@ int oc_collision_interaction(const oc_params *p, const double *spec, int np,
     double vp2 = s2[4] + (o2[2] * rx2[0] - o2[0] * rx2[2]);
     double wp2 = s2[5] + (o2[0] * rx2[1] - o2[1] * rx2[0]);
     double vrx = up2 - up1, vry = vp2 - vp1, vrz = wp2 - wp1;
+    double nrm_vrel = sqrt(vrx * vrx + vry * vry + vrz * vrz);
     double nvrel = vrx * r->cn[0] + vry * r->cn[1] + vrz * r->cn[2];
+    /* collision_interaction.f90:326-345 */
+    int64_t bin = (int64_t)species[i] * npoly + species[j] % npoly;
+    double *cb = (col && bin < nbin) ? col + bin * 8 : NULL;
+    if (cb) {
+      double theta = acos(nvrel / nrm_vrel);
+      double vpc[3] = {r->e1[0] - r->e2[0], r->e1[1] - r->e2[1], r->e1[2] - r->e2[2]};
+      cb[0] = cb[0] + 1.0;
+      cb[3] = cb[3] + sqrt(vpc[0] * vpc[0] + vpc[1] * vpc[1] + vpc[2] * vpc[2]);
+      cb[5] = cb[5] + nvrel;
+      cb[6] = cb[6] + nrm_vrel;
+      cb[7] = cb[7] + theta;
+    }
     if (nvrel < ZERO_R) {
+      if (cb) cb[1] = cb[1] + 1.0;   /* :409 */
       int e = oc_collision_response(spec + (int64_t)species[i] * OC_SP_STRIDE, s1, s2, q1, q2,
                                     rx1, rx2, r->cn);
       if (e && !err) err = e;
@@ -1472,6 +1498,12 @@ int oc_collision_interaction(const oc_params *p, const double *spec, int np,
   }
   free(used);
   free(res);
+  /* :423-441 */
+  if (col)
+    for (int64_t b = 0; b < nbin; ++b) {
+      col[b * 8 + 2] = 2.0 * col[b * 8 + 1] / p->dtime;
+      col[b * 8 + 4] = col[b * 8 + 3] / col[b * 8 + 0];
+    }
   if (ncoll) *ncoll = nc;
   return err;
 }

@@ -127,6 +127,16 @@ int oc_collision_interaction(const oc_params *p, const double *spec, int np,
                              int64_t nclose, const int *home,
                              const int *near_, int *flag_out,
                              int *collided_out, int *ncoll, int nthreads);
+/* the same pass with the level-1 collision statistics of
+ * collision_interaction.f90:330-345, 409, 423-441: col (nbin x 8, bin =
+ * species[HOME] * npoly + species[NEAR] % npoly) receives MEAN_PART_COL(1:8)
+ * of the pass, every entry starting from the reference's ZERO = 1e-33 */
+int oc_collision_interaction_stats(const oc_params *p, const double *spec, int np,
+                                   double *state, const int *species,
+                                   int64_t nclose, const int *home,
+                                   const int *near_, int *flag_out,
+                                   int *collided_out, int *ncoll, int nthreads,
+                                   int npoly, int nbin, double *col);
 int oc_step(const oc_params *p, const double *field, const double *spec,
             int np, double *state, const int *species, const oc_cgrid *g,
             const int *con, const int *ncon, int64_t nclosemax, int *home,

@@ -368,4 +368,26 @@ contains
     cv = c1(:, 1); p1 = pt1(:, 1); p2 = pt2(:, 1)
   end subroutine ref_pair_depth
 
+  !! level-1 collision statistics on: every later ref_collision whose cycle
+  !! is a multiple of fout2 adds its MEAN_PART_COL (collision_interaction.f90:
+  !! 330-441) to MEAN_TIME_PART_COL (:453), zeroed here.  The reference also
+  !! writes each such pass to unit 560 + ... (fort.NNN in the current
+  !! directory).
+  subroutine ref_cstat_on(fout2_) bind(C, name='ref_cstat_on')
+    integer(c_int), value :: fout2_
+    FOUT2 = fout2_
+    NSTAT = max(NSTAT, 8)
+    LEVEL1_STPAR = .true.; STAT_TIME = .true.
+    LEVEL1_STPDF = .false.; LEVEL0_STPAR = .false.; LEVELX_STPAR = .false.
+    if (allocated(MEAN_TIME_PART_COL)) deallocate(MEAN_TIME_PART_COL)
+    allocate(MEAN_TIME_PART_COL(NSTAT, POLYDISPMAX, POLYDISPMAX))
+    MEAN_TIME_PART_COL = 0.0d0
+  end subroutine ref_cstat_on
+
+  !! MEAN_TIME_PART_COL(1:8, IDP1, IDP2)
+  subroutine ref_cstat_get(out) bind(C, name='ref_cstat_get')
+    real(c_double), intent(out) :: out(8, POLYDISPMAX, POLYDISPMAX)
+    out = MEAN_TIME_PART_COL(1:8, :, :)
+  end subroutine ref_cstat_get
+
 end module gpref_driver

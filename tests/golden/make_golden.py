@@ -10,6 +10,8 @@ C-callable driver).  Each fixture holds inputs and the reference's outputs:
   golden_cells.npz       COLLISION_INITIATION grid, connectivity, HOME/NEAR
   golden_trajectory.npz  full ADV_PARTICLE_ELLP + COLLISION steps of a dense
                          small ensemble (collisions and wall bounces happen)
+  golden_collision_stats.npz  level-1 collision statistics (MEAN_PART_COL)
+                         of such steps with three particle sizes
 
 Usage:  python tests/golden/make_golden.py
 """
@@ -198,11 +200,57 @@ def trajectory(nsteps=30, scheme=3):
                 collided=np.array(colls), nclose=np.array(ncloses))
 
 
+def collision_stats(nsteps=10, fout2=2):
+    """Level-1 collision statistics (MEAN_PART_COL of COLLISION_INTERACTION)
+    of a dense ensemble with three sizes in one class, on every cycle that
+    is a multiple of fout2: per statistics pass the state it starts from
+    (after that cycle's advance) and its MEAN_PART_COL(1:8, IDP1, IDP2).
+    The reference also writes each pass to fort.NNN files: it runs in a
+    scratch directory."""
+    import tempfile
+    n = 12
+    fld, dtime, emaj0, apr, rhop0, st = dense_setup(n=n, np_=400, seed=41)
+    sizes = np.array([0.8, 1.0, 1.2])
+    emaj = emaj0 * sizes
+    rhop = rhop0 * np.ones(3)
+    np_ = st.shape[1]
+    idp = (W.rng(43).integers(0, 3, np_) + 1).astype(np.int32)
+    R = O.Ref()
+    R.setup(n, L, VISC, RHOF, dtime, 1.0, 0.0, 3, [emaj], [apr], [rhop])
+    R.set_field(fld)
+    R.set_particles(1, st, idp)
+    R.collision_init()
+    cwd = os.getcwd()
+    mids, cols, steps = [], [], []
+    with tempfile.TemporaryDirectory() as tmp:
+        os.chdir(tmp)
+        try:
+            for s in range(1, nsteps + 1):
+                R.adv(s)
+                if s % fout2 == 0:
+                    mid, _, _ = R.get_particles(1, np_)
+                    R.cstat_on(fout2)            # MEAN_TIME_PART_COL = 0
+                    R.collision(1, s, 0.0, 200 * np_)
+                    mids.append(mid)
+                    cols.append(R.cstat_get())   # = this pass's MEAN_PART_COL
+                    steps.append(s)
+                else:
+                    R.collision(1, s, 0.0, 200 * np_)
+            final, _, _ = R.get_particles(1, np_)
+        finally:
+            os.chdir(cwd)
+    return dict(n=n, L=L, visc=VISC, rhof=RHOF, dtime=dtime, emaj=emaj, apr=apr, rhop=rhop,
+                scheme=3, field=fld, state0=st, idp=idp, fout2=fout2, nsteps=nsteps,
+                stat_steps=np.array(steps), mid_states=np.array(mids), col=np.array(cols),
+                final=final)
+
+
 def main():
     np.savez_compressed(os.path.join(HERE, "golden_leaf.npz"), **leaf())
     np.savez_compressed(os.path.join(HERE, "golden_cells.npz"), **cells())
     np.savez_compressed(os.path.join(HERE, "golden_trajectory.npz"), **trajectory(30, 3))
     np.savez_compressed(os.path.join(HERE, "golden_trajectory_lag4.npz"), **trajectory(10, 4))
+    np.savez_compressed(os.path.join(HERE, "golden_collision_stats.npz"), **collision_stats())
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
