@@ -49,6 +49,7 @@ EXPORTS = (
     "gp_seed_net_train", "gp_seed_net_pairs", "gp_seed_net_fit", "gp_seed_net_eval",
     "gp_part_file_write", "gp_part_file_info", "gp_part_file_read", "gp_field_file_write",
     "gp_field_file_read", "gp_particle_stats", "gp_set_collision_stats", "gp_get_collision_stats",
+    "gp_partfluid_stats",
 )
 SEED_NET_NPARAM = 5252
 
@@ -131,6 +132,7 @@ def load_library(path: str = LIB_PATH):
     L.gp_reset_timers.argtypes = [vp]
     L.gp_get_collision_counts.argtypes = [vp, C.c_int, ip, C.c_int]
     L.gp_particle_stats.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
+    L.gp_partfluid_stats.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
     L.gp_set_collision_stats.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.gp_get_collision_stats.argtypes = [vp, dp, dp, C.POINTER(C.c_int64)]
     cp = C.c_char_p
@@ -266,6 +268,16 @@ class Context:
         out = np.zeros((self.nspec, 41))
         self._chk(self.lib.gp_particle_stats(self.h, self.nspec, npoly, int(bool(ref_norm)),
                                              _ptr(out)))
+        return out
+
+    def partfluid_stats(self, npoly=None, ref_norm=True):
+        """STAT_PARTFLUID: (nspec, 36), row (J-1) npoly + IDP-1 =
+        MEAN_PARTFLUID(1:36, J, IDP) from the current particles and the fluid
+        at them of the last advance (needs keep_fluid=True)."""
+        npoly = npoly or self.nspec
+        out = np.zeros((self.nspec, 36))
+        self._chk(self.lib.gp_partfluid_stats(self.h, self.nspec, npoly, int(bool(ref_norm)),
+                                              _ptr(out)))
         return out
 
     def collision_grid(self):

@@ -1525,6 +1525,10 @@ struct gp_ctx {
   unsigned long long* cs_cnt = nullptr;  // 0 records 1 statistics steps
   long long* cs_cycle = nullptr;         // cycle number of the next collision pass
   CStat* cs_dev = nullptr;               // device copy of cstat() for k_contact
+  // the species rows as given (host), and whether flout holds the fluid at
+  // the particles of an advance of the current particles
+  std::vector<double> sp_rows;
+  bool fluid_valid = false;
 };
 
 static void drop_graph(gp_ctx* ctx) {
@@ -1739,6 +1743,140 @@ __global__ void k_pstat_final(const Species* __restrict__ spec, double* acc,
   a[22] = n;
 }
 
+// ---------------------------------------------------------------------------
+// STAT_PARTFLUID (stat_partfluid.f90:113-381): per particle, from its state
+// and the fluid velocity and vorticity at it of the last advance (UFAP ..
+// VORTZAP), the particle Reynolds numbers, the Zastawny et al. (2012)
+// coefficients with the fixed fits of that routine (ellipsoid 2, a/b =
+// 1.25, :63-100), drag and lift (rotated to the lab frame), pitching and
+// rotation torques (body frame), relaxation times, incidence angle, the
+// lab-frame major axis and the velocities; summed per species row from the
+// reference's ZERO, divided by the count NPM(IDP) (the last class's with
+// ref_norm, as :362-381 leaves it).  Per-species constants (PMASS, dequiv,
+// TAUPF, GRAVITY) come from the host: pfc[4 * s ..].
+// ---------------------------------------------------------------------------
+constexpr int NPFS = 36;
+
+struct PfFit {   // stat_partfluid.f90:63-100
+  static constexpr double a0 = 1.95, a1 = 18.12, a2 = 1.023, a3 = 4.26, a4 = 0.384, a5 = 21.52,
+                          a6 = 0.990, a7 = 2.86, a8 = 0.260;
+  static constexpr double b1 = 0.083, b2 = -0.21, b3 = 1.582, b4 = 0.851, b5 = 1.842, b6 = -0.802,
+                          b7 = -0.006, b8 = 0.874, b9 = 0.009, b10 = 0.570;
+  static constexpr double c1 = 0.935, c2 = 0.146, c3 = -0.469, c4 = 0.145, c5 = 0.116, c6 = 0.748,
+                          c7 = 0.041, c8 = 0.221, c9 = 0.657, c10 = 0.044;
+  static constexpr double r1 = 0.573, r2 = -0.154, r3 = 116.61, r4 = 1.0;
+  static constexpr double r1_ = 1.244, r2_ = 0.239, r3_ = 378.12, r4_ = 0.789;
+};
+
+// the 36 entries of one particle (index k - 1 of MEAN_PARTFLUID(k, J, IDP))
+__device__ __forceinline__ void partfluid_entries(const Rot<double>& R, const double up[3],
+                                                  const double om[3], const double uf[6],
+                                                  const double* c, double visc, double rhof,
+                                                  double e[NPFS]) {
+  using F = PfFit;
+  const double pm = c[0], dq = c[1];
+  const double ur[3] = {uf[0] - up[0], uf[1] - up[1], uf[2] - up[2]};
+  const double urm = sqrt(ur[0] * ur[0] + ur[1] * ur[1] + ur[2] * ur[2]);
+  const double rep = (urm * dq) / visc;
+  double u[3];
+  R.mulT(ur[0], ur[1], ur[2], u[0], u[1], u[2]);   // by conj_q: to the body frame
+  const double s23 = sqrt(u[1] * u[1] + u[2] * u[2]);
+  const double phi = fabs(atan(s23 / u[0]));
+  const double um = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  const double sp = sin(phi), cp = cos(phi);
+  const double cd0 = F::a1 / pow(rep, F::a2) + F::a3 / pow(rep, F::a4);
+  const double cd90 = F::a5 / pow(rep, F::a6) + F::a7 / pow(rep, F::a8);
+  const double cd = cd0 + (cd90 - cd0) * pow(sp, F::a0);
+  const double cl = (F::b1 / pow(rep, F::b2) + F::b3 / pow(rep, F::b4)) *
+                    pow(sp, F::b5 + F::b6 * pow(rep, F::b7)) *
+                    pow(cp, F::b8 + F::b9 * pow(rep, F::b10));
+  const double k = (1.0 / 2.0) * (rhof / pm) * (1.0 / 4.0) * (M_PI * (dq * dq));
+  double fd[3], fl[3];
+  for (int a = 0; a < 3; ++a) fd[a] = k * cd * um * u[a];
+  fl[0] = k * cl * (um * um) * sp * copysign(1.0, -u[0]);
+  fl[1] = k * cl * (um * um) * cp * (u[1] / s23);
+  fl[2] = k * cl * (um * um) * cp * (u[2] / s23);
+  double fdl[3], fll[3];
+  R.mul(fd[0], fd[1], fd[2], fdl[0], fdl[1], fdl[2]);
+  R.mul(fl[0], fl[1], fl[2], fll[0], fll[1], fll[2]);
+  e[0] = rep; e[1] = cd; e[2] = cl;
+  for (int a = 0; a < 3; ++a) { e[3 + a] = fdl[a]; e[6 + a] = fll[a]; }
+  e[9] = sqrt(fdl[0] * fdl[0] + fdl[1] * fdl[1] + fdl[2] * fdl[2]);
+  e[10] = sqrt(fll[0] * fll[0] + fll[1] * fll[1] + fll[2] * fll[2]);
+  double cu[3];
+  R.mulT(uf[3], uf[4], uf[5], cu[0], cu[1], cu[2]);
+  const double orl[3] = {0.5 * cu[0] - om[0], 0.5 * cu[1] - om[1], 0.5 * cu[2] - om[2]};
+  const double orm = sqrt(orl[0] * orl[0] + orl[1] * orl[1] + orl[2] * orl[2]);
+  const double oyz = sqrt(orl[2] * orl[2] + orl[1] * orl[1]);
+  const double rer = (orm * (dq * dq)) / visc;
+  const double ctp = (F::c1 / pow(rep, F::c2) + F::c3 / pow(rep, F::c4)) *
+                     pow(sp, F::c5 + F::c6 * pow(rep, F::c7)) *
+                     pow(cp, F::c8 + F::c9 * pow(rep, F::c10));
+  const double ctr = F::r1 * pow(rer, F::r2) + F::r3 / pow(rer, F::r4);
+  const double ctr2 = F::r1_ * pow(rer, F::r2_) + F::r3_ / pow(rer, F::r4_);
+  const double kp = (1.0 / 4.0) * (rhof / pm) * (1.0 / 4.0) * M_PI * (dq * dq * dq) * ctp *
+                    (um * um);
+  const double tp[3] = {0.0, kp * (fabs(u[2]) / s23) * copysign(1.0, u[0] * u[2]),
+                        kp * (fabs(u[1]) / s23) * copysign(1.0, u[0] * u[1])};
+  const double h = dq / 2.0, h5 = h * h * h * h * h;
+  const double kr = (rhof / pm) / 2.0 * h5;
+  const double tr[3] = {ctr * kr * fabs(orl[0]) * orl[0], ctr2 * kr * oyz * orl[1],
+                        ctr2 * kr * oyz * orl[2]};
+  e[11] = rer; e[12] = ctp; e[13] = ctr;
+  for (int a = 0; a < 3; ++a) { e[14 + a] = tp[a]; e[17 + a] = tr[a]; }
+  e[20] = sqrt(tp[0] * tp[0] + tp[1] * tp[1] + tp[2] * tp[2]);
+  e[21] = sqrt(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2]);
+  e[22] = c[2];   // TAUPF
+  e[23] = c[3];   // GRAVITY
+  e[24] = phi;
+  e[25] = R.r00; e[26] = R.r10; e[27] = R.r20;   // R (1, 0, 0)
+  e[28] = (0.5 * rhof * cd * (M_PI / 4.0) * (dq * dq) * um) / pm;
+  for (int a = 0; a < 3; ++a) { e[29 + a] = up[a]; e[32 + a] = om[a]; }
+  e[35] = 1.0;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(BLOCK) void k_pfstat(PState<VT> p, const double* __restrict__ fl,
+                                                  const double* __restrict__ pfc, int np,
+                                                  int nspec, double visc, double rhof,
+                                                  double* acc) {
+  __shared__ double s_acc[PST_LDS * NPFS];
+  const bool lds = nspec <= PST_LDS;
+  if (lds) {
+    for (int t = threadIdx.x; t < nspec * NPFS; t += blockDim.x) s_acc[t] = 0.0;
+    __syncthreads();
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+    const int sp = p.sp[i];
+    const long long id = p.id[i];
+    const Rot<double> R(p.qa[i], p.qb[i], p.qc[i], p.qd[i]);
+    const double up[3] = {(double)p.u[i], (double)p.v[i], (double)p.w[i]};
+    const double om[3] = {(double)p.ox[i], (double)p.oy[i], (double)p.oz[i]};
+    double uf[6];
+    for (int c = 0; c < 6; ++c) uf[c] = fl[(long long)c * np + id];
+    double e[NPFS];
+    partfluid_entries(R, up, om, uf, pfc + 4 * (size_t)sp, visc, rhof, e);
+    double* dst = lds ? s_acc + sp * NPFS : acc + (size_t)sp * NPFS;
+    for (int k = 0; k < NPFS; ++k) atomicAdd(dst + k, e[k]);
+  }
+  if (lds) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < nspec * NPFS; t += blockDim.x)
+      if (s_acc[t] != 0.0) atomicAdd(&acc[t], s_acc[t]);
+  }
+}
+
+// division by NPM(IDP) (stat_partfluid.f90:378-381); counts in acc[.. + 35]
+__global__ void k_pfstat_final(const double* __restrict__ acc, double* __restrict__ out,
+                               int nspec, int npoly, int ref_norm) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nspec) return;
+  const int nrow = ref_norm ? nspec - npoly + s % npoly : s;
+  const double n = acc[(size_t)nrow * NPFS + 35];   // ZERO + the count
+  for (int k = 0; k < NPFS; ++k) out[(size_t)s * NPFS + k] = acc[(size_t)s * NPFS + k] / n;
+  out[(size_t)s * NPFS + 35] = n;
+}
+
 static void grid_init(gp_ctx* ctx) {
   // COLLISION_INITIATION (collision_initiation.f90:50-102), INIT_GRID = 2,
   // NPROC = 1: DELTA_COLL = ZERO; the y/z range spans L + DY.
@@ -1948,6 +2086,7 @@ int gp_set_species(gp_ctx* ctx, int nspec, const double* rows) {
   HIPCHK(hipMemcpy(ctx->spec, h.data(), h.size() * sizeof(Species), hipMemcpyHostToDevice));
   ctx->nspec = nspec;
   ctx->max_emaj = emax;
+  ctx->sp_rows.assign(rows, rows + (size_t)nspec * GP_SP_STRIDE);
   grid_init(ctx);
   if (ctx->np > 0) {
     if ((rc = ensure_grid_buffers(ctx))) return rc;
@@ -2041,6 +2180,7 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   ctx->perm_identity = true;
   ctx->cells_valid = false;
   ctx->ngroups = 1;
+  ctx->fluid_valid = false;
   return GP_OK;
 }
 
@@ -2216,6 +2356,7 @@ template <typename VT>
 static void launch_advance_kernel(gp_ctx* ctx, hipStream_t st, const int* perm, const int* skip,
                                   bool bin) {
   const int np = ctx->np;
+  if (ctx->flout) ctx->fluid_valid = true;
   auto ps = pstate<VT>(ctx, ctx->cur), pd = pstate<VT>(ctx, 1 - ctx->cur);
   int* bcnt = bin && ctx->use_bricks ? ctx->bcnt : nullptr;
   int* cell = bin ? ctx->cell : nullptr;
@@ -2911,6 +3052,71 @@ int gp_particle_stats(gp_ctx* ctx, int nspec, int npoly, int ref_norm, double* o
   hipFree(mean);
   if (e != hipSuccess) {
     ctx->err = std::string("particle statistics: ") + hipGetErrorString(e);
+    return GP_ERR_HIP;
+  }
+  return check_err(ctx);
+}
+
+int gp_partfluid_stats(gp_ctx* ctx, int nspec, int npoly, int ref_norm, double* out) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (!out || nspec != ctx->nspec || npoly < 1 || nspec % npoly) return GP_ERR_ARG;
+  if (!ctx->flout || !ctx->fluid_valid) {
+    ctx->err = "particle-fluid statistics need gp_params.keep_fluid = 1 and an advance of the "
+               "current particles";
+    return GP_ERR_STATE;
+  }
+  // per species: PMASS, dequiv (stat_partfluid.f90:124, 135), TAUPF with the
+  // elongation factor (:300-314), GRAVITY
+  std::vector<double> pfc((size_t)nspec * 4);
+  for (int s = 0; s < nspec; ++s) {
+    const double* r = ctx->sp_rows.data() + (size_t)s * GP_SP_STRIDE;
+    const double rhop = r[GP_SP_RHOP], emaj = r[GP_SP_EMAJ], apr = r[GP_SP_APR];
+    double* c = pfc.data() + (size_t)s * 4;
+    c[0] = rhop * (4.0 / 3.0) * M_PI * std::pow(emaj, 3.0) / std::pow(apr, 2.0);
+    c[1] = 2.0 * emaj / std::pow(apr, 2.0 / 3.0);
+    const double minax = emaj / apr;
+    const double fbeta = apr == 1.0 ? 1.0
+                                    : (apr * std::log(apr + std::sqrt(apr * apr - 1.0))) /
+                                          std::sqrt(apr * apr - 1.0);
+    c[2] = (2.0 / 9.0) * (rhop * (minax * minax)) / (ctx->prm.rhof * ctx->prm.visc) * fbeta;
+    c[3] = r[GP_SP_GRAV];
+  }
+  double *acc = nullptr, *dout = nullptr, *dpfc = nullptr;
+  const size_t na = (size_t)nspec * NPFS;
+  if (hipMalloc((void**)&acc, na * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&dout, na * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&dpfc, pfc.size() * sizeof(double)) != hipSuccess) {
+    if (acc) hipFree(acc);
+    if (dout) hipFree(dout);
+    ctx->err = "particle-fluid statistics: out of device memory";
+    return GP_ERR_NOMEM;
+  }
+  hipError_t e = hipMemcpyAsync(dpfc, pfc.data(), pfc.size() * sizeof(double),
+                                hipMemcpyHostToDevice, ctx->st);
+  // MEAN_PARTFLUID_LOC(:,:,:) = ZERO (:114)
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk((long long)na)), dim3(BLOCK), 0, ctx->st, acc,
+                     ZERO_R, (long long)na);
+  const unsigned grid = std::max(1u, std::min(nblk(ctx->np), 2048u));
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_pfstat<float>, dim3(grid), dim3(BLOCK), 0, ctx->st,
+                       pstate<float>(ctx, ctx->cur), (const double*)ctx->flout,
+                       (const double*)dpfc, ctx->np, nspec, ctx->prm.visc, ctx->prm.rhof, acc);
+  else
+    hipLaunchKernelGGL(k_pfstat<double>, dim3(grid), dim3(BLOCK), 0, ctx->st,
+                       pstate<double>(ctx, ctx->cur), (const double*)ctx->flout,
+                       (const double*)dpfc, ctx->np, nspec, ctx->prm.visc, ctx->prm.rhof, acc);
+  hipLaunchKernelGGL(k_pfstat_final, dim3(nblk(nspec)), dim3(BLOCK), 0, ctx->st,
+                     (const double*)acc, dout, nspec, npoly, ref_norm ? 1 : 0);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, dout, na * sizeof(double), hipMemcpyDeviceToHost, ctx->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
+  hipFree(acc);
+  hipFree(dout);
+  hipFree(dpfc);
+  if (e != hipSuccess) {
+    ctx->err = std::string("particle-fluid statistics: ") + hipGetErrorString(e);
     return GP_ERR_HIP;
   }
   return check_err(ctx);

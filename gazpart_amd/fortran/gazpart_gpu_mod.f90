@@ -230,6 +230,14 @@ module gazpart_gpu
       integer(c_int), value :: nspec, npoly, ref_norm
       real(c_double), intent(out) :: out(41, nspec)
     end function
+    ! out(36, nspec) = MEAN_PARTFLUID(1:36, J, IDP), column (J-1)*npoly + IDP
+    integer(c_int) function gp_partfluid_stats(ctx, nspec, npoly, ref_norm, out) &
+        bind(C, name='gp_partfluid_stats')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nspec, npoly, ref_norm
+      real(c_double), intent(out) :: out(36, nspec)
+    end function
     integer(c_int) function gp_set_collision_stats(ctx, nclass, npoly, fout2, ncycle) &
         bind(C, name='gp_set_collision_stats')
       import :: c_int, c_ptr

@@ -191,6 +191,19 @@ int gp_get_stats(gp_ctx *ctx, gp_stats *out);
  * values); ref_norm = 0 divides each row by its own count.  The sums run in
  * another order than the reference's particle loop.                      */
 int gp_particle_stats(gp_ctx *ctx, int nspec, int npoly, int ref_norm, double *out);
+/* STAT_PARTFLUID (stat_partfluid.f90:113-381) over the resident particles
+ * and the fluid at them of the last advance (needs gp_params.keep_fluid =
+ * 1 and an advance since gp_set_particles): out[s][k-1] =
+ * MEAN_PARTFLUID(k, J, IDP), k = 1..36 (Rep, CD, CL, drag and lift in the
+ * lab frame and their norms, Re_R, pitching / rotation torque
+ * coefficients, the torques in the body frame and their norms, TAUPF,
+ * GRAVITY, incidence angle, major axis, 1/tau_p, particle velocity and
+ * body angular velocity, count), with the Zastawny fits that routine fixes;
+ * species rows, npoly and ref_norm as for gp_particle_stats (the routine
+ * divides by the same last-class counts, :362-381).  PMASS, dequiv and
+ * TAUPF come from the rows' RHOP, EMAJ, APR columns as the routine forms
+ * them.                                                                  */
+int gp_partfluid_stats(gp_ctx *ctx, int nspec, int npoly, int ref_norm, double *out);
 /* zero the accumulated stage times                                       */
 int gp_reset_timers(gp_ctx *ctx);
 /* introspection for benchmarks: collision grid of the context           */

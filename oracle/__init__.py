@@ -398,6 +398,14 @@ class Ref:
         f(ig, nmax, home, near, C.byref(nc))
         return home[:nc.value], near[:nc.value]
 
+    def partfluid_stats(self, ncycle=1):
+        """MEAN_PARTFLUID(1:36, J, IDP) of STAT_PARTFLUID as [J-1, IDP-1, k-1]"""
+        f = self.L.ref_partfluid_stats
+        f.argtypes = [C.c_int, _dp]
+        out = np.zeros(36 * self.nsize * self.nclass)
+        f(ncycle, out)
+        return out.reshape(self.nclass, self.nsize, 36).copy()
+
     def cstat_on(self, fout2):
         self.L.ref_cstat_on.argtypes = [C.c_int]
         self.L.ref_cstat_on(fout2)

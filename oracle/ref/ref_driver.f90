@@ -384,6 +384,35 @@ contains
     MEAN_TIME_PART_COL = 0.0d0
   end subroutine ref_cstat_on
 
+  !! STAT_PARTFLUID (stat_partfluid.f90) on the current PART, whose fluid
+  !! at the particles (UFAP .. VORTZAP) the last ref_adv left: out(1:36,
+  !! IDP, J) = MEAN_PARTFLUID(1:36, J, IDP), read back through
+  !! MEAN_TIME_PARTFLUID (with STAT_TIME the routine copies it there, :460).
+  !! The routine also writes a line per (J, IDP) to unit 540 + ... .
+  subroutine ref_partfluid_stats(ncycle, out) bind(C, name='ref_partfluid_stats')
+    use mpi
+    integer(c_int), value :: ncycle
+    real(c_double), intent(out) :: out(36, POLYDISPMAX, NIG)
+    integer :: j, idp, ierr
+    logical :: st, up
+    !! the routine times itself and sums over ranks through MPI: one rank
+    call MPI_INITIALIZED(up, ierr)
+    if (.not. up) call MPI_INIT(ierr)
+    st = STAT_TIME
+    NSTAT = max(NSTAT, 36)
+    STAT_TIME = .true.
+    if (allocated(MEAN_TIME_PARTFLUID)) deallocate(MEAN_TIME_PARTFLUID)
+    allocate(MEAN_TIME_PARTFLUID(NSTAT, NIG, POLYDISPMAX))
+    MEAN_TIME_PARTFLUID = 0.0d0
+    call STAT_PARTFLUID(ncycle, 0.0d0)
+    do j = 1, NIG
+      do idp = 1, POLYDISPMAX
+        out(:, idp, j) = MEAN_TIME_PARTFLUID(1:36, j, idp)
+      end do
+    end do
+    STAT_TIME = st
+  end subroutine ref_partfluid_stats
+
   !! MEAN_TIME_PART_COL(1:8, IDP1, IDP2)
   subroutine ref_cstat_get(out) bind(C, name='ref_cstat_get')
     real(c_double), intent(out) :: out(8, POLYDISPMAX, POLYDISPMAX)

@@ -12,6 +12,8 @@ C-callable driver).  Each fixture holds inputs and the reference's outputs:
                          small ensemble (collisions and wall bounces happen)
   golden_collision_stats.npz  level-1 collision statistics (MEAN_PART_COL)
                          of such steps with three particle sizes
+  golden_partfluid_stats.npz  STAT_PARTFLUID after one advance (two classes
+                         of two sizes)
 
 Usage:  python tests/golden/make_golden.py
 """
@@ -245,12 +247,59 @@ def collision_stats(nsteps=10, fout2=2):
                 final=final)
 
 
+def partfluid_stats():
+    """STAT_PARTFLUID after one ADV_PARTICLE_ELLP (which leaves the fluid at
+    the particles in PART%UFAP .. VORTZAP): two classes (APR 2 and 5) of two
+    sizes each, 300 particles per class with random velocities and angular
+    velocities.  The routine writes to fort.NNN: it runs in a scratch
+    directory."""
+    import tempfile
+    n = 12
+    fld, umax = small_field(n, seed=9)
+    dx = L / (n - 1)
+    dtime = 0.1 * dx / umax
+    apr = np.array([2.0, 5.0])
+    emaj = np.array([[0.2, 0.3], [0.25, 0.35]]) * dx
+    rhop = np.array([[50.0, 80.0], [30.0, 60.0]])
+    R = O.Ref()
+    R.setup(n, L, VISC, RHOF, dtime, 1.0, 0.0, 3, emaj, apr, rhop)
+    R.set_field(fld)
+    npc = 300
+    g = W.rng(61)
+    st0, idp0 = [], []
+    for j in range(2):
+        st = W.random_particles(npc, L, float(emaj.max()), 62 + j)
+        st[3:6] = g.standard_normal((3, npc)) * umax
+        st[10:13] = g.standard_normal((3, npc)) * 2.0
+        idp = (g.integers(0, 2, npc) + 1).astype(np.int32)
+        R.set_particles(j + 1, st, idp)
+        st0.append(st)
+        idp0.append(idp)
+    R.adv(1)
+    st1, fl1 = [], []
+    for j in range(2):
+        s1, f1, _ = R.get_particles(j + 1, npc)
+        st1.append(s1)
+        fl1.append(f1)
+    cwd = os.getcwd()
+    with tempfile.TemporaryDirectory() as tmp:
+        os.chdir(tmp)
+        try:
+            out = R.partfluid_stats(1)
+        finally:
+            os.chdir(cwd)
+    return dict(n=n, L=L, visc=VISC, rhof=RHOF, dtime=dtime, field=fld, apr=apr, emaj=emaj,
+                rhop=rhop, state0=np.array(st0), idp=np.array(idp0), state1=np.array(st1),
+                fluid1=np.array(fl1), mean=out)
+
+
 def main():
     np.savez_compressed(os.path.join(HERE, "golden_leaf.npz"), **leaf())
     np.savez_compressed(os.path.join(HERE, "golden_cells.npz"), **cells())
     np.savez_compressed(os.path.join(HERE, "golden_trajectory.npz"), **trajectory(30, 3))
     np.savez_compressed(os.path.join(HERE, "golden_trajectory_lag4.npz"), **trajectory(10, 4))
     np.savez_compressed(os.path.join(HERE, "golden_collision_stats.npz"), **collision_stats())
+    np.savez_compressed(os.path.join(HERE, "golden_partfluid_stats.npz"), **partfluid_stats())
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
