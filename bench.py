@@ -235,6 +235,64 @@ def make_workload(cfg, args, W):
     return W.c5(n=args.n or 256, np_=args.np or 2_000_000, scheme=scheme), scheme
 
 
+def stats_bench(wl, G, W, steps):
+    """Host-timed calls (device work + result copy) of the statistics
+    routines on the workload's state after 3 steps: with the workload's
+    own species rows and with the particles regrouped into 2 classes x 3
+    sizes; and the graph-mode step with collision statistics off and on
+    every step (the 6-row grouping, so that bins stay few)."""
+    import numpy as np
+
+    def per_call(f, n=5):
+        f()
+        t = time.perf_counter()
+        for _ in range(n):
+            f()
+        return round((time.perf_counter() - t) / n * 1e3, 4)
+
+    out = {"metric": "statistics_ms_per_call", "config": {"workload": wl.name, "np": wl.np_}}
+    ctx = G.Context.from_workload(wl, precision=G.PREC_MIXED, keep_fluid=True)
+    ctx.step(3)
+    ctx.sync()
+    out["rows_own"] = ctx.nspec
+    out["particle_stats_own_rows"] = per_call(lambda: ctx.particle_stats())
+    out["partfluid_stats_own_rows"] = per_call(lambda: ctx.partfluid_stats())
+    st = ctx.get_particles()
+    ctx.close()
+    rows6 = W.species_spheroid(np.full(6, float(wl.species[:, W.SP_EMAJ].mean())),
+                               np.repeat([2.0, 5.0], 3), np.full(6, 1000.0))
+    sid6 = (np.arange(wl.np_) % 6).astype(np.int32)
+
+    def make6(**kw):
+        c = G.Context(wl.n, wl.L, wl.visc, wl.rhof, wl.dtime, scheme=wl.scheme, dpmax=wl.dpmax,
+                      precision=G.PREC_MIXED, **kw)
+        c.bind_field(wl.field)
+        c.set_species(rows6)
+        c.set_particles(st, sid6)
+        return c
+
+    c6 = make6(keep_fluid=True)
+    c6.adv_particle_ellp(1)
+    out["particle_stats_2x3"] = per_call(lambda: c6.particle_stats(npoly=3))
+    out["partfluid_stats_2x3"] = per_call(lambda: c6.partfluid_stats(npoly=3))
+    c6.close()
+    for name, fout2 in (("step_ms_collision_stats_off", 0), ("step_ms_collision_stats_every_step", 1)):
+        cx = make6(graph=True, overlap=1)
+        if fout2:
+            cx.set_collision_stats(nclass=2, npoly=3, fout2=1)
+        cx.step(9)
+        cx.sync()
+        t = time.perf_counter()
+        cx.step(steps)
+        cx.sync()
+        out[name] = round((time.perf_counter() - t) / steps * 1e3, 4)
+        if fout2:
+            out["contacts_reached_last_step"] = int(round(
+                cx.collision_stats()["last"][..., 0].sum()))
+        cx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +313,9 @@ def main():
     ap.add_argument("--pairs", type=int, default=10_000_000,
                     help="C5: solver-labelled pairs the seed network is trained on")
     ap.add_argument("--epochs", type=int, default=3, help="C5: training epochs")
+    ap.add_argument("--stats", action="store_true",
+                    help="time the statistics routines on the workload (particle, "
+                         "particle-fluid and collision statistics) instead of the step")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (torch), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profile")
@@ -284,6 +345,9 @@ def main():
     log(f"{cfg} workload built in {time.perf_counter() - t0:.1f}s: n={wl.n} np={wl.np_} "
         f"dt={wl.dtime:.3e} dpmax={wl.dpmax:.3e}")
     groups = wl.meta.get("group") if cfg == "C4" else None
+    if args.stats:
+        print(json.dumps(stats_bench(wl, G, W, args.steps)))
+        return
 
     kw = {"brick": args.brick, "lm_waves": args.lm_waves}
     extra = {}
