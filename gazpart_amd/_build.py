@@ -7,6 +7,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc", "gp_lib.hip")
+SRCS = [SRC, os.path.join(HERE, "csrc", "gp_dns.hip")]
+LIBS = ["-L/opt/rocm/lib", "-lrocfft", "-Wl,-rpath,/opt/rocm/lib"]
 OUT = os.path.join(HERE, "libgazpart_gpu.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -18,7 +20,7 @@ FLAGS = [
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [HIPCC, *FLAGS, SRC, "-o", OUT + ".tmp"]
+    cmd = [HIPCC, *FLAGS, *SRCS, *LIBS, "-o", OUT + ".tmp"]
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -2029,20 +2029,14 @@ int gp_sync(gp_ctx* ctx) {
   return check_err(ctx);
 }
 
-int gp_bind_field(gp_ctx* ctx, const double* field) {
-  if (!ctx || !field) return GP_ERR_ARG;
-  hipStreamSynchronize(ctx->st);
-  drop_graph(ctx);
+// the reference-layout field dref (device) -> the context's node layout;
+// the node buffer is allocated once (its size depends only on n and the
+// precision), so captured step graphs stay valid across re-binds
+static int bind_field_dev(gp_ctx* ctx, const double* dref) {
   const int n = ctx->prm.n, n4 = n + 4;
-  const size_t nref = (size_t)6 * n * n4 * n4;
   const size_t nodes = (size_t)n4 * n4 * n4;
-  double* dref = nullptr;
-  HIPCHK(hipMalloc((void**)&dref, nref * sizeof(double)));
-  HIPCHK(hipMemcpyAsync(dref, field, nref * sizeof(double), hipMemcpyHostToDevice, ctx->st));
-  if (ctx->field) hipFree(ctx->field);
-  ctx->field = nullptr;
   const size_t esz = ctx->mixed ? sizeof(float) : sizeof(double);
-  HIPCHK(hipMalloc(&ctx->field, nodes * 6 * esz + 256));
+  if (!ctx->field) HIPCHK(hipMalloc(&ctx->field, nodes * 6 * esz + 256));
   if (ctx->mixed)
     hipLaunchKernelGGL(k_field_prep<float>, dim3(nblk((long long)nodes)), dim3(BLOCK), 0, ctx->st,
                        dref, (float*)ctx->field, n);
@@ -2051,9 +2045,29 @@ int gp_bind_field(gp_ctx* ctx, const double* field) {
                        ctx->st, dref, (double*)ctx->field, n);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->st));
-  hipFree(dref);
   ctx->have_field = true;
   return GP_OK;
+}
+
+int gp_bind_field(gp_ctx* ctx, const double* field) {
+  if (!ctx || !field) return GP_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st2));
+  const int n = ctx->prm.n, n4 = n + 4;
+  const size_t nref = (size_t)6 * n * n4 * n4;
+  double* dref = nullptr;
+  HIPCHK(hipMalloc((void**)&dref, nref * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(dref, field, nref * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+  const int rc = bind_field_dev(ctx, dref);
+  hipFree(dref);
+  return rc;
+}
+
+int gp_bind_field_device(gp_ctx* ctx, const double* field) {
+  if (!ctx || !field) return GP_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st2));
+  return bind_field_dev(ctx, field);
 }
 
 int gp_set_species(gp_ctx* ctx, int nspec, const double* rows) {

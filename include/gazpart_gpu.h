@@ -133,6 +133,11 @@ int gp_sync(gp_ctx *ctx);
 /* data in / out ------------------------------------------------------- */
 /* UFLU..VORTICITY_Z after FLUIDCOMM (main.f90:255-259); copied once.   */
 int gp_bind_field(gp_ctx *ctx, const double *field);
+/* the same from device memory (same layout, on the context's device):
+ * for a field supplier that keeps the fluid on the GPU (gp_dns_bind).
+ * Re-binding keeps the context's buffers, so captured step graphs stay
+ * valid. (No reference counterpart: its field lives in host arrays.)    */
+int gp_bind_field_device(gp_ctx *ctx, const double *field);
 /* per-species constants (PARAM_PHYS EMAJ_PART, APR_PART, RHOP, IPXX..)  */
 int gp_set_species(gp_ctx *ctx, int nspec, const double *rows);
 /* PART(1:np) -> device; species[i] in [0, nspec)                       */
@@ -319,6 +324,56 @@ int gp_part_file_read(const char *path, int64_t npart, int npoly, int *npclass, 
  * NX NX NX, then NX^3 doubles in Fortran order (x fastest, no ghosts).   */
 int gp_field_file_write(const char *path, int n, const double *f);
 int gp_field_file_read(const char *path, int n, double *f);
+
+/* spectral DNS field supplier ---------------------------------------------
+ * The step before the path when the fluid evolves (SOLVE_FLUID = 1): the
+ * reference's pseudo-spectral Navier-Stokes step ADV_FLUID
+ * (adv_fluid.f90:78-234: forward transform of UFLU, two-stage
+ * integrating-factor Runge-Kutta on the rotational-form nonlinear term
+ * of NONLINEAR_TERMS (nonlinear_terms.f90:96-206), PROJ_DIVFREE
+ * (proj_divfree.f90:46-60), the smooth dealiasing FILTER of MESHING
+ * (meshing.f90:151-180), backward transform), with the STEADY forcing of
+ * ADV_FORCING / ADD_FORCING3 (adv_forcing.f90:33-125, add_forcing3.f90:
+ * 40-58; Ornstein-Uhlenbeck amplitudes on KFORCE_MIN <= |k| <= KFORCE_MAX
+ * from the reference's RAN1 / GAUSS sequence, initial amplitudes 0 as
+ * INITIATION_FORCING_NEW sets them for INIT_FLUID_VELOCITY <= 2) and
+ * CHECK_CFL (check_cfl.f90:61-110).  Velocity and spectral state stay in
+ * device memory (rocFFT, double precision); gp_dns_bind hands velocity and
+ * a spectrally computed vorticity to a particle context without a host
+ * copy.  Grids: n^3 points, the transform grid x = (i-1) L / n; arrays
+ * double[3][n][n][n], x fastest.                                         */
+typedef struct gp_dns gp_dns;
+typedef struct {
+  int n;                 /* NX = NY = NZ (even) */
+  double L;              /* LXMAX = LYMAX = LZMAX */
+  double visc, dtime;    /* VISC, DTIME */
+  int forcing;           /* 1: STEADY forcing */
+  int seed;              /* IDFORCE (RAN1 seed; <= 0 re-initialises the shuffle) */
+  double kforce_min, kforce_max, sigma_force, time_force;
+  int reserved[4];
+} gp_dns_params;
+int gp_dns_create(const gp_dns_params *p, gp_dns **out);
+int gp_dns_destroy(gp_dns *d);
+const char *gp_dns_error_string(const gp_dns *d);
+/* UFLU, VFLU, WFLU (double[3][n][n][n]) -> device                         */
+int gp_dns_set_velocity(gp_dns *d, const double *uvw);
+int gp_dns_get_velocity(gp_dns *d, double *uvw);
+/* curl of the spectral velocity, i k x U, back on the grid               */
+int gp_dns_get_vorticity(gp_dns *d, double *w);
+/* nsteps x ADV_FLUID at the current DTIME                                 */
+int gp_dns_step(gp_dns *d, int nsteps);
+/* CHECK_CFL with NPROC = 1: DTIME = min(Courant 0.05, Fourier 1.0,
+ * dtime_user) with DX = L/(n-1) as the reference's mesh; sets and returns it */
+int gp_dns_cfl(gp_dns *d, double dtime_user, double *dtime);
+/* EPS_FLU of the last step's first stage (nonlinear_terms.f90:124-128),
+ * the velocity's largest magnitudes per component, and the forcing mode
+ * count                                                                  */
+int gp_dns_info(gp_dns *d, double *eps, double *umax3, int *nforce);
+/* velocity and vorticity -> the particle context's field (its n must be
+ * the DNS's), device to device.  mesh_vorticity = 1 scales the vorticity
+ * by (n-1)/n: the curl of the velocity as the particles see it on the
+ * reference's mesh XMESH = (i-1) L/(n-1); 0 binds the transform grid's.  */
+int gp_dns_bind(gp_dns *d, gp_ctx *ctx, int mesh_vorticity);
 
 #ifdef __cplusplus
 }

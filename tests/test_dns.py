@@ -1,0 +1,265 @@
+"""The spectral DNS field supplier (row f1): the reference's ADV_FLUID
+pseudo-spectral step (adv_fluid.f90:78-234, NONLINEAR_TERMS, PROJ_DIVFREE,
+the MESHING filter, STEADY forcing, CHECK_CFL).
+
+The reference's fluid step needs P3DFFT, which this image lacks, so it is
+not compiled here; its CPU restatement (oracle/dns_ref.py) is pinned by
+exact Navier-Stokes solutions whose nonlinear term the projection removes
+(2D Taylor-Green vortex, ABC flow, a shear wave: exact exp(-nu k^2 t)
+decay), the projection's k . U = 0, the filter's bands, second-order
+convergence in time on a nonlinear flow, and the Park-Miller minimal
+standard's published check value for the forcing's generator.
+
+Tolerances
+  oracle, exact solutions : 1e-13 of the amplitude.
+  GPU vs oracle (FP64)    : 1e-11 of each field's largest magnitude after
+                            several steps (rocFFT and numpy round
+                            differently); forcing amplitudes and EPS_FLU
+                            1e-11 relative; CFL time step 1e-14 relative.
+  bound field             : exact at the nodes (interpolation at a node
+                            returns the node value) up to 1e-14.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import dns_ref as D
+
+L2PI = 2.0 * math.pi
+
+
+def grid(n):
+    x = np.arange(n) * L2PI / n
+    return np.meshgrid(x, x, x, indexing="ij")   # Z, Y, X ([k][j][i])
+
+
+def taylor_green(n):
+    Z, Y, X = grid(n)
+    return np.stack([np.sin(X) * np.cos(Y), -np.cos(X) * np.sin(Y), np.zeros_like(X)])
+
+
+def abc(n, A=1.0, B=0.7, C=0.4):
+    Z, Y, X = grid(n)
+    return np.stack([A * np.sin(Z) + C * np.cos(Y), B * np.sin(X) + A * np.cos(Z),
+                     C * np.sin(Y) + B * np.cos(X)])
+
+
+def random_velocity(n, k0, seed):
+    from gazpart_amd import workload as W
+    fld, _ = W.random_fourier_field(n, k0, seed)
+    return np.stack([fld[c, 2:n + 2, 2:n + 2, :] for c in range(3)])
+
+
+# -------------------------------------------------------------------- oracle
+@pytest.mark.parametrize("flow,k2", [(taylor_green, 2.0), (abc, 1.0)])
+def test_oracle_exact_decay(flow, k2):
+    n, nu, dt, ns = 16, 0.05, 0.01, 20
+    u0 = flow(n)
+    s = D.DnsRef(n, L2PI, nu, dt)
+    s.set_velocity(u0)
+    for _ in range(ns):
+        s.step()
+    assert np.abs(s.u - u0 * math.exp(-nu * k2 * ns * dt)).max() <= 1e-13
+
+
+def test_oracle_shear_wave_and_projection():
+    n, nu, dt = 16, 0.1, 0.02
+    Z, Y, X = grid(n)
+    u0 = np.stack([np.sin(3 * Y), np.zeros_like(X), np.zeros_like(X)])
+    s = D.DnsRef(n, L2PI, nu, dt)
+    s.set_velocity(u0)
+    s.step()
+    assert np.abs(s.u - u0 * math.exp(-nu * 9 * dt)).max() <= 1e-13
+    # a compressible field is projected onto k . U = 0 (PROJ_DIVFREE)
+    s.set_velocity(random_velocity(n, 2.0, 5) + 0.3 * np.stack([np.sin(X), np.zeros_like(X),
+                                                                 np.zeros_like(X)]))
+    s.step()
+    kx, ky, kz = s.k
+    div = np.abs(kx * s.U[0] + ky * s.U[1] + kz * s.U[2]).max()
+    assert div <= 1e-15 * np.abs(s.U).max()
+
+
+def test_oracle_filter_bands():
+    n = 32
+    f = D.dealias_filter(n, L2PI)
+    kx, ky, kz = D.wavenumbers(n, L2PI)
+    klast = math.pi * n / L2PI
+    r = np.sqrt(kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2) / klast
+    assert (f[r <= D.RTRUNC] == 1.0).all()
+    assert (f[r > 1.0] == 0.0).all()
+    band = (r > D.RTRUNC) & (r <= 1.0)
+    assert band.any() and (f[band] < 1.0).all() and (f[band] >= 0.0).all()
+    # monotone in |k| across the taper
+    o = np.argsort(r[band])
+    assert (np.diff(f[band][o]) <= 1e-15).all()
+
+
+def test_oracle_second_order_in_time():
+    n = 32
+    u0 = random_velocity(n, 1.5, 3)
+
+    def run(dt, k):
+        s = D.DnsRef(n, L2PI, 0.02, dt)
+        s.set_velocity(u0)
+        for _ in range(k):
+            s.step()
+        return s.u
+
+    ref = run(0.00125, 128)
+    e = [np.abs(run(0.02 / 2 ** m, 8 * 2 ** m) - ref).max() for m in range(3)]
+    assert 3.5 < e[0] / e[1] < 4.6 and 3.5 < e[1] / e[2] < 4.6, e
+
+
+def test_oracle_random_generator():
+    """RAN1's generator is the Park-Miller minimal standard: from seed 1 the
+    10000th state is 1043618065 (Park and Miller, CACM 31, 1988)."""
+    r = D.Ran1()
+    x = 1
+    for _ in range(10000):
+        x = r._lcg(x)
+    assert x == 1043618065
+    g = D.Ran1()
+    vals = []
+    idum = -3
+    for _ in range(20000):
+        v, idum = g.gauss(idum)
+        vals.append(v)
+    v = np.array(vals)
+    assert abs(v.mean()) < 0.03 and abs(v.std() - 1.0) < 0.03
+
+
+def test_oracle_forcing_modes_and_statistics():
+    """Modes in the reference's (K, J, I) loop order with KFORCE_MIN <= |k|
+    <= KFORCE_MAX; the Ornstein-Uhlenbeck amplitudes approach variance
+    sigma^2 xampl^2 per part."""
+    n = 16
+    f = D.OuForcing(n, L2PI, 0.5, 2.5, 0.3, 0.2, -11)
+    ks = [m[3] for m in f.modes]
+    assert all(0.5 <= k <= 2.5 for k in ks) and len(ks) == len(set((m[:3]) for m in f.modes))
+    for _ in range(3000):
+        f.advance(0.05)
+    xampl = np.array([math.tanh((2.5 - k) / 0.2 / 2.5) for k in ks])
+    var = (np.abs(f.f) ** 2).mean(axis=0) / 2.0
+    sel = xampl > 0.5
+    ratio = var[sel] / (0.3 ** 2 * xampl[sel] ** 2)
+    assert 0.5 < ratio.mean() < 1.5
+
+
+# ----------------------------------------------------------------------- GPU
+@pytest.fixture(scope="module")
+def G():
+    import gazpart_amd
+    return gazpart_amd
+
+
+def _dns(n, nu, dt, forcing=None):
+    from gazpart_amd.dns import Dns
+    return Dns(n, L2PI, nu, dt, forcing=forcing)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flow,k2", [(taylor_green, 2.0), (abc, 1.0)])
+def test_gpu_exact_decay(flow, k2):
+    n, nu, dt, ns = 32, 0.05, 0.01, 20
+    u0 = flow(n)
+    d = _dns(n, nu, dt)
+    d.set_velocity(u0)
+    d.step(ns)
+    assert np.abs(d.velocity() - u0 * math.exp(-nu * k2 * ns * dt)).max() <= 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,k0", [(32, 3.0), (64, 6.0)])
+def test_gpu_matches_oracle(n, k0):
+    """Several steps of a nonlinear field with energy in the filter's taper:
+    velocity, vorticity and EPS_FLU as the oracle's."""
+    u0 = random_velocity(n, k0, 17)
+    nu, dt, ns = 0.01, 0.004, 6
+    d = _dns(n, nu, dt)
+    d.set_velocity(u0)
+    s = D.DnsRef(n, L2PI, nu, dt)
+    s.set_velocity(u0)
+    for _ in range(ns):
+        d.step(1)
+        s.step()
+    u = d.velocity()
+    assert np.abs(u - s.u).max() <= 1e-11 * np.abs(s.u).max()
+    w = d.vorticity()
+    wr = D.btran(s.vorticity(), n)
+    assert np.abs(w - wr).max() <= 1e-11 * np.abs(wr).max()
+    assert abs(d.info()["eps"] - s.eps) <= 1e-11 * s.eps
+    assert not np.array_equal(u, u0)
+
+
+@pytest.mark.gpu
+def test_gpu_forcing_matches_oracle():
+    n, nu, dt = 32, 0.02, 0.005
+    fp = dict(kmin=0.5, kmax=2.5, sigma=0.5, tforce=0.1, seed=-11)
+    u0 = random_velocity(n, 2.0, 23)
+    d = _dns(n, nu, dt, forcing=fp)
+    d.set_velocity(u0)
+    f = D.OuForcing(n, L2PI, fp["kmin"], fp["kmax"], fp["sigma"], fp["tforce"], fp["seed"])
+    s = D.DnsRef(n, L2PI, nu, dt, forcing=f)
+    s.set_velocity(u0)
+    assert d.info()["nforce"] == len(f.modes)
+    for _ in range(8):
+        d.step(1)
+        s.step()
+    u = d.velocity()
+    assert np.abs(u - s.u).max() <= 1e-11 * np.abs(s.u).max()
+    # without the forcing the field differs visibly
+    s0 = D.DnsRef(n, L2PI, nu, dt)
+    s0.set_velocity(u0)
+    for _ in range(8):
+        s0.step()
+    assert np.abs(s0.u - s.u).max() > 1e-4 * np.abs(s.u).max()
+
+
+@pytest.mark.gpu
+def test_gpu_cfl_matches_oracle():
+    n = 32
+    u0 = random_velocity(n, 3.0, 29) * 3.0
+    d = _dns(n, 0.01, 1e-3)
+    d.set_velocity(u0)
+    s = D.DnsRef(n, L2PI, 0.01, 1e-3)
+    s.set_velocity(u0)
+    for user in (1.0, 1e-4):
+        assert abs(d.cfl(user) - s.cfl(user)) <= 1e-14 * s.cfl(user)
+    np.testing.assert_allclose(d.info()["umax"], np.abs(u0).max(axis=(1, 2, 3)), rtol=1e-15)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [0, 1])
+def test_gpu_bind_feeds_the_particle_step(G, prec):
+    """velocity and spectral vorticity land in the particle context's field
+    without a host copy: interpolation at the nodes returns them (vorticity
+    scaled by (n-1)/n to the reference mesh); particles then step."""
+    from gazpart_amd import workload as W
+    n = 32
+    u0 = random_velocity(n, 3.0, 31)
+    d = _dns(n, 0.01, 2e-3)
+    d.set_velocity(u0)
+    d.step(2)
+    ctx = G.Context(n, L2PI, 0.01, 1.0, 1e-3, precision=prec)
+    d.bind(ctx)
+    u, w = d.velocity(), d.vorticity() * ((n - 1) / n)
+    g = np.random.default_rng(3)
+    idx = g.integers(2, n - 3, size=(3, 200))
+    dx = L2PI / (n - 1)
+    x, y, z = idx[0] * dx, idx[1] * dx, idx[2] * dx
+    tol = 1e-14 if prec == 0 else 1e-6
+    for c in range(6):
+        ref = (u if c < 3 else w)[c % 3][idx[2], idx[1], idx[0]]
+        got = ctx.interph(3, c, x, y, z)
+        assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), c
+    sp = W.species_spheroid(0.02, 2.0, 1000.0)[None, :]
+    st = W.random_particles(500, L2PI, 0.05, 7)
+    ctx.set_species(sp)
+    ctx.set_particles(st, np.zeros(500, dtype=np.int32))
+    ctx.step(3)
+    assert np.isfinite(ctx.get_particles()).all()
+    # a second step of the fluid re-binds into the same buffers
+    d.step(1)
+    d.bind(ctx)
+    ctx.step(1)
