@@ -293,6 +293,68 @@ def stats_bench(wl, G, W, steps):
     return out
 
 
+def dns_bench(wl, G, W, steps, warmup, cpu=True):
+    """The spectral DNS supplier on the workload's grid (n^3, L, VISC),
+    started from the workload's own velocity: ADV_FLUID steps alone, then
+    the coupled step with the fluid evolving (fluid step, device-to-device
+    bind of velocity and vorticity, one particle step in graph mode).
+    Host-timed around synchronised calls; per-kernel times come from a
+    kernel trace of the same command."""
+    import numpy as np
+    from gazpart_amd.dns import Dns
+
+    n = wl.n
+    u0 = np.stack([wl.field[c, 2:n + 2, 2:n + 2, :] for c in range(3)])
+    d = Dns(n, wl.L, wl.visc, wl.dtime)
+    d.set_velocity(u0)
+    dt = d.cfl(wl.dtime)
+    d.step(warmup)
+    t = time.perf_counter()
+    d.step(steps)
+    fluid_ms = (time.perf_counter() - t) / steps * 1e3
+    info = d.info()
+    # algorithmic HBM bytes of one ADV_FLUID: 7 batched transforms (read +
+    # write of three fields each), the velocity copy, and the element-wise
+    # kernels' reads and writes (spectral fields 16 B x n^2 (n/2+1) each,
+    # physical 8 B x n^3)
+    rb = 3 * 8.0 * n ** 3
+    cb = 3 * 16.0 * n * n * (n // 2 + 1)
+    bytes_step = (7 * (rb + cb) + 2 * rb + 3 * cb + 2 * (3 * rb) + 6 * cb + 4 * cb)
+    ctx = G.Context.from_workload(wl, precision=G.PREC_MIXED, graph=True, overlap=1)
+    d.bind(ctx)
+    ctx.step(9)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        d.step(1)
+        d.bind(ctx)
+        ctx.step(1)
+    ctx.sync()
+    coupled_ms = (time.perf_counter() - t) / steps * 1e3
+    cpu_line = None
+    if cpu:
+        # one step of the numpy restatement on the host (FFTs on one core)
+        from oracle import dns_ref as DR
+        ref = DR.DnsRef(n, wl.L, wl.visc, dt)
+        ref.set_velocity(u0)
+        t = time.perf_counter()
+        ref.step()
+        sec = time.perf_counter() - t
+        cpu_line = {"value": round(sec * 1e3, 1), "unit": "ms/step", "cores": 1, "kind": "port",
+                    "sample": f"one ADV_FLUID step of the same {n}^3 field through the numpy "
+                              f"restatement (numpy FFT, one core), {sec:.1f}s"}
+    return {"metric": "dns_ms_per_step", "value": round(fluid_ms, 4), "unit": "ms",
+            "cpu_baseline": cpu_line,
+            "higher_is_better": False, "steps": steps, "warmup": warmup, "dtype": "f64",
+            "data": "synthetic", "config": {"workload": f"{wl.name} grid {n}^3, ADV_FLUID",
+                                             "dtime": dt, "visc": wl.visc},
+            "coupled_ms_per_step": round(coupled_ms, 4),
+            "coupled_particles_per_s": round(wl.np_ / (coupled_ms * 1e-3), 1),
+            "eps_flu": info["eps"], "umax": [float(x) for x in info["umax"]],
+            "algorithmic_bytes_per_step": bytes_step,
+            "effective_GBps": round(bytes_step / (fluid_ms * 1e-3) / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -316,6 +378,10 @@ def main():
     ap.add_argument("--stats", action="store_true",
                     help="time the statistics routines on the workload (particle, "
                          "particle-fluid and collision statistics) instead of the step")
+    ap.add_argument("--dns", action="store_true",
+                    help="time the spectral DNS field supplier on the workload's grid: "
+                         "ADV_FLUID alone and coupled (fluid step, device bind, particle "
+                         "step) instead of the frozen-field step")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (torch), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profile")
@@ -347,6 +413,9 @@ def main():
     groups = wl.meta.get("group") if cfg == "C4" else None
     if args.stats:
         print(json.dumps(stats_bench(wl, G, W, args.steps)))
+        return
+    if args.dns:
+        print(json.dumps(dns_bench(wl, G, W, args.steps, args.warmup, not args.no_cpu)))
         return
 
     kw = {"brick": args.brick, "lm_waves": args.lm_waves}

@@ -49,7 +49,7 @@ EXPORTS = (
     "gp_seed_net_train", "gp_seed_net_pairs", "gp_seed_net_fit", "gp_seed_net_eval",
     "gp_part_file_write", "gp_part_file_info", "gp_part_file_read", "gp_field_file_write",
     "gp_field_file_read", "gp_particle_stats", "gp_set_collision_stats", "gp_get_collision_stats",
-    "gp_partfluid_stats", "gp_bind_field_device", "gp_dns_create", "gp_dns_destroy",
+    "gp_partfluid_stats", "gp_bind_field_device", "gp_set_dtime", "gp_dns_create", "gp_dns_destroy",
     "gp_dns_error_string", "gp_dns_set_velocity", "gp_dns_get_velocity", "gp_dns_get_vorticity",
     "gp_dns_step", "gp_dns_cfl", "gp_dns_info", "gp_dns_bind",
 )
@@ -118,6 +118,7 @@ def load_library(path: str = LIB_PATH):
     L.gp_sync.argtypes = [vp]
     L.gp_bind_field.argtypes = [vp, dp]
     L.gp_bind_field_device.argtypes = [vp, vp]
+    L.gp_set_dtime.argtypes = [vp, C.c_double]
     L.gp_set_species.argtypes = [vp, C.c_int, dp]
     L.gp_set_particles.argtypes = [vp, C.c_int, dp, ip]
     L.gp_set_groups.argtypes = [vp, C.c_int, ip]
@@ -344,6 +345,10 @@ class Context:
         n = C.c_int64(0)
         self._chk(self.lib.gp_get_collision_stats(self.h, _ptr(last), _ptr(tsum), C.byref(n)))
         return {"last": last, "time_sum": tsum, "nstat": n.value}
+
+    def set_dtime(self, dtime):
+        """DTIME of the following steps (e.g. from the fluid's CHECK_CFL)."""
+        self._chk(self.lib.gp_set_dtime(self.h, float(dtime)))
 
     def step(self, nsteps=1):
         """nsteps x (ADV_PARTICLE_ELLP + COLLISION), asynchronous."""

@@ -2905,6 +2905,17 @@ int gp_collision(gp_ctx* ctx, int ncycle, double time) {
   return rc;
 }
 
+int gp_set_dtime(gp_ctx* ctx, double dtime) {
+  if (!ctx || !(dtime > 0.0)) return GP_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st2));
+  // the step's kernels take DTIME by value: a captured graph is re-captured
+  if (dtime != ctx->prm.dtime) drop_graph(ctx);
+  ctx->prm.dtime = dtime;
+  ctx->sp.dtime = dtime;
+  return GP_OK;
+}
+
 int gp_set_collision_stats(gp_ctx* ctx, int nclass, int npoly, int fout2, int ncycle) {
   if (!ctx) return GP_ERR_ARG;
   HIPCHK(hipStreamSynchronize(ctx->st));

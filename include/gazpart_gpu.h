@@ -185,6 +185,9 @@ int gp_collision(gp_ctx *ctx, int ncycle, double time);
 /* nsteps x (ADV_PARTICLE_ELLP + COLLISION), asynchronous on the context's
  * stream; errors surface at the next gp_sync / gp_get_particles.        */
 int gp_step(gp_ctx *ctx, int nsteps);
+/* DTIME for the following steps (CHECK_CFL changes it every cycle when the
+ * fluid evolves, check_cfl.f90:92); a captured step graph is re-captured */
+int gp_set_dtime(gp_ctx *ctx, double dtime);
 int gp_get_stats(gp_ctx *ctx, gp_stats *out);
 
 /* STAT_PARTICLE, level 1 (stat_particle.f90:169-450) over the resident
