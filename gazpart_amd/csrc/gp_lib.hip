@@ -487,15 +487,6 @@ __global__ __launch_bounds__(ADVL_BLOCK) void k_advance_listed(
   }
 }
 
-// flags of the listed particles back to 0 (after both advances are done)
-__global__ void k_clear_listed(const int* __restrict__ list, const unsigned long long* nlist,
-                               int* __restrict__ flag, int np) {
-  const long long nl = (long long)min(*nlist, (unsigned long long)np);
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nl;
-       t += (long long)gridDim.x * blockDim.x)
-    flag[list[t]] = 0;
-}
-
 // cell binning of the current positions (when no advance preceded)
 template <typename VT>
 __global__ __launch_bounds__(BLOCK) void k_cells(PState<VT> p, int np, CellGrid g,
@@ -519,6 +510,36 @@ __global__ __launch_bounds__(BLOCK) void k_cells(PState<VT> p, int np, CellGrid 
 
 // visiting order of the next advance: particles grouped by brick (any
 // order inside a brick; results do not depend on the storage order)
+// Clears folded into kernels that run at that point anyway, each saving a
+// launch on the step's critical path: int arrays to zero (bin counts),
+// counters to zero, and flag[list[t]] = 0 for the listed particles.  Every
+// thread of the grid takes part (grid-stride); members left null do nothing.
+struct SideClear {
+  int* zi = nullptr;
+  long long nzi = 0;
+  int* zi2 = nullptr;
+  long long nzi2 = 0;
+  unsigned long long* zc = nullptr;
+  int nzc = 0;
+  const int* list = nullptr;
+  const unsigned long long* nlist = nullptr;
+  int* flag = nullptr;
+  int np = 0;
+};
+
+__device__ __forceinline__ void side_clear(const SideClear& sc) {
+  const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long st = (long long)gridDim.x * blockDim.x;
+  for (long long t = t0; t < sc.nzi; t += st) sc.zi[t] = 0;
+  for (long long t = t0; t < sc.nzi2; t += st) sc.zi2[t] = 0;
+  if (sc.zc)
+    for (long long t = t0; t < sc.nzc; t += st) sc.zc[t] = 0ull;
+  if (sc.list) {
+    const long long nl = (long long)min(*sc.nlist, (unsigned long long)sc.np);
+    for (long long t = t0; t < nl; t += st) sc.flag[sc.list[t]] = 0;
+  }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_brick_scatter(const int* __restrict__ brick,
                                                          const int* __restrict__ bslot,
                                                          const int* __restrict__ boff,
@@ -534,7 +555,8 @@ __global__ __launch_bounds__(BLOCK) void k_brick_scatter(const int* __restrict__
 __global__ __launch_bounds__(BLOCK) void k_scatter(const int* __restrict__ cell,
                                                    const int* __restrict__ slot,
                                                    const int* __restrict__ off,
-                                                   int* __restrict__ tmp, int np) {
+                                                   int* __restrict__ tmp, int np, SideClear sc) {
+  side_clear(sc);
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   int c = cell[i];
@@ -556,7 +578,9 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
                                                 int* __restrict__ sorted,
                                                 float4* __restrict__ pos4,
                                                 int* __restrict__ csort,
-                                                int* __restrict__ inv, int np, int xcd) {
+                                                int* __restrict__ inv, int np, int xcd,
+                                                SideClear sc) {
+  side_clear(sc);
   int i = block_index(xcd) * BLOCK + threadIdx.x;
   if (i >= np) return;
   int c = cell[i];
@@ -782,7 +806,8 @@ __global__ __launch_bounds__(BLOCK) void k_pw_filter(
     PState<VT> p, const Species* __restrict__ spec, const WorkRec* __restrict__ work,
     const unsigned long long* nwork, unsigned long long cap, double L, int use_pw,
     WorkRec* __restrict__ lm, unsigned long long* nlm, int* __restrict__ flag,
-    int* __restrict__ plist, unsigned long long* nplist) {
+    int* __restrict__ plist, unsigned long long* nplist, SideClear sc) {
+  side_clear(sc);
   const unsigned long long nw = min(*nwork, cap);
   const int lane = threadIdx.x & 63;
   // the loop bound is uniform over the workgroup, so every lane of a
@@ -1529,6 +1554,9 @@ struct gp_ctx {
   // the particles of an advance of the current particles
   std::vector<double> sp_rows;
   bool fluid_valid = false;
+  // set around a sort inside pipelined steps: its scatter clears the
+  // previous step's solve flags
+  bool clear_plist = false;
 };
 
 static void drop_graph(gp_ctx* ctx) {
@@ -2459,13 +2487,24 @@ static int launch_sort(gp_ctx* ctx, bool offsets = true) {
     if ((rc = launch_bin<VT>(ctx))) return rc;
   scan_exclusive_int(ctx->cnt, ctx->off, g.nbox, (int*)ctx->scan_scr, ctx->st);
   auto ps = pstate<VT>(ctx, ctx->cur);
+  // the scatter also clears the previous pipelined step's solve flags (its
+  // advances are done); the ranking zeroes the step's counters 1..11
+  SideClear sc_flags, sc_cnt;
+  if (ctx->clear_plist) {
+    sc_flags.list = ctx->plist;
+    sc_flags.nlist = ctx->d_cnt + 10;
+    sc_flags.flag = ctx->inlm;
+    sc_flags.np = np;
+  }
+  sc_cnt.zc = ctx->d_cnt + 1;
+  sc_cnt.nzc = 11;
   hipLaunchKernelGGL(k_scatter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->slot,
-                     ctx->off, ctx->tmp, np);
+                     ctx->off, ctx->tmp, np, sc_flags);
   const int* grp = ctx->ngroups > 1 ? ps.grp : nullptr;
   hipLaunchKernelGGL(k_rank, dim3(xcd_grid(ctx, np)), dim3(BLOCK), 0, ctx->st, ctx->cell,
                      ctx->off, ctx->cnt, ctx->tmp, ps.id, grp, ps.x, ps.y, ps.z, ps.rm,
                      ctx->sorted, ctx->pos4, ctx->csort, ctx->use_bricks ? nullptr : ctx->inv, np,
-                     ctx->prm.xcd_remap);
+                     ctx->prm.xcd_remap, sc_cnt);
   if (offsets) {
     hipLaunchKernelGGL(k_pair_count, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, g, ctx->sorted,
                        ctx->cell, ctx->off, ctx->cnt, grp, ctx->pcount, np);
@@ -2572,13 +2611,15 @@ static float prefilter_tol(const gp_ctx* ctx) {
   return (float)(1.0e-5 * ext);
 }
 
-// prefilters on the sorted lists (asynchronous); list_lm: also flag and
-// list the particles of the contact solves (for a pipelined next advance)
+// prefilters on the sorted lists (asynchronous; the counters 1..11 were
+// zeroed by the sort); list_lm: also flag and list the particles of the
+// contact solves (for a pipelined next advance); clear_bins: the PW filter
+// also zeroes the bin counts and the wall count for the next advance (the
+// prefilter was their last reader)
 template <typename VT>
-static int launch_contacts_filter(gp_ctx* ctx, bool list_lm) {
+static int launch_contacts_filter(gp_ctx* ctx, bool list_lm, bool clear_bins = false) {
   const int np = ctx->np;
   unsigned long long* c = ctx->d_cnt;
-  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(16), 0, ctx->st, c + 1, 0ull, 11ll);
   auto ps = pstate<VT>(ctx, ctx->cur);
   hipLaunchKernelGGL(k_prefilter, dim3(xcd_grid(ctx, np)), dim3(BLOCK), 0, ctx->st, ctx->g,
                      ctx->pos4, ctx->sorted, ctx->csort, ctx->off, ctx->cnt,
@@ -2588,9 +2629,20 @@ static int launch_contacts_filter(gp_ctx* ctx, bool list_lm) {
   // the survivor count stays on the device: a fixed grid, grid-stride inside
   const unsigned fgrid = (unsigned)std::min<unsigned long long>((ctx->work_cap + BLOCK - 1) / BLOCK,
                                                                  2048ull);
+  SideClear sc;
+  if (clear_bins) {
+    sc.zi = ctx->cnt;
+    sc.nzi = ctx->g.nbox;
+    if (ctx->use_bricks) {
+      sc.zi2 = ctx->bcnt;
+      sc.nzi2 = ctx->bg.n;
+    }
+    sc.zc = c;
+    sc.nzc = 1;
+  }
   hipLaunchKernelGGL(k_pw_filter<VT>, dim3(fgrid), dim3(BLOCK), 0, ctx->st, ps, ctx->spec,
                      ctx->work, c + 1, ctx->work_cap, ctx->prm.L, ctx->prm.prefilter >= 2 ? 1 : 0,
-                     ctx->lmwork, c + 7, list_lm ? ctx->inlm : nullptr, ctx->plist, c + 10);
+                     ctx->lmwork, c + 7, list_lm ? ctx->inlm : nullptr, ctx->plist, c + 10, sc);
   HIPCHK(hipGetLastError());
   return GP_OK;
 }
@@ -2677,7 +2729,11 @@ static int launch_steps(gp_ctx* ctx, int k) {
   if ((rc = launch_advance<VT>(ctx, true))) return rc;
   for (int s = 0; s < k; ++s) {
     const bool next = s + 1 < k;
-    if ((rc = launch_sort<VT>(ctx, false))) return rc;
+    // from the second step on, the sort clears the previous step's flags
+    ctx->clear_plist = pipe && s > 0;
+    rc = launch_sort<VT>(ctx, false);
+    ctx->clear_plist = false;
+    if (rc) return rc;
     if (!(pipe && next)) {
       if ((rc = launch_contacts<VT>(ctx))) return rc;
       if (next) {
@@ -2686,11 +2742,10 @@ static int launch_steps(gp_ctx* ctx, int k) {
       }
       continue;
     }
-    if ((rc = launch_contacts_filter<VT>(ctx, true))) return rc;
-    // the prefilter was the last reader of this step's bins: clear them (and
-    // the wall count) on the main stream, before both advances
-    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, c, 0ull, 1ll);
-    zero_bins(ctx, ctx->st);
+    // the prefilter is the last reader of this step's bins: the PW filter
+    // clears them (and the wall count) on the main stream, before both
+    // advances
+    if ((rc = launch_contacts_filter<VT>(ctx, true, true))) return rc;
     // fork: the visiting order of the next advance is this sort's order
     const int* order = ctx->use_bricks ? ctx->trav : ctx->sorted;
     HIPCHK(hipEventRecord(ctx->ev_fork, ctx->st));
@@ -2702,9 +2757,8 @@ static int launch_steps(gp_ctx* ctx, int k) {
     launch_advance_listed<VT>(ctx, ctx->st);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamWaitEvent(ctx->st, ctx->ev_join, 0));
-    // the flags were read by the advance beside the solves: clear them now
-    hipLaunchKernelGGL(k_clear_listed, dim3(64), dim3(BLOCK), 0, ctx->st, ctx->plist, c + 10,
-                       ctx->inlm, ctx->np);
+    // the flags were read by the advance beside the solves: the next step's
+    // sort clears them (and the next gp_step starts from a full clear)
     ctx->cur = 1 - ctx->cur;
     ctx->perm_identity = true;
     ctx->cells_valid = true;
