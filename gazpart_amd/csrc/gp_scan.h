@@ -149,6 +149,57 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_from_sums(const int* __rest
   if ((int)blockIdx.x == ntiles - 1 && threadIdx.x == 0) out[n] = s_excl + agg;
 }
 
+// Short inputs (the cell counts of a typical grid, a few ten thousand) in
+// one workgroup and one launch: passes of SCAN1_CHUNK inputs are staged in
+// LDS with coalesced loads, each thread scans its SCAN1_ITEMS consecutive
+// values there, the thread totals are scanned across the workgroup and
+// carried from pass to pass, and the results leave coalesced.  Above
+// SCAN1_MAX inputs the two-launch scan's parallelism wins.
+constexpr int SCAN1_BLOCK = 1024;
+constexpr int SCAN1_ITEMS = 8;
+constexpr int SCAN1_CHUNK = SCAN1_BLOCK * SCAN1_ITEMS;   // 32 KiB of LDS
+constexpr int SCAN1_MAX = 4 * SCAN1_CHUNK;
+
+__global__ __launch_bounds__(SCAN1_BLOCK) void k_scan_onewg(const int* __restrict__ in,
+                                                            int* __restrict__ out, int n) {
+  __shared__ int buf[SCAN1_CHUNK];
+  __shared__ int wsum[SCAN1_BLOCK / 64];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int carry = 0;
+  for (int c0 = 0; c0 < n; c0 += SCAN1_CHUNK) {
+    for (int k = threadIdx.x; k < SCAN1_CHUNK; k += SCAN1_BLOCK)
+      buf[k] = (c0 + k < n) ? in[c0 + k] : 0;
+    __syncthreads();
+    int v[SCAN1_ITEMS];
+    int tot = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN1_ITEMS; ++k) {
+      v[k] = buf[threadIdx.x * SCAN1_ITEMS + k];
+      tot += v[k];
+    }
+    const int incl = wave_incl_scan(tot);
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int woff = 0, agg = 0;
+    for (int w = 0; w < SCAN1_BLOCK / 64; ++w) {
+      if (w < wid) woff += wsum[w];
+      agg += wsum[w];
+    }
+    int run = carry + woff + incl - tot;
+#pragma unroll
+    for (int k = 0; k < SCAN1_ITEMS; ++k) {
+      buf[threadIdx.x * SCAN1_ITEMS + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < SCAN1_CHUNK; k += SCAN1_BLOCK)
+      if (c0 + k < n) out[c0 + k] = buf[k];
+    carry += agg;
+    __syncthreads();   // buf and wsum are rewritten by the next pass
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
 // out[i] = sum_{k<i} in[k] for i < n, out[n] = total; sums holds one int
 // per tile of SCAN_TILE inputs
 inline int64_t scan2_sums(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
@@ -157,6 +208,10 @@ inline void scan_exclusive_int(const int* in, int* out, int n, int* sums, hipStr
   const int ntiles = (int)scan2_sums(n);
   if (n <= 0) {
     hipLaunchKernelGGL((k_copy1<int>), dim3(1), dim3(1), 0, st, out, (const int*)out, 1);
+    return;
+  }
+  if (n <= SCAN1_MAX) {
+    hipLaunchKernelGGL(k_scan_onewg, dim3(1), dim3(SCAN1_BLOCK), 0, st, in, out, n);
     return;
   }
   hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, st, in, sums, n);

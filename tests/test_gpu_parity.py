@@ -427,3 +427,23 @@ def test_error_out_of_grid_and_interp_range(c1):
     with pytest.raises(G.GpError) as e:
         ctx.interph(5, 0, np.array([1.0]), np.array([1.0]), np.array([1.0]))
     assert e.value.code == 10
+
+
+@pytest.mark.parametrize("dpmax", [0.9, 0.06])
+def test_cell_list_bitexact_short_and_long_scans(dpmax):
+    """The cell-count scan runs in one workgroup for grids of at most 32768
+    cells and as two launches above: both give the reference's HOME/NEAR
+    lists (C oracle) bit for bit (dpmax 0.9: 27 cells; 0.06: 140608)."""
+    wl = W.c1(np_=2000)
+    wl.dpmax = dpmax
+    p = O.params(wl)
+    g = O.collision_grid(p, dpmax)
+    assert (g.nbox <= 32768) == (dpmax > 0.5)
+    ctx = G.Context.from_workload(wl)
+    home, near = ctx.collision_detection_connectivity()
+    con, ncon = O.connectivity(g)
+    st = wl.state
+    ho, no, nc, err = O.detect(g, con, ncon, st[0], st[1], st[2], 10 ** 7)
+    assert err == 0 and nc > 0
+    np.testing.assert_array_equal(home, ho)
+    np.testing.assert_array_equal(near, no)
