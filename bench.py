@@ -235,7 +235,7 @@ def make_workload(cfg, args, W):
     return W.c5(n=args.n or 256, np_=args.np or 2_000_000, scheme=scheme), scheme
 
 
-def stats_bench(wl, G, W, steps):
+def stats_bench(wl, G, W, steps, cpu=True):
     """Host-timed calls (device work + result copy) of the statistics
     routines on the workload's state after 3 steps: with the workload's
     own species rows and with the particles regrouped into 2 classes x 3
@@ -275,6 +275,20 @@ def stats_bench(wl, G, W, steps):
     c6.adv_particle_ellp(1)
     out["particle_stats_2x3"] = per_call(lambda: c6.particle_stats(npoly=3))
     out["partfluid_stats_2x3"] = per_call(lambda: c6.partfluid_stats(npoly=3))
+    if cpu:
+        # the numpy restatements on the same state, one host core
+        from oracle import stats_ref as SR
+        st6, fl6 = c6.get_particles(), c6.get_fluid_at_particles()
+        t = time.perf_counter()
+        SR.particle_stats(st6, sid6, rows6, npoly=3)
+        t1 = time.perf_counter()
+        SR.partfluid_stats(st6, fl6, sid6, rows6, wl.visc, wl.rhof, npoly=3)
+        t2 = time.perf_counter()
+        out["cpu_baseline"] = {
+            "particle_stats_2x3_ms": round((t1 - t) * 1e3, 1),
+            "partfluid_stats_2x3_ms": round((t2 - t1) * 1e3, 1), "cores": 1, "kind": "port",
+            "sample": f"one call of each numpy restatement on the same {wl.np_} particles "
+                      f"(2 classes x 3 sizes), one core"}
     c6.close()
     for name, fout2 in (("step_ms_collision_stats_off", 0), ("step_ms_collision_stats_every_step", 1)):
         cx = make6(graph=True, overlap=1)
@@ -412,7 +426,7 @@ def main():
         f"dt={wl.dtime:.3e} dpmax={wl.dpmax:.3e}")
     groups = wl.meta.get("group") if cfg == "C4" else None
     if args.stats:
-        print(json.dumps(stats_bench(wl, G, W, args.steps)))
+        print(json.dumps(stats_bench(wl, G, W, args.steps, not args.no_cpu)))
         return
     if args.dns:
         print(json.dumps(dns_bench(wl, G, W, args.steps, args.warmup, not args.no_cpu)))

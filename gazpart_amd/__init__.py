@@ -49,7 +49,7 @@ EXPORTS = (
     "gp_seed_net_train", "gp_seed_net_pairs", "gp_seed_net_fit", "gp_seed_net_eval",
     "gp_part_file_write", "gp_part_file_info", "gp_part_file_read", "gp_field_file_write",
     "gp_field_file_read", "gp_particle_stats", "gp_set_collision_stats", "gp_get_collision_stats",
-    "gp_partfluid_stats", "gp_bind_field_device", "gp_set_dtime", "gp_dns_create", "gp_dns_destroy",
+    "gp_partfluid_stats", "gp_bind_field_device", "gp_bind_field_planar", "gp_set_dtime", "gp_dns_create", "gp_dns_destroy",
     "gp_dns_error_string", "gp_dns_set_velocity", "gp_dns_get_velocity", "gp_dns_get_vorticity",
     "gp_dns_step", "gp_dns_cfl", "gp_dns_info", "gp_dns_bind",
 )

@@ -248,26 +248,6 @@ __global__ __launch_bounds__(BLOCK) void k_absmax3(Grid g, const double* __restr
     for (int c = 0; c < 3; ++c) part[3 * blockIdx.x + c] = s[c][0];
 }
 
-// velocity (u) and vorticity (z) -> the reference layout
-// [6][n+4][n+4][n] with periodic ghost planes (index period n)
-__global__ __launch_bounds__(BLOCK) void k_to_uain(Grid g, const double* __restrict__ u,
-                                                   const double* __restrict__ z, double vscale,
-                                                   double* __restrict__ out) {
-  const int n = g.n, n4 = n + 4;
-  const long long per = (long long)n4 * n4 * n;
-  for (long long q = (long long)blockIdx.x * BLOCK + threadIdx.x; q < 6 * per;
-       q += (long long)gridDim.x * BLOCK) {
-    const int c = (int)(q / per);
-    const long long r = q % per;
-    const int i = (int)(r % n);
-    const int jj = (int)((r / n) % n4);
-    const int kk = (int)(r / ((long long)n * n4));
-    const int j = ((jj - 2) % n + n) % n, k = ((kk - 2) % n + n) % n;
-    const long long p = ((long long)k * n + j) * n + i;
-    out[q] = c < 3 ? u[c * g.nr + p] : vscale * z[(c - 3) * g.nr + p];
-  }
-}
-
 // ---------------------------------------------------------------------------
 // the reference's RAN1 (Park-Miller minimal standard, Schrage's method,
 // Bays-Durham shuffle of 32) and GAUSS (polar Box-Muller, one value kept),
@@ -338,7 +318,6 @@ struct gp_dns {
   rocfft_plan fwd = nullptr, bwd = nullptr;
   rocfft_execution_info info = nullptr;
   void* work = nullptr;
-  double* stage = nullptr;  // the reference field layout for gp_dns_bind
   bool have_u = false;
   double eps = 0.0;
   // forcing: modes in the reference's loop order, amplitudes (host), the
@@ -547,8 +526,7 @@ int gp_dns_destroy(gp_dns* d) {
   if (d->fwd) rocfft_plan_destroy(d->fwd);
   if (d->bwd) rocfft_plan_destroy(d->bwd);
   if (d->info) rocfft_execution_info_destroy(d->info);
-  void* bufs[] = {d->u, d->z, d->U, d->Un, d->A, d->B, d->part, d->work, d->fmap, d->fdev,
-                  d->stage};
+  void* bufs[] = {d->u, d->z, d->U, d->Un, d->A, d->B, d->part, d->work, d->fmap, d->fdev};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (d->st) hipStreamDestroy(d->st);
@@ -673,19 +651,11 @@ int gp_dns_bind(gp_dns* d, gp_ctx* ctx, int mesh_vorticity) {
   }
   int rc;
   if ((rc = vorticity_dev(d))) return rc;
-  const long long nout = 6LL * (d->g.n + 4) * (d->g.n + 4) * d->g.n;
-  if (!d->stage && hipMalloc((void**)&d->stage, (size_t)nout * sizeof(double)) != hipSuccess) {
-    d->stage = nullptr;
-    d->err = "gp_dns_bind: out of device memory";
-    return GP_ERR_NOMEM;
-  }
-  double* out = d->stage;
   const double vs = mesh_vorticity ? (double)(d->g.n - 1) / (double)d->g.n : 1.0;
-  hipLaunchKernelGGL(k_to_uain, dim3(grid_of(nout)), dim3(BLOCK), 0, d->st, d->g,
-                     (const double*)d->u, (const double*)d->z, vs, out);
-  DCHK(hipGetLastError());
   DCHK(hipStreamSynchronize(d->st));
-  return gp_bind_field_device(ctx, out);
+  const int rc2 = gp_bind_field_planar(ctx, d->g.n, d->u, d->z, vs);
+  if (rc2) d->err = gp_last_error_string(ctx);
+  return rc2;
 }
 
 }  // extern "C"

@@ -341,6 +341,27 @@ __global__ void k_field_prep(const double* __restrict__ ref, FT* __restrict__ ou
   for (int c = 0; c < NC; ++c) out[t * NC + c] = (FT)ref[c * comp + src];
 }
 
+// the same node layout straight from planar device arrays (velocity and
+// vorticity, [3][n][n][n], x fastest, no ghosts; all ghosts index period n),
+// the vorticity scaled by vscale: one pass instead of the reference layout
+// and a second copy
+template <typename FT>
+__global__ void k_field_prep_planar(const double* __restrict__ u, const double* __restrict__ w,
+                                    double vscale, FT* __restrict__ out, int n) {
+  const int n4 = n + 4;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)n4 * n4 * n4;
+  if (t >= total) return;
+  const int ii = (int)(t % n4), jj = (int)((t / n4) % n4), kk = (int)(t / ((long long)n4 * n4));
+  const int i0 = ((ii - 2) % n + n) % n, j0 = ((jj - 2) % n + n) % n, k0 = ((kk - 2) % n + n) % n;
+  const long long nr = (long long)n * n * n;
+  const long long src = ((long long)k0 * n + j0) * n + i0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[t * 6 + c] = (FT)u[c * nr + src];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[t * 6 + 3 + c] = (FT)(vscale * w[c * nr + src]);
+}
+
 template <typename FT, typename CT, int S>
 __global__ __launch_bounds__(BLOCK) void k_interph(const FT* __restrict__ f, int n, double dx,
                                                    const double* __restrict__ x,
@@ -2089,6 +2110,32 @@ int gp_bind_field(gp_ctx* ctx, const double* field) {
   const int rc = bind_field_dev(ctx, dref);
   hipFree(dref);
   return rc;
+}
+
+int gp_bind_field_planar(gp_ctx* ctx, int n_arrays, const double* uvw, const double* vort,
+                         double vort_scale) {
+  if (!ctx || !uvw || !vort) return GP_ERR_ARG;
+  if (n_arrays != ctx->prm.n) {
+    ctx->err = "field arrays of n = " + std::to_string(n_arrays) + " for a context of n = " +
+               std::to_string(ctx->prm.n);
+    return GP_ERR_ARG;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st2));
+  const int n = ctx->prm.n, n4 = n + 4;
+  const size_t nodes = (size_t)n4 * n4 * n4;
+  const size_t esz = ctx->mixed ? sizeof(float) : sizeof(double);
+  if (!ctx->field) HIPCHK(hipMalloc(&ctx->field, nodes * 6 * esz + 256));
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_field_prep_planar<float>, dim3(nblk((long long)nodes)), dim3(BLOCK), 0,
+                       ctx->st, uvw, vort, vort_scale, (float*)ctx->field, n);
+  else
+    hipLaunchKernelGGL(k_field_prep_planar<double>, dim3(nblk((long long)nodes)), dim3(BLOCK), 0,
+                       ctx->st, uvw, vort, vort_scale, (double*)ctx->field, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->have_field = true;
+  return GP_OK;
 }
 
 int gp_bind_field_device(gp_ctx* ctx, const double* field) {

@@ -138,6 +138,12 @@ int gp_bind_field(gp_ctx *ctx, const double *field);
  * Re-binding keeps the context's buffers, so captured step graphs stay
  * valid. (No reference counterpart: its field lives in host arrays.)    */
 int gp_bind_field_device(gp_ctx *ctx, const double *field);
+/* the same from planar device arrays without ghosts: velocity and
+ * vorticity double[3][n][n][n] each (x fastest; n must be the context's),
+ * the vorticity multiplied by vort_scale; ghosts (x too) with index
+ * period n.  One pass.                                                   */
+int gp_bind_field_planar(gp_ctx *ctx, int n, const double *uvw, const double *vort,
+                         double vort_scale);
 /* per-species constants (PARAM_PHYS EMAJ_PART, APR_PART, RHOP, IPXX..)  */
 int gp_set_species(gp_ctx *ctx, int nspec, const double *rows);
 /* PART(1:np) -> device; species[i] in [0, nspec)                       */

@@ -286,3 +286,12 @@ def test_gpu_set_dtime_equals_a_context_created_with_it(G, graph):
     c.set_particles(b.get_particles(), wl.species_id)
     c.step(6)
     np.testing.assert_array_equal(a.get_particles(), c.get_particles())
+
+
+@pytest.mark.gpu
+def test_gpu_bind_rejects_a_context_of_another_grid(G):
+    d = _dns(32, 0.01, 1e-3)
+    d.set_velocity(random_velocity(32, 3.0, 5))
+    ctx = G.Context(64, L2PI, 0.01, 1.0, 1e-3)
+    with pytest.raises(G.GpError):
+        d.bind(ctx)
