@@ -327,6 +327,13 @@ def dns_bench(wl, G, W, steps, warmup, cpu=True):
     d.step(steps)
     fluid_ms = (time.perf_counter() - t) / steps * 1e3
     info = d.info()
+    dr = Dns(n, wl.L, wl.visc, dt, reuse_spectral=True)
+    dr.set_velocity(u0)
+    dr.step(warmup)
+    t = time.perf_counter()
+    dr.step(steps)
+    reuse_ms = (time.perf_counter() - t) / steps * 1e3
+    dr.close()
     # algorithmic HBM bytes of one ADV_FLUID: 7 batched transforms (read +
     # write of three fields each), the velocity copy, and the element-wise
     # kernels' reads and writes (spectral fields 16 B x n^2 (n/2+1) each,
@@ -362,6 +369,7 @@ def dns_bench(wl, G, W, steps, warmup, cpu=True):
             "higher_is_better": False, "steps": steps, "warmup": warmup, "dtype": "f64",
             "data": "synthetic", "config": {"workload": f"{wl.name} grid {n}^3, ADV_FLUID",
                                              "dtime": dt, "visc": wl.visc},
+            "reuse_spectral_ms_per_step": round(reuse_ms, 4),
             "coupled_ms_per_step": round(coupled_ms, 4),
             "coupled_particles_per_s": round(wl.np_ / (coupled_ms * 1e-3), 1),
             "eps_flu": info["eps"], "umax": [float(x) for x in info["umax"]],

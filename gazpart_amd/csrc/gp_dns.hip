@@ -405,11 +405,17 @@ int one_step(gp_dns* d) {
   const int nf = (int)d->fmode.size();
   const int* fmap = (d->prm.forcing && nf) ? d->fmap : nullptr;
   int rc;
-  // entry transform of the velocity (adv_fluid.f90:86-94); u is kept
-  DCHK(hipMemcpyAsync(d->z, d->u, 3 * g.nr * sizeof(double), hipMemcpyDeviceToDevice, d->st));
-  if ((rc = r2c(d, d->z, d->U))) return rc;
-  if ((rc = advance_forcing(d))) return rc;
-  hipLaunchKernelGGL(k_scale_curl, dim3(gc), dim3(BLOCK), 0, d->st, g, d->U, d->A, factor);
+  if (d->prm.reuse_spectral) {
+    // U is the spectral state u came from (set_velocity or the last step)
+    if ((rc = advance_forcing(d))) return rc;
+    hipLaunchKernelGGL(k_curl, dim3(gc), dim3(BLOCK), 0, d->st, g, (const double2*)d->U, d->A);
+  } else {
+    // entry transform of the velocity (adv_fluid.f90:86-94); u is kept
+    DCHK(hipMemcpyAsync(d->z, d->u, 3 * g.nr * sizeof(double), hipMemcpyDeviceToDevice, d->st));
+    if ((rc = r2c(d, d->z, d->U))) return rc;
+    if ((rc = advance_forcing(d))) return rc;
+    hipLaunchKernelGGL(k_scale_curl, dim3(gc), dim3(BLOCK), 0, d->st, g, d->U, d->A, factor);
+  }
   // stage 1
   if ((rc = c2r(d, d->A, d->z))) return rc;
   hipLaunchKernelGGL(k_cross, dim3(RED_BLOCKS), dim3(BLOCK), 0, d->st, g, (const double*)d->u,

@@ -23,7 +23,7 @@ class DnsParams(C.Structure):
                 ("dtime", C.c_double), ("forcing", C.c_int), ("seed", C.c_int),
                 ("kforce_min", C.c_double), ("kforce_max", C.c_double),
                 ("sigma_force", C.c_double), ("time_force", C.c_double),
-                ("reserved", C.c_int * 4)]
+                ("reuse_spectral", C.c_int), ("reserved", C.c_int * 3)]
 
 
 def _lib():
@@ -48,10 +48,12 @@ def _lib():
 class Dns:
     """One DNS state on the GPU (double precision)."""
 
-    def __init__(self, n: int, L: float, visc: float, dtime: float, forcing: dict | None = None):
+    def __init__(self, n: int, L: float, visc: float, dtime: float, forcing: dict | None = None,
+                 reuse_spectral: bool = False):
         self.lib = _lib()
         p = DnsParams()
         p.n, p.L, p.visc, p.dtime = n, L, visc, dtime
+        p.reuse_spectral = 1 if reuse_spectral else 0
         if forcing:
             p.forcing = 1
             p.seed = int(forcing.get("seed", -1))

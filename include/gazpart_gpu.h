@@ -359,7 +359,13 @@ typedef struct {
   int forcing;           /* 1: STEADY forcing */
   int seed;              /* IDFORCE (RAN1 seed; <= 0 re-initialises the shuffle) */
   double kforce_min, kforce_max, sigma_force, time_force;
-  int reserved[4];
+  int reuse_spectral;    /* 1: start each step from the spectral state the
+                            previous one ended with instead of transforming
+                            UFLU again (adv_fluid.f90:86-94 does; the two
+                            differ by rounding only, the filter having
+                            removed the modes a round trip could change):
+                            one transform of seven less per step */
+  int reserved[3];
 } gp_dns_params;
 int gp_dns_create(const gp_dns_params *p, gp_dns **out);
 int gp_dns_destroy(gp_dns *d);

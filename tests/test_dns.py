@@ -295,3 +295,22 @@ def test_gpu_bind_rejects_a_context_of_another_grid(G):
     ctx = G.Context(64, L2PI, 0.01, 1.0, 1e-3)
     with pytest.raises(G.GpError):
         d.bind(ctx)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("forcing", [None, dict(kmin=0.5, kmax=2.5, sigma=0.5, tforce=0.1, seed=-3)])
+def test_gpu_reuse_spectral_differs_by_rounding_only(forcing):
+    """Starting each step from the last spectral state instead of
+    transforming UFLU again (one transform less) changes the field by
+    rounding only: 1e-12 of its magnitude after eight steps."""
+    n = 64
+    u0 = random_velocity(n, 6.0, 41)
+    out = []
+    for reuse in (False, True):
+        from gazpart_amd.dns import Dns
+        d = Dns(n, L2PI, 0.01, 0.004, forcing=forcing, reuse_spectral=reuse)
+        d.set_velocity(u0)
+        d.step(8)
+        out.append(d.velocity())
+    assert np.abs(out[1] - out[0]).max() <= 1e-12 * np.abs(out[0]).max()
+    assert not np.array_equal(out[0], u0)
