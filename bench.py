@@ -393,7 +393,8 @@ def main():
                          "(0: the library default)")
     ap.add_argument("--lm-waves", type=int, default=0,
                     help="wavefronts of the contact-solve launch (0: the library "
-                         "default, one per SIMD)")
+                         "default, from the solve count: one per SIMD while "
+                         "lanes sit idle, up to four per SIMD)")
     ap.add_argument("--pairs", type=int, default=10_000_000,
                     help="C5: solver-labelled pairs the seed network is trained on")
     ap.add_argument("--epochs", type=int, default=3, help="C5: training epochs")

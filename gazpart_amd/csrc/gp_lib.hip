@@ -1487,6 +1487,7 @@ struct gp_ctx {
   bool perm_identity = true;
   int max_sp = -1;              // largest species index of the particles
   int lm_waves = 1024;          // LM launch: one wavefront per SIMD
+  int nsimd = 1024;             // SIMDs of the device (4 per CU)
   // brick visiting order of the advance (gp_params.brick)
   BrickGrid bg{};
   bool use_bricks = false;
@@ -1997,7 +1998,8 @@ int gp_create(const gp_params* p, gp_ctx** out) {
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
             hipSuccess || ncu < 1)
       ncu = 256;
-    ctx->lm_waves = p->lm_waves > 0 ? p->lm_waves : 4 * ncu;   // default: one per SIMD
+    ctx->nsimd = 4 * ncu;
+    ctx->lm_waves = p->lm_waves > 0 ? p->lm_waves : ctx->nsimd;   // default: one per SIMD
   }
   StepParams& s = ctx->sp;
   s.n = p->n;
@@ -2845,6 +2847,18 @@ static int capture_graph(gp_ctx* ctx) {
   return GP_OK;
 }
 
+// Width of the contact-solve launch when gp_params.lm_waves = 0, from the
+// last known count of solves: while they leave lanes idle (fewer than one
+// per lane of one wavefront per SIMD) the solve is latency-bound and one
+// wavefront per SIMD is fastest; with several per lane it is throughput-
+// bound and more wavefronts hide each other's latency (2 per SIMD fit the
+// register budget; 4 queue behind them).  No effect on results.
+static void choose_lm_waves(gp_ctx* ctx, unsigned long long nlm) {
+  if (ctx->prm.lm_waves > 0) return;
+  const unsigned long long lanes = (unsigned long long)ctx->nsimd * 64ull;
+  ctx->lm_waves = ctx->nsimd * (nlm >= 4 * lanes ? 4 : nlm >= lanes ? 2 : 1);
+}
+
 template <typename VT>
 static int graph_steps(gp_ctx* ctx, int nsteps) {
   int rc;
@@ -2859,6 +2873,11 @@ static int graph_steps(gp_ctx* ctx, int nsteps) {
     --nsteps;
     const unsigned long long need = 2ull * (unsigned long long)ctx->last_nclose + 4096ull;
     if ((rc = ensure_lists(ctx, need))) return rc;
+    // the solve's launch width is fixed in the graph: from this step's count
+    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    choose_lm_waves(ctx, ctx->h_cnt[7]);
     if ((rc = capture_graph<VT>(ctx))) return rc;
   }
   const int unit = graph_unit(ctx);
@@ -2879,6 +2898,7 @@ static void accumulate_stats(gp_ctx* ctx) {
   ctx->stats.ncontact = (int64_t)h[4];
   ctx->stats.ncollision = (int64_t)h[5];
   ctx->stats.lm_nfev = (int64_t)h[3];
+  choose_lm_waves(ctx, h[7]);
   ctx->stats.greedy_rounds = (int64_t)h[6];
   // the candidate count the last prefilter saw (the sizing sort's before any)
   ctx->stats.nclose = h[11] ? (int64_t)h[11] : ctx->last_nclose;

@@ -96,7 +96,10 @@ typedef struct {
                           centre-line re-solve when the seeded solve ends
                           unconverged.  The solve decides every contact */
   int lm_waves;        /* wavefronts of the contact-solve launch (one pair per
-                          lane, pairs dealt lane-major); 0: one per SIMD */
+                          lane, pairs dealt lane-major); 0: automatic from the
+                          last known count of solves (one per SIMD while they
+                          leave lanes idle, two or four per SIMD when each
+                          lane has several) */
   int overlap;         /* 1: in graph-mode gp_step, the next step's advance
                           of the particles outside the contact solves runs on
                           a second stream beside the solves (8 steps per
