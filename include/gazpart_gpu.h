@@ -88,8 +88,9 @@ typedef struct {
                           GP_ERR_NCLOSE_OVERFLOW at the next gp_sync */
   int brick;           /* the advance visits particles grouped in bricks of
                           brick^3 collision cells (compact stencil sets per
-                          workgroup, less field re-fetch); 0: default (8);
-                          < 0: in reference cell order.  No effect on results */
+                          workgroup, less field re-fetch); 0 or < 0: in the
+                          reference's cell order (the default).  No effect on
+                          results */
   int lm_seed;         /* initial guess of the contact solve: 0 the centre
                           line (reference, ellipsoid_contact_detection.f90:
                           53-62); 1 the seed network (gp_seed_net_*), with a
