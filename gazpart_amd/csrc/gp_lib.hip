@@ -1999,7 +1999,7 @@ int gp_create(const gp_params* p, gp_ctx** out) {
             hipSuccess || ncu < 1)
       ncu = 256;
     ctx->nsimd = 4 * ncu;
-    ctx->lm_waves = p->lm_waves > 0 ? p->lm_waves : ctx->nsimd;   // default: one per SIMD
+    ctx->lm_waves = p->lm_waves > 0 ? p->lm_waves : ctx->nsimd;   // 0: one per SIMD until a solve count is known
   }
   StepParams& s = ctx->sp;
   s.n = p->n;
