@@ -499,7 +499,8 @@ def main():
         "dtype": "f32 gather/drag + f64 positions/quaternion/contact",
         "data": "synthetic",
         "config": {"workload": describe(cfg, wl, scheme), "np": wl.np_, "grid": wl.n,
-                   "interp": f"lag{scheme}", "launch": kw},
+                   "interp": f"lag{scheme}", "launch": kw,
+                   "solve_waves": m["stats"]["lm_waves"]},
         "fp64": {"value": round(f["updates_per_s"], 1), "ms_per_step": round(f["ms_per_step"], 4),
                  "dtype": "f64", "roofline": roof["fp64"], "stage_ms": stage(f["stats"]),
                  "per_step": f["stats"]["per_step"]},

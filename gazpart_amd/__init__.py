@@ -80,7 +80,8 @@ class Stats(C.Structure):
                 ("greedy_rounds", C.c_int64), ("t_advance_ms", C.c_double),
                 ("t_sort_ms", C.c_double), ("t_contact_ms", C.c_double),
                 ("t_greedy_ms", C.c_double), ("ntimed", C.c_int64), ("nlm", C.c_int64),
-                ("seed_fallbacks", C.c_int64), ("reserved", C.c_int64 * 2)]
+                ("seed_fallbacks", C.c_int64), ("lm_waves", C.c_int64),
+                ("reserved", C.c_int64 * 1)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -1488,6 +1488,7 @@ struct gp_ctx {
   int max_sp = -1;              // largest species index of the particles
   int lm_waves = 1024;          // LM launch: one wavefront per SIMD
   int nsimd = 1024;             // SIMDs of the device (4 per CU)
+  int lm_waves_last = 0;        // width of the last contact-solve launch issued
   // brick visiting order of the advance (gp_params.brick)
   BrickGrid bg{};
   bool use_bricks = false;
@@ -2708,6 +2709,7 @@ static int launch_contacts_solve(gp_ctx* ctx) {
     hipLaunchKernelGGL(k_mlp_seed<VT>, dim3(ctx->lm_waves), dim3(64), 0, ctx->st, ps, ctx->spec,
                        ctx->lmwork, c + 7, ctx->work_cap, ctx->prm.L, mlp_dev(ctx), ctx->seeds);
   const CStat cs = cstat(ctx);
+  ctx->lm_waves_last = ctx->lm_waves;   // a captured graph keeps this width
   hipLaunchKernelGGL(k_contact<VT>, dim3(ctx->lm_waves), dim3(LM_BLOCK), 0, ctx->st, ps, ctx->spec,
                      ctx->lmwork, c + 7, ctx->work_cap, ctx->prm.L, ctx->act, c + 2, ctx->act_cap,
                      c + 3, c + 4, ctx->d_err, net ? ctx->seeds : nullptr, c + 9,
@@ -2898,6 +2900,7 @@ static void accumulate_stats(gp_ctx* ctx) {
   ctx->stats.ncontact = (int64_t)h[4];
   ctx->stats.ncollision = (int64_t)h[5];
   ctx->stats.lm_nfev = (int64_t)h[3];
+  ctx->stats.lm_waves = ctx->lm_waves_last;
   choose_lm_waves(ctx, h[7]);
   ctx->stats.greedy_rounds = (int64_t)h[6];
   // the candidate count the last prefilter saw (the sizing sort's before any)

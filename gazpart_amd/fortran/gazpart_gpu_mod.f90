@@ -20,8 +20,8 @@ module gazpart_gpu
     integer(c_int64_t) :: nclose, nprefilter, ncontact, ncollision, lm_nfev, nwall
     integer(c_int64_t) :: greedy_rounds
     real(c_double) :: t_advance_ms, t_sort_ms, t_contact_ms, t_greedy_ms
-    integer(c_int64_t) :: ntimed, nlm, seed_fallbacks
-    integer(c_int64_t) :: reserved(2)
+    integer(c_int64_t) :: ntimed, nlm, seed_fallbacks, lm_waves
+    integer(c_int64_t) :: reserved(1)
   end type gp_stats
 
   type, bind(C) :: gp_seed_report

@@ -125,7 +125,10 @@ typedef struct {
   int64_t ntimed;       /* steps included in the stage times */
   int64_t nlm;          /* pairs that reached the LM (after every prefilter) */
   int64_t seed_fallbacks; /* network-seeded solves redone from the centre line */
-  int64_t reserved[2];
+  int64_t lm_waves;     /* wavefronts of the last contact-solve launch issued
+                           (a replayed graph keeps the width it was captured
+                           with; gp_params.lm_waves = 0 chooses it) */
+  int64_t reserved[1];
 } gp_stats;
 
 /* lifetime ----------------------------------------------------------- */

@@ -2,6 +2,7 @@
 among them (gazpart_amd/autotune.py)."""
 import numpy as np
 import pytest
+import torch
 
 import gazpart_amd as G
 from gazpart_amd import workload as W
@@ -25,6 +26,10 @@ def test_contact_launch_width_changes_no_result(dense, prec):
         for _ in range(5):
             ctx.step(1)
             n += ctx.stats()["ncollision"]
+        # the width used: as asked, or (0) one wavefront per SIMD while the
+        # solves (tens here) leave lanes idle
+        nsimd = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+        assert ctx.stats()["lm_waves"] == (waves if waves > 0 else nsimd)
         out.append(ctx.get_particles())
         ncoll.append(n)
         ctx.close()
