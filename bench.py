@@ -97,6 +97,21 @@ def run_gpu(wl, precision, steps, warmup, G, groups=None, theta=None, **kw):
         ncoll += s["ncollision"]
     tctx.sync()
     tst = tctx.stats()
+    # a caller that keeps the particles in host memory and hands them over
+    # every step: upload, one step, download (pageable numpy arrays)
+    st["host_handover_ms_per_step"] = None
+    if groups is None:   # ensembles: the groups are set once, not handed over
+        cur = tctx.get_particles()
+        tctx.step(1)
+        tctx.sync()
+        npc = 3
+        p0 = time.perf_counter()
+        for _ in range(npc):
+            tctx.set_particles(cur, wl.species_id)
+            tctx.step(1)
+            cur = tctx.get_particles()
+        p1 = time.perf_counter()
+        st["host_handover_ms_per_step"] = 1e3 * (p1 - p0) / npc
     tctx.close()
     for k in ("t_advance_ms", "t_sort_ms", "t_contact_ms", "t_greedy_ms", "ntimed"):
         st[k] = tst[k]
@@ -509,6 +524,11 @@ def main():
         "speedup_mixed_over_fp64": round(m["updates_per_s"] / f["updates_per_s"], 3),
         "stage_ms": stage(m["stats"]),
         "per_step": m["stats"]["per_step"],
+        # not the headline: particles uploaded from and downloaded to host
+        # memory around every step (PCIe-inclusive), one step per call
+        "host_handover": None if m["stats"]["host_handover_ms_per_step"] is None else {
+            "ms_per_step": round(m["stats"]["host_handover_ms_per_step"], 3),
+            "updates_per_s": round(wl.np_ * 1e3 / m["stats"]["host_handover_ms_per_step"], 1)},
         "collisions_per_step": m["stats"]["ncollision"],
         "candidate_pairs": m["stats"]["nclose"],
         "roofline": roof["mixed"],
