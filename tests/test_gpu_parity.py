@@ -293,6 +293,29 @@ def test_step_equals_adv_plus_collision_and_is_deterministic(c1):
     np.testing.assert_array_equal(sb, sc)
 
 
+@pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
+def test_host_handover_between_steps_changes_no_result(c1, prec):
+    """The Fortran drop-in copies the particles out to the reference's host
+    arrays and back in between steps (GPU_GET_PART / GPU_PUT_PART): the
+    library carries no per-particle state across steps beyond the 13
+    components of the record, so handing them over changes nothing -- the
+    dense ensemble and C1, in eager and in graph mode."""
+    d = np.load(GOLD + "golden_trajectory.npz")
+    for wl in (golden_workload(d), c1):
+        for graph in (False, True):
+            a = G.Context.from_workload(wl, precision=prec, graph=graph)
+            b = G.Context.from_workload(wl, precision=prec, graph=graph)
+            a.step(6)
+            b.step(3)
+            b.set_particles(b.get_particles(), wl.species_id)
+            b.step(1)
+            b.set_particles(b.get_particles(), wl.species_id)
+            b.step(2)
+            np.testing.assert_array_equal(a.get_particles(), b.get_particles())
+            a.close()
+            b.close()
+
+
 @pytest.mark.parametrize("overlap", [0, 1])
 @pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
 def test_graph_replays_equal_eager_steps(c1, prec, overlap):
