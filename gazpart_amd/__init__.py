@@ -51,9 +51,33 @@ EXPORTS = (
     "gp_field_file_read", "gp_particle_stats", "gp_set_collision_stats", "gp_get_collision_stats",
     "gp_partfluid_stats", "gp_bind_field_device", "gp_bind_field_planar", "gp_set_dtime", "gp_dns_create", "gp_dns_destroy",
     "gp_dns_error_string", "gp_dns_set_velocity", "gp_dns_get_velocity", "gp_dns_get_vorticity",
-    "gp_dns_step", "gp_dns_cfl", "gp_dns_info", "gp_dns_bind",
+    "gp_dns_step", "gp_dns_cfl", "gp_dns_info", "gp_dns_bind", "gp_get_contacts",
+    "gp_seed_net_gate",
 )
 SEED_NET_NPARAM = 5252
+
+
+def seed_net_init(seed=0):
+    """Initial parameters of the contact-seed network (12 -> 64 -> 64 -> 4,
+    the flat layout gp_seed_net_set takes: W1, b1, W2, b2, W3, b3
+    row-major): weights uniform in +-sqrt(6 / fan_in) (He), biases zero,
+    from a Philox stream of the given seed."""
+    g = np.random.Generator(np.random.Philox(int(seed)))
+    parts = []
+    for fan_out, fan_in in ((64, 12), (64, 64), (4, 64)):
+        lim = np.sqrt(6.0 / fan_in)
+        parts.append(g.uniform(-lim, lim, fan_out * fan_in))
+        parts.append(np.zeros(fan_out))
+    theta = np.concatenate(parts).astype(np.float32)
+    assert theta.size == SEED_NET_NPARAM
+    return theta
+
+
+def train_seed_net(ctx, npairs=2_000_000, seed=2024, epochs=3, init_seed=7):
+    """Initialise and train the seed network of a context on npairs
+    solver-labelled pairs (gp_seed_net_fit); returns the fit report."""
+    ctx.seed_net_set(seed_net_init(init_seed))
+    return ctx.seed_net_fit(npairs, seed, epochs=epochs, batch=4096, lr=2e-3)
 
 
 class GpError(RuntimeError):
@@ -157,6 +181,8 @@ def load_library(path: str = LIB_PATH):
     L.gp_seed_net_fit.argtypes = [vp, C.c_int64, C.c_uint64, C.c_int, C.c_int, C.c_float,
                                   C.POINTER(SeedReport)]
     L.gp_seed_net_eval.argtypes = [vp, C.c_int64, C.c_uint64, C.POINTER(SeedReport)]
+    L.gp_seed_net_gate.argtypes = [vp, C.c_int, C.c_double]
+    L.gp_get_contacts.argtypes = [vp, C.c_int64, ip, ip, vp, C.POINTER(C.c_int64)]
     _lib = L
     return L
 
@@ -383,7 +409,26 @@ class Context:
                                             _ptr(e2), _ptr(cn), _ptr(nfev)))
         return flag, e1, e2, cn, nfev
 
+    def contacts(self, nmax=None):
+        """The approaching contacts of the last collision pass in the order
+        COLLISION_INTERACTION's loop meets them: (HOME, NEAR, resolved),
+        1-based; resolved marks the pairs the one-collision pass resolved."""
+        n = C.c_int64(0)
+        if nmax is None:
+            self._chk(self.lib.gp_get_contacts(self.h, 0, None, None, None, C.byref(n)))
+            nmax = n.value
+        h = np.zeros(max(nmax, 1), dtype=np.int32)
+        nr = np.zeros(max(nmax, 1), dtype=np.int32)
+        r = np.zeros(max(nmax, 1), dtype=np.uint8)
+        self._chk(self.lib.gp_get_contacts(self.h, nmax, _ptr(h), _ptr(nr), _ptr(r), C.byref(n)))
+        m = min(n.value, nmax)
+        return h[:m], nr[:m], r[:m].astype(bool)
+
     # -- contact-seed network (no reference counterpart) -----------------
+    def seed_net_gate(self, maxfev=6, keep_angle=0.02):
+        """Acceptance gate of the network-seeded solve (gp_seed_net_gate)."""
+        self._chk(self.lib.gp_seed_net_gate(self.h, int(maxfev), float(keep_angle)))
+
     def seed_net_set(self, theta):
         t = np.ascontiguousarray(theta, dtype=np.float32)
         assert t.size == SEED_NET_NPARAM

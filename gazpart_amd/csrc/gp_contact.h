@@ -396,12 +396,13 @@ __device__ __forceinline__ void lmpar2(double r[6], const int ipvt[2], const dou
 // lmder with lmder1's settings (minpack_reduc.f90:243-640, :732-745):
 // mode 1, factor 100, maxfev 300, ftol = xtol = sqrt(eps), gtol = 0.
 // Returns the number of function evaluations; *info_out = MINPACK's info
-// (1 converged, 4 zero gradient, 5 maxfev or tolerances too small);
-// cur = the Jet of the returned x.
-__device__ int lm_solve(const ContactPair& cp, double x[2], int* info_out, Jet& cur) {
+// (1 converged, 4 zero gradient, 5 maxfev, 6 tolerances too small for a
+// further reduction: MINPACK's 6-8); cur = the Jet of the returned x.
+// maxfev below 300 only for a network-seeded first attempt.
+__device__ int lm_solve(const ContactPair& cp, double x[2], int* info_out, Jet& cur,
+                        int maxfev = 300) {
   const double tol = 1.4901161193847656e-08;
   const double p1 = 0.1, p5 = 0.5, p25 = 0.25, p75 = 0.75, p0001 = 1.0e-4;
-  const int maxfev = 300;
   double fvec[3], fjac[6], diag[2], qtf[2], wa1[2], wa2[2], wa3[2], wa4[3];
   int ipvt[2];
   int info = 0, nfev = 1, iter = 1;
@@ -493,9 +494,10 @@ __device__ int lm_solve(const ContactPair& cp, double x[2], int* info_out, Jet& 
       }
       bool small = fabs(actred) <= tol && prered <= tol && p5 * ratio <= 1.0;
       if (small || delta <= tol * xnorm) { info = 1; break; }
-      if (nfev >= maxfev || (fabs(actred) <= EPSMCH && prered <= EPSMCH && p5 * ratio <= 1.0) ||
+      if (nfev >= maxfev) { info = 5; break; }
+      if ((fabs(actred) <= EPSMCH && prered <= EPSMCH && p5 * ratio <= 1.0) ||
           delta <= EPSMCH * xnorm || gnorm <= EPSMCH) {
-        info = 5;
+        info = 6;
         break;
       }
       if (ratio >= p0001) break;
@@ -517,7 +519,7 @@ struct ContactOut {
 __device__ __forceinline__ ContactOut contact_detect(const double bi[3], const double qi[4],
                                                      const Species& si, const double bj[3],
                                                      const double qj[4], const Species& sj,
-                                                     const double* seed) {
+                                                     const double* seed, int maxfev = 300) {
   ContactPair cp;
   cp.E1 = ell_matrix(qi, si.a, si.b, si.c);
   cp.E2 = ell_matrix(qj, sj.a, sj.b, sj.c);
@@ -534,7 +536,7 @@ __device__ __forceinline__ ContactOut contact_detect(const double bi[3], const d
   }
   ContactOut o;
   Jet s;
-  o.nfev = lm_solve(cp, x, &o.info, s);
+  o.nfev = lm_solve(cp, x, &o.info, s, maxfev);
   double cv[3];
   support_points(cp, s, cv, o.e1, o.e2);
   double d0 = o.e1[0] - o.e2[0], d1 = o.e1[1] - o.e2[1], d2 = o.e1[2] - o.e2[2];

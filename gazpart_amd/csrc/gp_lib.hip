@@ -928,7 +928,7 @@ __global__ __launch_bounds__(LM_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) v
     const unsigned long long* nwork, unsigned long long cap, double L,
     ActiveRec* __restrict__ act, unsigned long long* nact, unsigned long long acap,
     unsigned long long* nfev_sum, unsigned long long* ncontact, unsigned* err,
-    const double* __restrict__ seeds, unsigned long long* nfallback,
+    const double* __restrict__ seeds, SeedGate gate, unsigned long long* nfallback,
     const CStat* __restrict__ csp) {
   const unsigned long long nw = min(*nwork, cap);
   const unsigned long long waves = gridDim.x;
@@ -947,17 +947,14 @@ __global__ __launch_bounds__(LM_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) v
       const double qi[4] = {p.qa[r.i], p.qb[r.i], p.qc[r.i], p.qd[r.i]};
       const double qj[4] = {p.qa[r.j], p.qb[r.j], p.qc[r.j], p.qd[r.j]};
       const Species si = spec[p.sp[r.i]], sj = spec[p.sp[r.j]];
-      // network seed when given; a seeded solve that ends without
-      // converging (maxfev / tolerances) is redone from the centre line
+      // network seed when given, kept only through the gate (else the
+      // centre-line solve decides)
       const double* sd = (seeds && seeds[2 * w] == seeds[2 * w]) ? seeds + 2 * w : nullptr;
-      ContactOut o;
-      for (int pass = 0; pass < 2; ++pass) {
-        o = contact_detect(bi, qi, si, bj, qj, sj, sd);
-        nfev += (unsigned long long)o.nfev;
-        if (!(sd && o.info == 5)) break;
-        sd = nullptr;
-        ++nfb;
-      }
+      int nf = 0;
+      bool fb = false;
+      const ContactOut o = contact_detect_gated(bi, qi, si, bj, qj, sj, sd, gate, nf, fb);
+      nfev += (unsigned long long)nf;
+      nfb += fb ? 1ull : 0ull;
       if (o.flag) {
         // velocities of the deepest points (collision_interaction.f90:269-324)
         double r1[3] = {o.e1[0] - bi[0], o.e1[1] - bi[1], o.e1[2] - bi[2]};
@@ -1082,7 +1079,7 @@ struct SeedEvalOut {
   float* cn_dot;
 };
 __global__ __launch_bounds__(64) void k_mlp_eval(unsigned long long seed, long long n, MlpDev W,
-                                                 SeedEvalOut out) {
+                                                 SeedGate gate, SeedEvalOut out) {
   __shared__ __attribute__((aligned(16))) float sF[64][16];
   __shared__ float sY[64][4];
   for (long long k0 = (long long)blockIdx.x * 64; k0 < n; k0 += (long long)gridDim.x * 64) {
@@ -1101,17 +1098,11 @@ __global__ __launch_bounds__(64) void k_mlp_eval(unsigned long long seed, long l
     if (!valid) continue;
     const Species s1 = pose_species(pp.a1), s2 = pose_species(pp.a2);
     const ContactOut r = contact_detect(pp.b1, pp.q1, s1, pp.b2, pp.q2, s2, nullptr);
-    const double* sd = ok ? x : nullptr;
-    ContactOut o;
     int nf = 0;
-    unsigned char fb = 0;
-    for (int pass = 0; pass < 2; ++pass) {
-      o = contact_detect(pp.b1, pp.q1, s1, pp.b2, pp.q2, s2, sd);
-      nf += o.nfev;
-      if (!(sd && o.info == 5)) break;
-      sd = nullptr;
-      fb = 1;
-    }
+    bool fbb = false;
+    const ContactOut o =
+        contact_detect_gated(pp.b1, pp.q1, s1, pp.b2, pp.q2, s2, ok ? x : nullptr, gate, nf, fbb);
+    const unsigned char fb = fbb ? 1 : 0;
     out.nfev_ref[k] = r.nfev;
     out.nfev_net[k] = nf;
     out.flag_ref[k] = (unsigned char)r.flag;
@@ -1386,7 +1377,8 @@ template <typename VT>
 __global__ __launch_bounds__(BLOCK) void k_contact_batch(
     PState<VT> p, const Species* __restrict__ spec, const int* __restrict__ home,
     const int* __restrict__ nr, const int* __restrict__ pos_of_id, long long npairs, double L,
-    int np, unsigned char* flag, double* e1, double* e2, double* cn, int* nfev, unsigned* err) {
+    int np, unsigned char* flag, double* e1, double* e2, double* cn, int* nfev, unsigned* err,
+    const double* __restrict__ seeds, SeedGate gate, unsigned long long* nfallback) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= npairs) return;
   const int hi = home[t] - 1, ni = nr[t] - 1;
@@ -1399,14 +1391,59 @@ __global__ __launch_bounds__(BLOCK) void k_contact_batch(
   min_image(p, i, j, L, bi, bj);
   const double qi[4] = {p.qa[i], p.qb[i], p.qc[i], p.qd[i]};
   const double qj[4] = {p.qa[j], p.qb[j], p.qc[j], p.qd[j]};
-  ContactOut o = contact_detect(bi, qi, spec[p.sp[i]], bj, qj, spec[p.sp[j]], nullptr);
+  // the step's own solve: the network seed through its gate when given
+  const double* sd = (seeds && seeds[2 * t] == seeds[2 * t]) ? seeds + 2 * t : nullptr;
+  int nf = 0;
+  bool fb = false;
+  const ContactOut o =
+      contact_detect_gated(bi, qi, spec[p.sp[i]], bj, qj, spec[p.sp[j]], sd, gate, nf, fb);
+  if (fb) atomicAdd(nfallback, 1ull);
   flag[t] = (unsigned char)o.flag;
   for (int c = 0; c < 3; ++c) {
     e1[t * 3 + c] = o.e1[c];
     e2[t * 3 + c] = o.e2[c];
     cn[t * 3 + c] = o.cn[c];
   }
-  nfev[t] = o.nfev;
+  nfev[t] = nf;
+}
+
+// the work list of a batch of 1-based (HOME, NEAR) pairs (slots of the
+// resident particles), for the network's seeds
+__global__ __launch_bounds__(BLOCK) void k_work_of_pairs(const int* __restrict__ home,
+                                                         const int* __restrict__ nr,
+                                                         const int* __restrict__ pos_of_id,
+                                                         long long npairs, int np,
+                                                         WorkRec* __restrict__ work,
+                                                         unsigned* err) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs) return;
+  const int hi = home[t] - 1, ni = nr[t] - 1;
+  WorkRec r;
+  r.k = t;
+  if (hi < 0 || hi >= np || ni < 0 || ni >= np) {
+    atomicOr(err, EB_OUT_OF_GRID);
+    r.i = r.j = 0;
+  } else {
+    r.i = pos_of_id[hi];
+    r.j = pos_of_id[ni];
+  }
+  work[t] = r;
+}
+
+// the approaching contacts of the last collision pass: list key, 1-based
+// reference indices of both particles, resolved by the one-collision pass
+template <typename VT>
+__global__ __launch_bounds__(BLOCK) void k_contacts_out(PState<VT> p,
+                                                        const ActiveRec* __restrict__ act,
+                                                        const unsigned char* __restrict__ status,
+                                                        long long na, long long* key, int* home,
+                                                        int* nr, unsigned char* resolved) {
+  const long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= na) return;
+  key[a] = act[a].k;
+  home[a] = p.id[act[a].i] + 1;
+  nr[a] = p.id[act[a].j] + 1;
+  resolved[a] = status[a] == 1 ? 1 : 0;
 }
 
 template <typename VT>
@@ -1519,6 +1556,12 @@ struct gp_ctx {
   int mlp_step = 0;
   bool mlp_ready = false;
   double* seeds = nullptr;      // 2 per LM-list entry
+  // acceptance gate of a network-seeded solve (gp_seed_net_gate)
+  SeedGate gate{SEED_GATE_MAXFEV, 0.0};
+  // the approaching-contact list (act, act_status) belongs to the particles
+  // of buffer cur until the next advance (gp_get_contacts)
+  bool contacts_valid = false;
+  int64_t batch_fallbacks = 0;  // gated seeded solves redone in the last gp_contact_batch
   // collision grid
   CellGrid g{};
   int* cell = nullptr;
@@ -1993,6 +2036,7 @@ int gp_create(const gp_params* p, gp_ctx** out) {
   gp_ctx* ctx = new gp_ctx();
   ctx->prm = *p;
   ctx->mixed = p->precision == GP_PREC_MIXED;
+  ctx->gate.cos_keep = std::cos(SEED_GATE_KEEP);
   hipGetDevice(&ctx->device);
   {
     int ncu = 0;
@@ -2201,6 +2245,7 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   if (!ctx || np < 1 || !state || !species) return GP_ERR_ARG;
   hipStreamSynchronize(ctx->st);
   drop_graph(ctx);
+  ctx->contacts_valid = false;
   if (!ctx->spec) {
     ctx->err = "gp_set_species must precede gp_set_particles";
     return GP_ERR_STATE;
@@ -2503,6 +2548,7 @@ static int launch_advance(gp_ctx* ctx, bool bin) {
   HIPCHK(hipGetLastError());
   mark(ctx, 1);
   ctx->cur = 1 - ctx->cur;
+  ctx->contacts_valid = false;
   ctx->perm_identity = true;
   ctx->cells_valid = bin;
   return GP_OK;
@@ -2712,7 +2758,7 @@ static int launch_contacts_solve(gp_ctx* ctx) {
   ctx->lm_waves_last = ctx->lm_waves;   // a captured graph keeps this width
   hipLaunchKernelGGL(k_contact<VT>, dim3(ctx->lm_waves), dim3(LM_BLOCK), 0, ctx->st, ps, ctx->spec,
                      ctx->lmwork, c + 7, ctx->work_cap, ctx->prm.L, ctx->act, c + 2, ctx->act_cap,
-                     c + 3, c + 4, ctx->d_err, net ? ctx->seeds : nullptr, c + 9,
+                     c + 3, c + 4, ctx->d_err, net ? ctx->seeds : nullptr, ctx->gate, c + 9,
                      cs.rec ? ctx->cs_dev : nullptr);
   mark(ctx, 3);
   hipLaunchKernelGGL(k_greedy_mark, dim3(GREEDY_GRID), dim3(BLOCK), 0, ctx->st, ctx->act, c + 2,
@@ -2759,7 +2805,9 @@ static int launch_collide(gp_ctx* ctx) {
   // prefilter survivors <= candidates
   if ((rc = ensure_lists(ctx, (unsigned long long)std::max<long long>(ctx->last_nclose, 1))))
     return rc;
-  return launch_contacts<VT>(ctx);
+  if ((rc = launch_contacts<VT>(ctx))) return rc;
+  ctx->contacts_valid = true;
+  return GP_OK;
 }
 
 // k full steps without host synchronisation (lists already sized).  With
@@ -2887,6 +2935,7 @@ static int graph_steps(gp_ctx* ctx, int nsteps) {
   if (nsteps > 0) {
     if ((rc = launch_steps<VT>(ctx, nsteps))) return rc;
   }
+  ctx->contacts_valid = false;   // pipelined steps reorder the particles
   return GP_OK;
 }
 
@@ -2986,30 +3035,114 @@ int gp_contact_batch(gp_ctx* ctx, int64_t npairs, const int* home, const int* ne
   unsigned char* df = (unsigned char*)(dnf + n);
   HIPCHK(hipMemcpyAsync(dh, home, n * sizeof(int), hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipMemcpyAsync(dn, near_, n * sizeof(int), hipMemcpyHostToDevice, ctx->st));
-  if (ctx->mixed) {
-    auto ps = pstate<float>(ctx, ctx->cur);
-    hipLaunchKernelGGL(k_pos_of_id<float>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st, ps,
-                       ctx->pos_of_id, ctx->np);
-    hipLaunchKernelGGL(k_contact_batch<float>, dim3(nblk(npairs)), dim3(BLOCK), 0, ctx->st, ps,
-                       ctx->spec, dh, dn, ctx->pos_of_id, (long long)npairs, ctx->prm.L, ctx->np,
-                       df, de1, de2, dcn, dnf, ctx->d_err);
-  } else {
-    auto ps = pstate<double>(ctx, ctx->cur);
-    hipLaunchKernelGGL(k_pos_of_id<double>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st, ps,
-                       ctx->pos_of_id, ctx->np);
-    hipLaunchKernelGGL(k_contact_batch<double>, dim3(nblk(npairs)), dim3(BLOCK), 0, ctx->st, ps,
-                       ctx->spec, dh, dn, ctx->pos_of_id, (long long)npairs, ctx->prm.L, ctx->np,
-                       df, de1, de2, dcn, dnf, ctx->d_err);
+  // with lm_seed = 1 the pairs are solved as the step solves them: network
+  // seeds for the whole batch, then the gated seeded solve
+  const bool net = ctx->prm.lm_seed == 1 && ctx->mlp_ready;
+  double* dseed = nullptr;
+  WorkRec* dwork = nullptr;
+  unsigned long long* dnw = nullptr;
+  if (net) {
+    const size_t sb = n * (2 * sizeof(double) + sizeof(WorkRec)) + 64;
+    HIPCHK(hipMalloc((void**)&dseed, sb));
+    dwork = (WorkRec*)(dseed + 2 * n);
+    dnw = (unsigned long long*)(dwork + n);
+    HIPCHK(hipMemcpyAsync(dnw, &n, sizeof(unsigned long long), hipMemcpyHostToDevice, ctx->st));
   }
+#define GP_BATCH(VT)                                                                              \
+  {                                                                                               \
+    auto ps = pstate<VT>(ctx, ctx->cur);                                                          \
+    hipLaunchKernelGGL(k_pos_of_id<VT>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st, ps,         \
+                       ctx->pos_of_id, ctx->np);                                                  \
+    if (net) {                                                                                    \
+      hipLaunchKernelGGL(k_work_of_pairs, dim3(nblk(npairs)), dim3(BLOCK), 0, ctx->st, dh, dn,    \
+                         ctx->pos_of_id, (long long)npairs, ctx->np, dwork, ctx->d_err);          \
+      hipLaunchKernelGGL(k_mlp_seed<VT>, dim3(ctx->nsimd), dim3(64), 0, ctx->st, ps, ctx->spec,   \
+                         dwork, dnw, (unsigned long long)n, ctx->prm.L, mlp_dev(ctx), dseed);     \
+    }                                                                                             \
+    hipLaunchKernelGGL(k_contact_batch<VT>, dim3(nblk(npairs)), dim3(BLOCK), 0, ctx->st, ps,      \
+                       ctx->spec, dh, dn, ctx->pos_of_id, (long long)npairs, ctx->prm.L, ctx->np, \
+                       df, de1, de2, dcn, dnf, ctx->d_err, dseed, ctx->gate, ctx->d_cnt + 9);     \
+  }
+  if (net) hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st,
+                              ctx->d_cnt + 9, 0ull, 1ll);
+  if (ctx->mixed) GP_BATCH(float) else GP_BATCH(double)
+#undef GP_BATCH
   HIPCHK(hipGetLastError());
   if (flag) HIPCHK(hipMemcpyAsync(flag, df, n, hipMemcpyDeviceToHost, ctx->st));
   if (ept1) HIPCHK(hipMemcpyAsync(ept1, de1, 3 * n * 8, hipMemcpyDeviceToHost, ctx->st));
   if (ept2) HIPCHK(hipMemcpyAsync(ept2, de2, 3 * n * 8, hipMemcpyDeviceToHost, ctx->st));
   if (cnormal) HIPCHK(hipMemcpyAsync(cnormal, dcn, 3 * n * 8, hipMemcpyDeviceToHost, ctx->st));
   if (nfev) HIPCHK(hipMemcpyAsync(nfev, dnf, n * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  if (net)
+    HIPCHK(hipMemcpyAsync(ctx->h_cnt + 9, ctx->d_cnt + 9, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   hipFree(d);
+  if (dseed) hipFree(dseed);
+  if (net) ctx->batch_fallbacks = (int64_t)ctx->h_cnt[9];
   return check_err(ctx);
+}
+
+int gp_get_contacts(gp_ctx* ctx, int64_t nmax, int* home, int* near_, unsigned char* resolved,
+                    int64_t* n) {
+  if (!ctx || !n || nmax < 0 || (nmax > 0 && (!home || !near_))) return GP_ERR_ARG;
+  *n = 0;
+  if (!ctx->contacts_valid || !ctx->act) {
+    ctx->err = "no contact list: call after gp_collision or an eager gp_step, before the next "
+               "advance";
+    return GP_ERR_STATE;
+  }
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt + 2, ctx->d_cnt + 2, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  const unsigned long long na = std::min<unsigned long long>(ctx->h_cnt[2], ctx->act_cap);
+  *n = (int64_t)na;
+  if (na == 0 || nmax == 0) return GP_OK;
+  const size_t m = (size_t)na;
+  long long* dk = nullptr;
+  HIPCHK(hipMalloc((void**)&dk, m * (sizeof(long long) + 2 * sizeof(int) + 1)));
+  int* dh = (int*)(dk + m);
+  int* dn = dh + m;
+  unsigned char* dr = (unsigned char*)(dn + m);
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_contacts_out<float>, dim3(nblk((long long)m)), dim3(BLOCK), 0, ctx->st,
+                       pstate<float>(ctx, ctx->cur), ctx->act, ctx->act_status, (long long)m, dk,
+                       dh, dn, dr);
+  else
+    hipLaunchKernelGGL(k_contacts_out<double>, dim3(nblk((long long)m)), dim3(BLOCK), 0, ctx->st,
+                       pstate<double>(ctx, ctx->cur), ctx->act, ctx->act_status, (long long)m, dk,
+                       dh, dn, dr);
+  std::vector<long long> k(m);
+  std::vector<int> hh(m), nn(m);
+  std::vector<unsigned char> rr(m);
+  HIPCHK(hipMemcpyAsync(k.data(), dk, m * sizeof(long long), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipMemcpyAsync(hh.data(), dh, m * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipMemcpyAsync(nn.data(), dn, m * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipMemcpyAsync(rr.data(), dr, m, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  hipFree(dk);
+  // candidate-list order: the list keys order the pairs as the reference's
+  // loop meets them
+  std::vector<size_t> ord(m);
+  for (size_t a = 0; a < m; ++a) ord[a] = a;
+  std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return k[a] < k[b]; });
+  const size_t out = std::min<size_t>(m, (size_t)nmax);
+  for (size_t a = 0; a < out; ++a) {
+    home[a] = hh[ord[a]];
+    near_[a] = nn[ord[a]];
+    if (resolved) resolved[a] = rr[ord[a]];
+  }
+  return check_err(ctx);
+}
+
+int gp_seed_net_gate(gp_ctx* ctx, int maxfev, double keep_angle) {
+  if (!ctx || maxfev < 1 || maxfev > 300 || !(keep_angle >= 0.0) || !(keep_angle <= M_PI))
+    return GP_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  drop_graph(ctx);
+  ctx->gate.maxfev = maxfev;
+  ctx->gate.cos_keep = std::cos(keep_angle);
+  return GP_OK;
 }
 
 int gp_collision(gp_ctx* ctx, int ncycle, double time) {
@@ -3616,7 +3749,7 @@ int gp_seed_net_eval(gp_ctx* ctx, int64_t npairs, uint64_t seed, gp_seed_report*
     SeedEvalOut o{nr, nn, fr, fn, fb, cd};
     hipLaunchKernelGGL(k_mlp_eval, dim3((unsigned)std::min<long long>((npairs + 63) / 64, 16384)),
                        dim3(64), 0, ctx->st, (unsigned long long)seed, (long long)npairs,
-                       mlp_dev(ctx), o);
+                       mlp_dev(ctx), ctx->gate, o);
     // the copies below run on the null stream: wait for the context's stream
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->st));

@@ -162,6 +162,45 @@ __device__ __forceinline__ bool seed_from_output(const double q1[4], float y0, f
   return true;
 }
 
+// The network-seeded solve and its acceptance gate.  Started from the
+// network's normal, the solve can end in another local minimum of its
+// residual than from the reference's centre-line guess, and with it take
+// another contact decision.  So the seeded attempt gets at most g.maxfev
+// evaluations, and its result is kept only when it converged (MINPACK info
+// other than 5) with its final normal within the angle acos(g.cos_keep) of
+// the seed's: the network had then predicted the minimum the solve found.
+// Otherwise the pair is solved again from the centre line with the full
+// budget of 300 evaluations, as the reference solves it.  nfev counts both
+// attempts; fallback reports a redone solve.
+struct SeedGate {
+  int maxfev;
+  double cos_keep;
+};
+constexpr int SEED_GATE_MAXFEV = 6;
+constexpr double SEED_GATE_KEEP = 0.02;   // radians
+
+__device__ __forceinline__ ContactOut contact_detect_gated(
+    const double bi[3], const double qi[4], const Species& si, const double bj[3],
+    const double qj[4], const Species& sj, const double* seed, const SeedGate& g, int& nfev,
+    bool& fallback) {
+  ContactOut o;
+  const double* sd = seed;
+  fallback = false;
+  for (int pass = 0; pass < 2; ++pass) {
+    o = contact_detect(bi, qi, si, bj, qj, sj, sd, sd ? g.maxfev : 300);
+    nfev += o.nfev;
+    if (!sd) break;
+    double s0, c0, s1, c1;
+    sincos(sd[0], &s0, &c0);
+    sincos(sd[1], &s1, &c1);
+    const double dot = c0 * c1 * o.cn[0] + s0 * c1 * o.cn[1] + s1 * o.cn[2];
+    if (o.info != 5 && dot >= g.cos_keep) break;
+    sd = nullptr;
+    fallback = true;
+  }
+  return o;
+}
+
 // ---------------------------------------------------------------------------
 // synthetic solver-labelled pairs: tri-axial shapes (semi-axis ratios
 // uniform in [1, 4], unit volume scale), uniform random orientations,

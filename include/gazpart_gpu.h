@@ -189,10 +189,25 @@ int gp_collision_detection(gp_ctx *ctx, int64_t nclosemax, int *home, int *near_
                            int64_t *nclose);
 /* ellipsoid_contact_detection (ellipsoid_contact_detection.f90:1-155) on
  * npairs (1-based) pairs of device-resident particles, with the minimum
- * image of collision_interaction.f90:221-237.  Any output may be NULL.  */
+ * image of collision_interaction.f90:221-237.  Any output may be NULL.
+ * With gp_params.lm_seed = 1 and a network set, each pair is solved as
+ * the step solves it (network seed through the gate of gp_seed_net_gate,
+ * nfev counting both attempts).  EPT1, EPT2 and cnormal are those of the
+ * solve's final iterate whatever the flag.                               */
 int gp_contact_batch(gp_ctx *ctx, int64_t npairs, const int *home, const int *near_,
                      unsigned char *flag, double *ept1, double *ept2, double *cnormal,
                      int *nfev);
+/* The approaching contacts (flag true and NVREL < ZERO) of the last
+ * collision pass, in the order COLLISION_INTERACTION's loop meets them
+ * (collision_interaction.f90:194-419): 1-based HOME and NEAR, and
+ * resolved = 1 where the one-collision pass resolved the pair (both
+ * particles still free when the loop reached it, :207, :348, :397-398).
+ * The reference prints these pairs (print_colliding_pair.f90) rather than
+ * returning them.  *n is the number of contacts, also when it exceeds
+ * nmax (then nmax are written).  Valid after gp_collision or an eager
+ * gp_step and until the next advance; GP_ERR_STATE otherwise.            */
+int gp_get_contacts(gp_ctx *ctx, int64_t nmax, int *home, int *near_, unsigned char *resolved,
+                    int64_t *n);
 /* COLLISION (collision.f90:7-201, called at main.f90:483) at NPROC = 1. */
 int gp_collision(gp_ctx *ctx, int ncycle, double time);
 /* nsteps x (ADV_PARTICLE_ELLP + COLLISION), asynchronous on the context's
@@ -312,6 +327,13 @@ int gp_seed_net_fit(gp_ctx *ctx, int64_t npairs, uint64_t seed, int epochs, int 
                     float lr, gp_seed_report *rep);
 /* solve fresh pairs from both seeds: evaluation counts, flag agreement */
 int gp_seed_net_eval(gp_ctx *ctx, int64_t npairs, uint64_t seed, gp_seed_report *rep);
+/* Acceptance gate of a network-seeded solve (no reference counterpart):
+ * the seeded attempt runs at most maxfev evaluations (1..300) and is kept
+ * only if it converged with its final normal within keep_angle radians of
+ * the seed's; otherwise the pair is solved from the centre line as the
+ * reference does (ellipsoid_contact_detection.f90:77-87, maxfev 300).
+ * Defaults: 6 evaluations, 0.02 rad.                                    */
+int gp_seed_net_gate(gp_ctx *ctx, int maxfev, double keep_angle);
 
 /* on-disk formats ------------------------------------------------------
  * Host only (no context, no GPU): the files either side of the step, so

@@ -15,15 +15,15 @@ Tolerances
                  oracle's (a restatement of ellipsoid_contact_detection.f90)
                  to 1e-6 for 98 % of pairs (the solves may end on either
                  side of a tolerance test).
-  gain         : after on-GPU training on 1M solver-labelled pairs the
-                 network-seeded solve needs fewer function evaluations on
-                 fresh pairs (>= 3 % fewer overall) while its contact flags
-                 agree with the centre-line solve on >= 98 % of them.  The
-                 reference's solve converges to different local minima from
-                 different seeds (its centre-line seed sits at a maximum of
-                 the gap for spheres), so a seed trades evaluations against
-                 agreement; the network seeds only pairs it rates as contacts
-                 with probability > 0.9."""
+  gain         : after on-GPU training on 1M solver-labelled pairs, the
+                 seeded solve without its gate needs >= 3 % fewer function
+                 evaluations on fresh pairs but changes ~1 % of the contact
+                 flags (the reference's solve converges to different local
+                 minima from different seeds); through the gate (seeded
+                 attempt of at most 6 evaluations kept only when it
+                 converged within 0.02 rad of the seed, else the centre-line
+                 solve) every flag is the centre-line one.
+"""
 import numpy as np
 import pytest
 
@@ -110,12 +110,24 @@ def test_labelled_pairs_match_oracles(ctx):
     assert nf.min() >= 1 and nf.max() <= 300
 
 
-def test_trained_network_cuts_solver_evaluations(ctx):
-    ctx.seed_net_set(M.init_params(5))
+def test_trained_network_gain_and_gate(ctx):
+    """Without the gate (keep angle pi, full budget) the seeded solve needs
+    fewer evaluations (>= 3 % fewer) but changes ~1 % of the contact flags
+    (other local minima of the residual); through the default gate every
+    flag is the centre-line solve's on these 50k fresh pairs, and the
+    rejected seeded attempts cost evaluations (reported, bounded by the
+    gate's 6-evaluation budget per seeded pair)."""
+    ctx.seed_net_set(G.seed_net_init(5))
     fit = ctx.seed_net_fit(1_000_000, 21, epochs=6, batch=4096, lr=2e-3)
+    ctx.seed_net_gate(300, np.pi)
     ev = ctx.seed_net_eval(50_000, 99)
-    print("fit", fit, "eval", ev)
+    ctx.seed_net_gate()
+    gated = ctx.seed_net_eval(50_000, 99)
+    print("fit", fit, "ungated", ev, "gated", gated)
     assert ev["flags_ref"] <= ev["npairs"] and ev["flags_net"] <= ev["npairs"]
     assert ev["nfev_net_mean"] <= 0.97 * ev["nfev_ref_mean"], ev
-    assert ev["flag_agree"] >= 0.98 * ev["npairs"], ev
-    assert ev["fallbacks"] <= 0.01 * ev["npairs"], ev
+    assert 0.98 * ev["npairs"] <= ev["flag_agree"] < ev["npairs"], ev
+    assert gated["flag_agree"] == gated["npairs"], gated
+    seeded = gated["npairs"] - gated["no_seed"]
+    assert gated["fallbacks"] <= seeded
+    assert gated["nfev_net_mean"] <= gated["nfev_ref_mean"] + 6.0 * seeded / gated["npairs"]
