@@ -52,7 +52,7 @@ EXPORTS = (
     "gp_partfluid_stats", "gp_bind_field_device", "gp_bind_field_planar", "gp_set_dtime", "gp_dns_create", "gp_dns_destroy",
     "gp_dns_error_string", "gp_dns_set_velocity", "gp_dns_get_velocity", "gp_dns_get_vorticity",
     "gp_dns_step", "gp_dns_cfl", "gp_dns_info", "gp_dns_bind", "gp_get_contacts",
-    "gp_seed_net_gate",
+    "gp_seed_net_gate", "gp_collision_advance",
 )
 SEED_NET_NPARAM = 5252
 
@@ -105,7 +105,7 @@ class Stats(C.Structure):
                 ("t_sort_ms", C.c_double), ("t_contact_ms", C.c_double),
                 ("t_greedy_ms", C.c_double), ("ntimed", C.c_int64), ("nlm", C.c_int64),
                 ("seed_fallbacks", C.c_int64), ("lm_waves", C.c_int64),
-                ("reserved", C.c_int64 * 1)]
+                ("t_solve_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -156,6 +156,7 @@ def load_library(path: str = LIB_PATH):
     L.gp_contact_batch.argtypes = [vp, C.c_int64, ip, ip, vp, dp, dp, dp, ip]
     L.gp_collision.argtypes = [vp, C.c_int, C.c_double]
     L.gp_step.argtypes = [vp, C.c_int]
+    L.gp_collision_advance.argtypes = [vp, C.c_int, C.c_double, C.c_int]
     L.gp_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.gp_collision_grid.argtypes = [vp, ip, dp]
     L.gp_reset_timers.argtypes = [vp]
@@ -347,6 +348,12 @@ class Context:
     def collision(self, ncycle=1, time=0.0):
         """COLLISION (collision.f90:7) at NPROC = 1."""
         self._chk(self.lib.gp_collision(self.h, ncycle, time))
+
+    def collision_advance(self, ncycle=1, time=0.0, ncycle_next=None):
+        """COLLISION of cycle ncycle then ADV_PARTICLE_ELLP of the next, in
+        one asynchronous call (gp_collision_advance)."""
+        nxt = ncycle + 1 if ncycle_next is None else ncycle_next
+        self._chk(self.lib.gp_collision_advance(self.h, ncycle, time, nxt))
 
     def collision_counts(self, reset=False):
         """Collisions each particle took part in (reference order) since

@@ -1561,6 +1561,12 @@ struct gp_ctx {
   // the approaching-contact list (act, act_status) belongs to the particles
   // of buffer cur until the next advance (gp_get_contacts)
   bool contacts_valid = false;
+  // graph-mode gp_step: the lists were sized with headroom by an eager step
+  // (steps may then run without host synchronisation)
+  bool lists_sized = false;
+  // the last collision pass was a pipelined gp_collision_advance: its solve
+  // flags (inlm) are set for the particles of plist only
+  bool pipe_prev = false;
   int64_t batch_fallbacks = 0;  // gated seeded solves redone in the last gp_contact_batch
   // collision grid
   CellGrid g{};
@@ -1596,9 +1602,10 @@ struct gp_ctx {
   // stage timing: ev[0] before the advance, ev[1] after it, ev[2] after the
   // pair-count scan, ev[3] after the LM, ev[4] after the greedy pass
   bool timing = false;
-  hipEvent_t ev[5] = {};
+  hipEvent_t ev[6] = {};   // ev[5]: before the contact-solve kernel
   bool ev_pending = false;
   double t_stage[4] = {0, 0, 0, 0};
+  double t_solve = 0.0;     // the contact-solve kernel alone (ev[5] -> ev[3])
   long long ntimed = 0;
   // captured two-step graph (gp_params.graph) and the buffer configuration
   // it was captured in
@@ -2246,6 +2253,8 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   hipStreamSynchronize(ctx->st);
   drop_graph(ctx);
   ctx->contacts_valid = false;
+  ctx->lists_sized = false;
+  ctx->pipe_prev = false;
   if (!ctx->spec) {
     ctx->err = "gp_set_species must precede gp_set_particles";
     return GP_ERR_STATE;
@@ -2466,6 +2475,8 @@ static void collect_timing(gp_ctx* ctx) {
     if (hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]) == hipSuccess)
       ctx->t_stage[k] += ms;
   }
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[3]) == hipSuccess) ctx->t_solve += ms;
   ctx->ntimed++;
   ctx->ev_pending = false;
 }
@@ -2751,6 +2762,7 @@ static int launch_contacts_solve(gp_ctx* ctx) {
   // one wavefront per workgroup: the LM is latency-bound (one pair per lane),
   // so the pairs are spread over as many CUs as possible
   const bool net = ctx->prm.lm_seed == 1;
+  mark(ctx, 5);
   if (net)
     hipLaunchKernelGGL(k_mlp_seed<VT>, dim3(ctx->lm_waves), dim3(64), 0, ctx->st, ps, ctx->spec,
                        ctx->lmwork, c + 7, ctx->work_cap, ctx->prm.L, mlp_dev(ctx), ctx->seeds);
@@ -2807,6 +2819,41 @@ static int launch_collide(gp_ctx* ctx) {
     return rc;
   if ((rc = launch_contacts<VT>(ctx))) return rc;
   ctx->contacts_valid = true;
+  ctx->pipe_prev = false;
+  return GP_OK;
+}
+
+// COLLISION of the advanced particles, with the next ADV_PARTICLE_ELLP of
+// every particle outside the contact solves on the second stream beside the
+// solves (as inside pipelined gp_step), then the solves' particles advanced
+// after the one-collision pass.  Lists sized (no host synchronisation).
+template <typename VT>
+static int launch_collide_advance(gp_ctx* ctx) {
+  int rc;
+  if (!ctx->pipe_prev)
+    hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st, ctx->inlm, 0,
+                       (long long)ctx->np);
+  // the sort clears the previous pipelined pass's flags
+  ctx->clear_plist = ctx->pipe_prev;
+  rc = launch_sort<VT>(ctx, false);
+  ctx->clear_plist = false;
+  if (rc) return rc;
+  if ((rc = launch_contacts_filter<VT>(ctx, true, true))) return rc;
+  const int* order = ctx->use_bricks ? ctx->trav : ctx->sorted;
+  HIPCHK(hipEventRecord(ctx->ev_fork, ctx->st));
+  HIPCHK(hipStreamWaitEvent(ctx->st2, ctx->ev_fork, 0));
+  launch_advance_kernel<VT>(ctx, ctx->st2, order, ctx->inlm, true);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev_join, ctx->st2));
+  if ((rc = launch_contacts_solve<VT>(ctx))) return rc;
+  launch_advance_listed<VT>(ctx, ctx->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamWaitEvent(ctx->st, ctx->ev_join, 0));
+  ctx->cur = 1 - ctx->cur;
+  ctx->perm_identity = true;
+  ctx->cells_valid = true;
+  ctx->contacts_valid = false;
+  ctx->pipe_prev = true;
   return GP_OK;
 }
 
@@ -2821,6 +2868,7 @@ static int launch_steps(gp_ctx* ctx, int k) {
   int rc;
   const bool pipe = ctx->prm.overlap && !ctx->timing;
   unsigned long long* c = ctx->d_cnt;
+  ctx->pipe_prev = false;
   if (pipe)
     hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st, ctx->inlm, 0,
                        (long long)ctx->np);
@@ -2912,8 +2960,17 @@ static void choose_lm_waves(gp_ctx* ctx, unsigned long long nlm) {
 template <typename VT>
 static int graph_steps(gp_ctx* ctx, int nsteps) {
   int rc;
+  const int unit = graph_unit(ctx);
   const bool same = ctx->gexec && ctx->g_cur == ctx->cur && ctx->g_perm == ctx->perm &&
                     ctx->g_perm_identity == ctx->perm_identity;
+  // fewer steps than a captured unit (a caller stepping once per call, as
+  // the reference's main loop does): launched directly, without capture and
+  // without host synchronisation, once the lists are sized
+  if (nsteps < unit && ctx->lists_sized) {
+    if ((rc = launch_steps<VT>(ctx, nsteps))) return rc;
+    ctx->contacts_valid = false;
+    return GP_OK;
+  }
   if (!same) {
     // one eager step sizes the lists; headroom for the candidate count to grow
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, ctx->d_cnt, 0ull,
@@ -2929,8 +2986,8 @@ static int graph_steps(gp_ctx* ctx, int nsteps) {
     HIPCHK(hipStreamSynchronize(ctx->st));
     choose_lm_waves(ctx, ctx->h_cnt[7]);
     if ((rc = capture_graph<VT>(ctx))) return rc;
+    ctx->lists_sized = true;
   }
-  const int unit = graph_unit(ctx);
   for (; nsteps >= unit; nsteps -= unit) HIPCHK(hipGraphLaunch(ctx->gexec, ctx->st));
   if (nsteps > 0) {
     if ((rc = launch_steps<VT>(ctx, nsteps))) return rc;
@@ -3162,6 +3219,37 @@ int gp_collision(gp_ctx* ctx, int ncycle, double time) {
   return rc;
 }
 
+int gp_collision_advance(gp_ctx* ctx, int ncycle, double time, int ncycle_next) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (ctx->cs_fout2 > 0 || ctx->timing) {
+    // statistics passes and stage timers: one after the other, eagerly
+    if ((rc = gp_collision(ctx, ncycle, time))) return rc;
+    return gp_adv_particle_ellp(ctx, ncycle_next);
+  }
+  if (!ctx->lists_sized) {
+    // an eager pass sizes the lists with headroom; later calls then run
+    // without host synchronisation
+    rc = ctx->mixed ? launch_collide<float>(ctx) : launch_collide<double>(ctx);
+    if (rc) return rc;
+    const unsigned long long need = 2ull * (unsigned long long)ctx->last_nclose + 4096ull;
+    if ((rc = ensure_lists(ctx, need))) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    choose_lm_waves(ctx, ctx->h_cnt[7]);
+    accumulate_stats(ctx);
+    ctx->lists_sized = true;
+    if ((rc = check_err(ctx))) return rc;
+    return gp_adv_particle_ellp(ctx, ncycle_next);
+  }
+  rc = ctx->mixed ? launch_collide_advance<float>(ctx) : launch_collide_advance<double>(ctx);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, ctx->st));
+  return GP_OK;
+}
+
 int gp_set_dtime(gp_ctx* ctx, double dtime) {
   if (!ctx || !(dtime > 0.0)) return GP_ERR_ARG;
   HIPCHK(hipStreamSynchronize(ctx->st));
@@ -3269,6 +3357,7 @@ int gp_get_stats(gp_ctx* ctx, gp_stats* out) {
   ctx->stats.t_sort_ms = ctx->t_stage[1];
   ctx->stats.t_contact_ms = ctx->t_stage[2];
   ctx->stats.t_greedy_ms = ctx->t_stage[3];
+  ctx->stats.t_solve_ms = ctx->t_solve;
   ctx->stats.ntimed = ctx->ntimed;
   *out = ctx->stats;
   return GP_OK;
@@ -3279,6 +3368,7 @@ int gp_reset_timers(gp_ctx* ctx) {
   HIPCHK(hipStreamSynchronize(ctx->st));
   collect_timing(ctx);
   for (double& t : ctx->t_stage) t = 0.0;
+  ctx->t_solve = 0.0;
   ctx->ntimed = 0;
   return GP_OK;
 }

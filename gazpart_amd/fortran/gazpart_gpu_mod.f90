@@ -21,7 +21,7 @@ module gazpart_gpu
     integer(c_int64_t) :: greedy_rounds
     real(c_double) :: t_advance_ms, t_sort_ms, t_contact_ms, t_greedy_ms
     integer(c_int64_t) :: ntimed, nlm, seed_fallbacks, lm_waves
-    integer(c_int64_t) :: reserved(1)
+    real(c_double) :: t_solve_ms
   end type gp_stats
 
   type, bind(C) :: gp_seed_report
@@ -138,6 +138,19 @@ module gazpart_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function gp_collision_advance(ctx, ncycle, time, ncycle_next) &
+        bind(C, name='gp_collision_advance')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ncycle
+      real(c_double), value :: time
+      integer(c_int), value :: ncycle_next
+    end function
+    integer(c_int) function gp_set_dtime(ctx, dtime) bind(C, name='gp_set_dtime')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dtime
     end function
     integer(c_int) function gp_get_stats(ctx, st) bind(C, name='gp_get_stats')
       import :: c_int, c_ptr, gp_stats

@@ -16,11 +16,26 @@
 !! PART is copied to the device before and back after every call, so every
 !! other routine of the reference sees the usual PART; with .false. the
 !! device copy is authoritative and GPU_SYNC_PART refreshes PART on demand
-!! (before statistics or output).  A non-zero status stops the run with
+!! (before statistics or output).  In that resident mode the collision
+!! pass of a colliding class is deferred to the next GPU_ADV_PARTICLE_ELLP,
+!! which runs it and the next advance in one call without host
+!! synchronisation, the advance of the particles outside the pass's contact
+!! solves beside the solves (gp_collision_advance); GPU_SYNC_PART runs a
+!! deferred collision pass first, so every routine that reads PART after
+!! GPU_COLLISION sees it after the pass.  A non-zero status stops the run with
 !! the library's message, as the reference's own `stop`s do.  With
 !! LEVEL1_STPAR (at GPU_INIT), GPU_COLLISION also writes the level-1
 !! collision statistics and adds them to MEAN_TIME_PART_COL as
 !! COLLISION_INTERACTION does, from sums formed on the device.
+!!
+!! When the fluid evolves (SOLVE_FLUID > 0: ADV_FLUID and CHECK_CFL change
+!! the field and DTIME every cycle, main.f90:496-542), GPU_ADV_PARTICLE_ELLP
+!! copies UFLU..VORTICITY_Z and DTIME to every context before it advances;
+!! the spectral supplier on the GPU (gp_dns_bind) avoids that host copy.
+!!
+!! GPU_INTERPH can be called before GPU_INIT (INITIATION_PARTICLE_VELOCITY
+!! does so, main.f90:286, before COLLISION_INITIATION): it uses a context
+!! of its own that holds no particles and no field.
 module gp_dropin
   use iso_c_binding
   use gazpart_gpu
@@ -33,9 +48,14 @@ module gp_dropin
   implicit none
 
   type(c_ptr), allocatable, save :: GPU_CTX(:)
+  type(c_ptr), save :: GPU_FIELD_CTX = c_null_ptr
   logical, save :: GPU_MIRROR = .true.
   integer(c_int), save :: GPU_PRECISION = GP_PREC_FP64
   logical, save :: GPU_DEVICE_AHEAD = .false.
+  logical, allocatable, save :: GPU_COLL_PENDING(:)
+  integer, allocatable, save :: GPU_COLL_NCYCLE(:)
+  real(kind=8), allocatable, save :: GPU_COLL_TIME(:)
+  real(kind=8), save :: GPU_DTIME_BOUND = -1.0
 
 contains
 
@@ -58,7 +78,10 @@ contains
     real(c_double), allocatable :: rows(:,:), fld(:,:,:,:)
 
     if (allocated(GPU_CTX)) call GPU_FINALIZE
-    allocate(GPU_CTX(NIG))
+    allocate(GPU_CTX(NIG), GPU_COLL_PENDING(NIG), GPU_COLL_NCYCLE(NIG), GPU_COLL_TIME(NIG))
+    GPU_COLL_PENDING = .false.
+    GPU_COLL_NCYCLE = 0
+    GPU_COLL_TIME = 0.0
     allocate(fld(NX, -1:NY+2, -1:NZ+2, 6))
     fld(:,:,:,1) = UFLU; fld(:,:,:,2) = VFLU; fld(:,:,:,3) = WFLU
     fld(:,:,:,4) = VORTICITY_X; fld(:,:,:,5) = VORTICITY_Y; fld(:,:,:,6) = VORTICITY_Z
@@ -122,17 +145,53 @@ contains
       call GPU_PUT_PART(J)
     end do
     deallocate(fld)
+    GPU_DTIME_BOUND = DTIME
     GPU_DEVICE_AHEAD = .false.
   end subroutine GPU_INIT
+
+  !! the evolving field and time step into every context (SOLVE_FLUID > 0)
+  subroutine GPU_REBIND_FIELD
+    integer :: J
+    integer(c_int) :: rc
+    real(c_double), allocatable :: fld(:,:,:,:)
+    allocate(fld(NX, -1:NY+2, -1:NZ+2, 6))
+    fld(:,:,:,1) = UFLU; fld(:,:,:,2) = VFLU; fld(:,:,:,3) = WFLU
+    fld(:,:,:,4) = VORTICITY_X; fld(:,:,:,5) = VORTICITY_Y; fld(:,:,:,6) = VORTICITY_Z
+    do J = 1, size(GPU_CTX)
+      rc = gp_bind_field(GPU_CTX(J), fld)
+      call GPU_CHECK(rc, GPU_CTX(J), 'gp_bind_field')
+      if (DTIME /= GPU_DTIME_BOUND) then
+        rc = gp_set_dtime(GPU_CTX(J), real(DTIME, c_double))
+        call GPU_CHECK(rc, GPU_CTX(J), 'gp_set_dtime')
+      end if
+    end do
+    GPU_DTIME_BOUND = DTIME
+    deallocate(fld)
+  end subroutine GPU_REBIND_FIELD
+
+  !! a deferred collision pass of class J, run now
+  subroutine GPU_FLUSH_COLLISION(J)
+    integer, intent(in) :: J
+    integer(c_int) :: rc
+    if (.not. GPU_COLL_PENDING(J)) return
+    rc = gp_collision(GPU_CTX(J), int(GPU_COLL_NCYCLE(J), c_int), &
+                      real(GPU_COLL_TIME(J), c_double))
+    call GPU_CHECK(rc, GPU_CTX(J), 'COLLISION')
+    GPU_COLL_PENDING(J) = .false.
+  end subroutine GPU_FLUSH_COLLISION
 
   subroutine GPU_FINALIZE
     integer :: J
     integer(c_int) :: rc
+    if (c_associated(GPU_FIELD_CTX)) then
+      rc = gp_destroy(GPU_FIELD_CTX)
+      GPU_FIELD_CTX = c_null_ptr
+    end if
     if (.not. allocated(GPU_CTX)) return
     do J = 1, size(GPU_CTX)
       if (c_associated(GPU_CTX(J))) rc = gp_destroy(GPU_CTX(J))
     end do
-    deallocate(GPU_CTX)
+    deallocate(GPU_CTX, GPU_COLL_PENDING, GPU_COLL_NCYCLE, GPU_COLL_TIME)
   end subroutine GPU_FINALIZE
 
   !! PART(1:NPART_LOC(J), J) -> device
@@ -166,6 +225,7 @@ contains
     real(c_double), allocatable :: st(:,:)
     np = NPART_LOC(J)
     if (np < 1) return
+    call GPU_FLUSH_COLLISION(J)
     allocate(st(np, 13))
     rc = gp_get_particles(GPU_CTX(J), np, st)
     call GPU_CHECK(rc, GPU_CTX(J), 'gp_get_particles')
@@ -194,8 +254,19 @@ contains
     integer, intent(in) :: NCYCLE
     integer :: J
     integer(c_int) :: rc
+    if (.not. allocated(GPU_CTX)) stop 'GPU_ADV_PARTICLE_ELLP: call GPU_INIT first'
+    if (SOLVE_FLUID > 0) call GPU_REBIND_FIELD
     do J = 1, NIG
       if (NPART_LOC(J) < 1) cycle
+      if (GPU_COLL_PENDING(J)) then
+        ! resident mode: the previous cycle's collision pass and this
+        ! advance in one call
+        rc = gp_collision_advance(GPU_CTX(J), int(GPU_COLL_NCYCLE(J), c_int), &
+                                  real(GPU_COLL_TIME(J), c_double), int(NCYCLE, c_int))
+        call GPU_CHECK(rc, GPU_CTX(J), 'COLLISION + ADV_PARTICLE_ELLP')
+        GPU_COLL_PENDING(J) = .false.
+        cycle
+      end if
       if (GPU_MIRROR) call GPU_PUT_PART(J)
       rc = gp_adv_particle_ellp(GPU_CTX(J), int(NCYCLE, c_int))
       call GPU_CHECK(rc, GPU_CTX(J), 'ADV_PARTICLE_ELLP')
@@ -210,8 +281,18 @@ contains
     real(kind=8), intent(in) :: TIME
     integer :: J
     integer(c_int) :: rc
+    if (.not. allocated(GPU_CTX)) stop 'GPU_COLLISION: call GPU_INIT first'
     do J = 1, NIG
       if (COLLIDEF(J) /= 1 .or. NPART_LOC(J) < 1) cycle
+      call GPU_FLUSH_COLLISION(J)
+      ! resident mode: run with the next advance (its statistics passes,
+      ! which write lines now, stay immediate)
+      if (.not. GPU_MIRROR .and. .not. (LEVEL1_STPAR .and. FOUT2 > 0)) then
+        GPU_COLL_PENDING(J) = .true.
+        GPU_COLL_NCYCLE(J) = NCYCLE
+        GPU_COLL_TIME(J) = TIME
+        cycle
+      end if
       if (GPU_MIRROR) call GPU_PUT_PART(J)
       rc = gp_collision(GPU_CTX(J), int(NCYCLE, c_int), real(TIME, c_double))
       call GPU_CHECK(rc, GPU_CTX(J), 'COLLISION')
@@ -266,13 +347,41 @@ contains
     real(kind=8), dimension(NINT), intent(in) :: XINT, YINT, ZINT
     real(kind=8), dimension(NINT), intent(out) :: UINT
     integer(c_int) :: rc
+    type(gp_params) :: p
     if (INT_SCHEME /= 3 .and. INT_SCHEME /= 4) then
       stop 'GPU_INTERPH: only INT_SCHEME 3 and 4 run on the GPU'
     end if
     if (XAIN(ISTART(1)+1) - XAIN(ISTART(1)) /= DX) stop 'GPU_INTERPH: mesh is not XMESH'
-    rc = gp_interph_field(GPU_CTX(1), int(INT_SCHEME, c_int), UAIN, int(NINT, c_int), &
+    if (.not. c_associated(GPU_FIELD_CTX)) then
+      ! a context for field-argument interpolation only (grid and scheme;
+      ! the physics entries are unused but must be valid)
+      p%n = NX
+      p%interp_scheme = int(INT_SCHEME, c_int)
+      p%precision = GP_PREC_FP64
+      p%renorm_quat = 0
+      p%L = LXMAX
+      p%visc = merge(VISC, 1.0d0, VISC > 0.0)
+      p%rhof = RHOF
+      p%dtime = merge(DTIME, 1.0d0, DTIME > 0.0)
+      p%ecw = ECW
+      p%muw = MUW
+      p%dpmax = 0.0
+      p%prefilter = 2
+      p%xcd_remap = 1
+      p%keep_fluid = 0
+      p%timing = 0
+      p%graph = 0
+      p%brick = 0
+      p%lm_seed = 0
+      p%lm_waves = 0
+      p%overlap = 0
+      p%reserved = 0
+      rc = gp_create(p, GPU_FIELD_CTX)
+      call GPU_CHECK(rc, GPU_FIELD_CTX, 'gp_create')
+    end if
+    rc = gp_interph_field(GPU_FIELD_CTX, int(INT_SCHEME, c_int), UAIN, int(NINT, c_int), &
                           XINT, YINT, ZINT, UINT)
-    call GPU_CHECK(rc, GPU_CTX(1), 'INTERPH')
+    call GPU_CHECK(rc, GPU_FIELD_CTX, 'INTERPH')
   end subroutine GPU_INTERPH
 
 end module gp_dropin

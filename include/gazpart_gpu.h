@@ -128,7 +128,8 @@ typedef struct {
   int64_t lm_waves;     /* wavefronts of the last contact-solve launch issued
                            (a replayed graph keeps the width it was captured
                            with; gp_params.lm_waves = 0 chooses it) */
-  int64_t reserved[1];
+  double t_solve_ms;    /* with timing: the contact-solve kernel alone (and
+                           the network's seeds), part of t_contact_ms */
 } gp_stats;
 
 /* lifetime ----------------------------------------------------------- */
@@ -213,6 +214,19 @@ int gp_collision(gp_ctx *ctx, int ncycle, double time);
 /* nsteps x (ADV_PARTICLE_ELLP + COLLISION), asynchronous on the context's
  * stream; errors surface at the next gp_sync / gp_get_particles.        */
 int gp_step(gp_ctx *ctx, int nsteps);
+/* COLLISION (collision.f90:7, called at main.f90:483) of cycle ncycle
+ * followed by ADV_PARTICLE_ELLP (adv_particle_new.f90:7, main.f90:466) of
+ * cycle ncycle_next, in one asynchronous call: the advance of every
+ * particle outside this pass's contact solves runs on a second stream
+ * beside the solves, the solves' particles are advanced after the
+ * one-collision pass.  Same results as gp_collision then
+ * gp_adv_particle_ellp.  For a caller stepping once per call (the Fortran
+ * drop-in's resident mode).  The first call after gp_set_particles sizes
+ * the pair lists with headroom and synchronises; later calls do not (an
+ * overflow surfaces as GP_ERR_NCLOSE_OVERFLOW at the next gp_sync).  With
+ * collision statistics or stage timers on, the two run one after the
+ * other.                                                                 */
+int gp_collision_advance(gp_ctx *ctx, int ncycle, double time, int ncycle_next);
 /* DTIME for the following steps (CHECK_CFL changes it every cycle when the
  * fluid evolves, check_cfl.f90:92); a captured step graph is re-captured */
 int gp_set_dtime(gp_ctx *ctx, double dtime);

@@ -18,13 +18,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "oracle", "_ref", "gp_dropin_test")
 
 
-def _write_input(path, d, nsteps, mirror, fout2=0):
+def _write_input(path, d, nsteps, mirror, fout2=0, interph=0, evolve=0):
     n, np_ = int(d["n"]), d["state0"].shape[1]
     emaj = np.atleast_1d(np.asarray(d["emaj"], dtype=np.float64))
     rhop = np.atleast_1d(np.asarray(d["rhop"], dtype=np.float64)) * np.ones(len(emaj))
     idp = d["idp"] if "idp" in d else np.ones(np_, dtype=np.int32)
     with open(path, "wb") as f:
-        np.array([n, np_, nsteps, int(d["scheme"]), mirror, len(emaj), fout2],
+        np.array([n, np_, nsteps, int(d["scheme"]), mirror, len(emaj), fout2, interph, evolve],
                  dtype=np.int32).tofile(f)
         np.array([float(d["L"]), float(d["visc"]), float(d["rhof"]), float(d["dtime"]),
                   float(d["apr"])]).tofile(f)
@@ -83,3 +83,44 @@ def test_fortran_dropin_collision_statistics(tmp_path):
         rows = np.loadtxt(tmp_path / f"fort.{unit}")
         assert rows.shape == (2 * npass, 9)
         np.testing.assert_allclose(rows[npass:], rows[:npass], rtol=2e-6, atol=1e-30)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran drop-in test driver not built")
+def test_fortran_interph_before_init(tmp_path):
+    """GPU_INTERPH where INITIATION_PARTICLE_VELOCITY calls INTERPH
+    (main.f90:286), before GPU_INIT: the six components at the particles
+    equal the reference's INTERPH to 1e-13 of their scale."""
+    d = np.load(os.path.join(ROOT, "tests", "golden", "golden_trajectory.npz"))
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    np_ = _write_input(fin, d, 1, 1, interph=1)
+    subprocess.run([EXE, str(fin), str(fout)], check=True, timeout=120)
+    out = np.fromfile(fout, dtype=np.float64)
+    iref = out[26 * np_:32 * np_].reshape(6, np_)
+    igpu = out[32 * np_:38 * np_].reshape(6, np_)
+    for c in range(6):
+        scale = np.abs(iref[c]).max()
+        assert scale > 0
+        assert np.abs(igpu[c] - iref[c]).max() <= 1e-13 * scale, c
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran drop-in test driver not built")
+@pytest.mark.parametrize("mirror", [1, 0])
+def test_fortran_dropin_evolving_field(tmp_path, mirror):
+    """SOLVE_FLUID = 1 with the field and DTIME changed half-way through (as
+    ADV_FLUID and CHECK_CFL change them every cycle): the drop-ins take the
+    new field and time step, in mirror and in resident mode, and match the
+    reference as in the frozen case."""
+    d = np.load(os.path.join(ROOT, "tests", "golden", "golden_trajectory.npz"))
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    np_ = _write_input(fin, d, 10, mirror, evolve=1)
+    subprocess.run([EXE, str(fin), str(fout)], check=True, timeout=120)
+    out = np.fromfile(fout, dtype=np.float64).reshape(2, 13, np_)
+    ref, gpu = out[0], out[1]
+    bad = _far(ref, gpu)
+    assert len(bad) <= 0.05 * np_, sorted(bad)
+    # the same run with a frozen field ends elsewhere: the change was seen
+    fin2, fout2 = tmp_path / "in2.bin", tmp_path / "out2.bin"
+    _write_input(fin2, d, 10, mirror)
+    subprocess.run([EXE, str(fin2), str(fout2)], check=True, timeout=120)
+    frozen = np.fromfile(fout2, dtype=np.float64).reshape(2, 13, np_)[0]
+    assert np.abs(frozen[0:3] - ref[0:3]).max() > 1e-6

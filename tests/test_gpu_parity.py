@@ -470,3 +470,33 @@ def test_cell_list_bitexact_short_and_long_scans(dpmax):
     assert err == 0 and nc > 0
     np.testing.assert_array_equal(home, ho)
     np.testing.assert_array_equal(near, no)
+
+
+@pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
+def test_collision_advance_equals_collision_then_advance(c1, prec):
+    """gp_collision_advance (COLLISION, then the next ADV_PARTICLE_ELLP beside
+    the contact solves, in one call) gives the bits of gp_collision followed
+    by gp_adv_particle_ellp: the dense reference ensemble and C1, with a
+    host handover and eager calls in between."""
+    d = np.load(GOLD + "golden_trajectory.npz")
+    for wl in (golden_workload(d), c1):
+        a = G.Context.from_workload(wl, precision=prec)
+        b = G.Context.from_workload(wl, precision=prec)
+        a.adv_particle_ellp(1)
+        b.adv_particle_ellp(1)
+        for s in range(1, 12):
+            a.collision(s)
+            a.adv_particle_ellp(s + 1)
+            b.collision_advance(s, 0.0, s + 1)
+            if s == 5:
+                np.testing.assert_array_equal(a.get_particles(), b.get_particles())
+            if s == 7:
+                b.set_particles(b.get_particles(), wl.species_id)
+                a.set_particles(a.get_particles(), wl.species_id)
+        a.collision(12)
+        b.collision(12)
+        np.testing.assert_array_equal(a.get_particles(), b.get_particles())
+        assert a.stats()["ncollision"] == b.stats()["ncollision"]
+        np.testing.assert_array_equal(a.collision_counts(), b.collision_counts())
+        a.close()
+        b.close()
