@@ -34,7 +34,7 @@ GP_OK = 0
 ERRORS = {
     1: "GP_ERR_NAN_DRAG", 2: "GP_ERR_NAN_OMEGA", 3: "GP_ERR_KE_COLLISION",
     4: "GP_ERR_KE_WALL", 5: "GP_ERR_OUT_OF_GRID", 6: "GP_ERR_NCLOSE_OVERFLOW",
-    7: "GP_ERR_INTERP_RANGE", 10: "GP_ERR_ARG", 11: "GP_ERR_STATE", 12: "GP_ERR_HIP",
+    7: "GP_ERR_INTERP_RANGE", 8: "GP_ERR_CELL_RULE", 10: "GP_ERR_ARG", 11: "GP_ERR_STATE", 12: "GP_ERR_HIP",
     13: "GP_ERR_NOMEM", 14: "GP_ERR_IO",
 }
 PREC_FP64, PREC_MIXED = 0, 1
@@ -94,7 +94,8 @@ class Params(C.Structure):
                 ("muw", C.c_double), ("dpmax", C.c_double), ("prefilter", C.c_int),
                 ("xcd_remap", C.c_int), ("keep_fluid", C.c_int), ("timing", C.c_int),
                 ("graph", C.c_int), ("brick", C.c_int), ("lm_seed", C.c_int),
-                ("lm_waves", C.c_int), ("overlap", C.c_int), ("reserved", C.c_int * 3)]
+                ("lm_waves", C.c_int), ("overlap", C.c_int), ("cell_rule", C.c_int),
+                ("reserved", C.c_int * 2)]
 
 
 class Stats(C.Structure):
@@ -199,7 +200,7 @@ class Context:
     def __init__(self, n, L, visc, rhof, dtime, *, scheme=3, precision=PREC_FP64, ecw=1.0,
                  muw=0.0, dpmax=0.0, prefilter=2, xcd_remap=1, renorm_quat=0,
                  keep_fluid=False, timing=False, graph=False, brick=0, lm_seed=0,
-                 lm_waves=0, overlap=1):
+                 lm_waves=0, overlap=1, cell_rule=0):
         self.lib = load_library()
         p = Params()
         p.n, p.interp_scheme, p.precision, p.renorm_quat = n, scheme, precision, renorm_quat
@@ -212,6 +213,7 @@ class Context:
         p.lm_seed = lm_seed
         p.lm_waves = lm_waves
         p.overlap = overlap
+        p.cell_rule = cell_rule
         self.params = p
         h = C.c_void_p()
         rc = self.lib.gp_create(C.byref(p), C.byref(h))

@@ -1515,6 +1515,7 @@ struct gp_ctx {
   Species* spec = nullptr;
   int nspec = 0;
   double max_emaj = 0.0;
+  double max_semi = 0.0;        // largest semi-axis of any species
   // particles
   int np = 0;
   void* pbuf[2] = {nullptr, nullptr};
@@ -1982,7 +1983,9 @@ static void grid_init(gp_ctx* ctx) {
   // COLLISION_INITIATION (collision_initiation.f90:50-102), INIT_GRID = 2,
   // NPROC = 1: DELTA_COLL = ZERO; the y/z range spans L + DY.
   const double L = ctx->prm.L, dy = L / (double)(ctx->prm.n - 1);
-  const double dpmax = ctx->prm.dpmax > 0.0 ? ctx->prm.dpmax : 2.0 * ctx->max_emaj;
+  // gp_params.cell_rule 1: from the largest semi-axis of any species
+  const double reach = ctx->prm.cell_rule == 1 ? ctx->max_semi : ctx->max_emaj;
+  const double dpmax = ctx->prm.dpmax > 0.0 ? ctx->prm.dpmax : 2.0 * reach;
   double dxbox = 2.0 * dpmax;
   int nx = (int)(L / dxbox);
   if (nx < 1) nx = 1;
@@ -2036,7 +2039,7 @@ int gp_create(const gp_params* p, gp_ctx** out) {
   *out = nullptr;
   if (p->n < 5 || (p->interp_scheme != 3 && p->interp_scheme != 4) ||
       (p->precision != GP_PREC_FP64 && p->precision != GP_PREC_MIXED) || !(p->L > 0.0) ||
-      !(p->dtime > 0.0) || !(p->visc > 0.0))
+      !(p->dtime > 0.0) || !(p->visc > 0.0) || p->cell_rule < 0 || p->cell_rule > 2)
     return GP_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return GP_ERR_HIP;
@@ -2229,8 +2232,23 @@ int gp_set_species(gp_ctx* ctx, int nspec, const double* rows) {
   HIPCHK(hipMemcpy(ctx->spec, h.data(), h.size() * sizeof(Species), hipMemcpyHostToDevice));
   ctx->nspec = nspec;
   ctx->max_emaj = emax;
+  ctx->max_semi = 0.0;
+  for (const Species& d : h) ctx->max_semi = std::max(ctx->max_semi, d.rmax);
   ctx->sp_rows.assign(rows, rows + (size_t)nspec * GP_SP_STRIDE);
   grid_init(ctx);
+  // two particles in contact are less than the sum of their largest
+  // semi-axes apart; the half shell of cells finds every pair closer than a
+  // cell width, so the reference rule is exact while no semi-axis exceeds
+  // half a cell (it fails for oblate species with APR < 0.5)
+  if (ctx->prm.cell_rule == 0 &&
+      ctx->max_semi > 0.5 * std::min(ctx->g.dx, std::min(ctx->g.dy, ctx->g.dz))) {
+    ctx->err = "largest semi-axis " + std::to_string(ctx->max_semi) +
+               " exceeds half the collision cell " +
+               std::to_string(0.5 * std::min(ctx->g.dx, std::min(ctx->g.dy, ctx->g.dz))) +
+               ": the reference cell rule would lose pairs (gp_params.cell_rule 1 sizes the "
+               "cells from the largest semi-axis, 2 keeps the reference rule unchecked)";
+    return GP_ERR_CELL_RULE;
+  }
   if (ctx->np > 0) {
     if ((rc = ensure_grid_buffers(ctx))) return rc;
     if (ctx->mixed)

@@ -56,6 +56,10 @@ module gp_dropin
   integer, allocatable, save :: GPU_COLL_NCYCLE(:)
   real(kind=8), allocatable, save :: GPU_COLL_TIME(:)
   real(kind=8), save :: GPU_DTIME_BOUND = -1.0
+  ! collision cell rule (gp_params.cell_rule): 0 the reference's, refused
+  ! when it would lose pairs; 2 the reference's unchecked; 1 from the
+  ! largest semi-axis
+  integer(c_int), save :: GPU_CELL_RULE = 0
 
 contains
 
@@ -106,6 +110,7 @@ contains
       p%lm_seed = 0
       p%lm_waves = 0
       p%overlap = 0
+      p%cell_rule = GPU_CELL_RULE
       p%reserved = 0
       GPU_CTX(J) = c_null_ptr
       rc = gp_create(p, GPU_CTX(J))
@@ -375,6 +380,7 @@ contains
       p%lm_seed = 0
       p%lm_waves = 0
       p%overlap = 0
+      p%cell_rule = 2
       p%reserved = 0
       rc = gp_create(p, GPU_FIELD_CTX)
       call GPU_CHECK(rc, GPU_FIELD_CTX, 'gp_create')

@@ -41,6 +41,11 @@ enum {
   GP_ERR_OUT_OF_GRID = 5,     /* collision_detection_connectivity.f90:81-89 */
   GP_ERR_NCLOSE_OVERFLOW = 6, /* NCLOSE > NCLOSEMAX (collision.f90:35-37) */
   GP_ERR_INTERP_RANGE = 7,    /* stencil outside UAIN (interp_lag3.f90:124) */
+  GP_ERR_CELL_RULE = 8,       /* gp_params.cell_rule = 0 and a species' largest
+                                 semi-axis exceeds half a collision cell: the
+                                 reference rule DPMAX = 2 max(EMAJ)
+                                 (collision_initiation.f90:50-75) would lose
+                                 candidate pairs (oblate APR < 0.5) */
   GP_ERR_ARG = 10,
   GP_ERR_STATE = 11,
   GP_ERR_HIP = 12,
@@ -106,7 +111,17 @@ typedef struct {
                           a second stream beside the solves (8 steps per
                           captured graph); 0: steps run one after another.
                           No effect on results */
-  int reserved[3];
+  int cell_rule;       /* collision cell size when dpmax <= 0.  0: the
+                          reference rule DPMAX = 2 max(EMAJ), DXBOX = 2 DPMAX
+                          (collision_initiation.f90:50-75), refused with
+                          GP_ERR_CELL_RULE when a species' largest semi-axis
+                          exceeds DXBOX / 2 (pairs would be lost); 1: DPMAX
+                          = 2 x the largest semi-axis of any species (every
+                          pair found; lists differ from the reference's when
+                          an oblate semi-axis is the largest); 2: the
+                          reference rule unchecked (the reference's lists,
+                          lost pairs included) */
+  int reserved[2];
 } gp_params;
 
 typedef struct {

@@ -500,3 +500,57 @@ def test_collision_advance_equals_collision_then_advance(c1, prec):
         np.testing.assert_array_equal(a.collision_counts(), b.collision_counts())
         a.close()
         b.close()
+
+
+def test_cell_rule_guard_for_oblate_species():
+    """All-oblate ensemble (APR 0.3: b = c = 3.3 a): the reference rule
+    DPMAX = 2 max(EMAJ) (collision_initiation.f90:50-75) makes cells of
+    4 max(EMAJ), narrower than two largest semi-axes, and loses pairs.
+    cell_rule 0 refuses it (GP_ERR_CELL_RULE); 2 keeps the reference's lists
+    bit for bit (oracle on the same grid); 1 sizes the cells from the
+    largest semi-axis, matches the oracle on that grid and contains every
+    pair whose centres are closer than the sum of the largest semi-axes
+    (brute force), which the reference rule misses some of."""
+    wl = W.build_workload("oblate", 48, 3000, k0=3.0, seed=21, apr=0.3, emaj_dx=0.6, st=1.0)
+    with pytest.raises(G.GpError) as e:
+        G.Context.from_workload(wl)
+    assert e.value.code == 8
+    st = wl.state
+    p = O.params(wl)
+    sp = wl.species[0]
+    rmax = sp[[W.SP_A, W.SP_B, W.SP_C]].max()
+    # brute force: pairs that could touch
+    d = st[0:3, :, None] - st[0:3, None, :]
+    d = np.where(np.abs(d) > wl.L / 2, d - np.copysign(wl.L, d), d)
+    close = np.linalg.norm(d, axis=0) < 2.0 * rmax
+    iu = np.triu_indices(wl.np_, 1)
+    need = {(int(a) + 1, int(b) + 1) for a, b in zip(*iu) if close[a, b]}
+    assert len(need) > 20
+    found = {}
+    for rule, dpmax in ((2, 2.0 * sp[W.SP_EMAJ]), (1, 2.0 * rmax)):
+        ctx = G.Context.from_workload(wl, cell_rule=rule)
+        h, n = ctx.collision_detection_connectivity()
+        g = O.collision_grid(p, dpmax)
+        con, ncon = O.connectivity(g)
+        ho, no, nc, err = O.detect(g, con, ncon, st[0], st[1], st[2], 10 ** 7)
+        assert err == 0
+        np.testing.assert_array_equal(h, ho)
+        np.testing.assert_array_equal(n, no)
+        found[rule] = {(min(a, b), max(a, b)) for a, b in zip(h.tolist(), n.tolist())}
+        ctx.close()
+    assert need <= found[1]
+    assert not need <= found[2]
+
+
+@pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
+def test_cell_rule_changes_nothing_when_safe(c1, prec):
+    """Prolate species (the largest semi-axis is EMAJ): cell_rule 0, 1 and 2
+    build the same grid and give the same steps bit for bit."""
+    out = []
+    for rule in (0, 1, 2):
+        ctx = G.Context.from_workload(c1, precision=prec, cell_rule=rule)
+        ctx.step(4)
+        out.append(ctx.get_particles())
+        ctx.close()
+    np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[0], out[2])
