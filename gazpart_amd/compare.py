@@ -10,10 +10,27 @@ compared.  A particle becomes history-divergent when its collision differs
 between the runs, or when it later collides with such a particle in either
 run.  The first divergent collision of every such chain (both particles
 still undivergent) is listed with the decision quantities of each run on
-its own state (gazpart_amd.contacts): depth, both margins, NVREL and the
-pair's closeness to the threshold.  Per-quantity errors are reported
-separately for particles that never collided, and for those that collided
-with the same history in both runs.
+its own state (gazpart_amd.contacts): depth, both margins, NVREL, the
+pair's closeness to the threshold, the angle between the two runs' contact
+normals and the relative difference of the pair's inputs, and is put in
+one of three classes:
+
+  threshold  the contact decision (flag and NVREL < ZERO) differs and, in
+             one run or the other, a test sits within MARGIN of its
+             switching point;
+  minimum    the contact decision differs and the two solves ended at
+             different minima of their residual (normals more than
+             MIN_ANGLE apart): the reference's solve is discontinuous in
+             its inputs there, and the FP64 form started from inputs 1e-12
+             apart shows such flips too;
+  order      the contact decision is the same in both runs; the pair was
+             resolved in one run only because the one-collision pass had
+             given one of its particles to another pair first.
+
+Anything else (decisions that differ at the same minimum, far from every
+threshold) is classed "other", which a correct pair of runs never shows.
+Per-quantity errors are reported separately for particles that never
+collided, and for those that collided with the same history in both runs.
 """
 from __future__ import annotations
 
@@ -22,6 +39,8 @@ import numpy as np
 from . import contacts as CT
 
 QUANT = (("x", slice(0, 3)), ("u", slice(3, 6)), ("q", slice(6, 10)), ("omega", slice(10, 13)))
+MARGIN = 1e-3       # closeness within which a threshold decision may differ
+MIN_ANGLE = 1e-3    # radians between normals: different minima of the solve
 
 
 def _relerr(a, b, sl, cols):
@@ -54,6 +73,7 @@ class Paired:
         flag, e1, e2, cn, _ = ctx.ellipsoid_contact_detection(home, near)
         qd = CT.decision_quantities(mid, self.species, self.species_id, self.L, home, near,
                                     e1, e2, cn)
+        qd["cn"] = cn
         return flag.astype(np.int32), qd
 
     def step(self):
@@ -79,15 +99,34 @@ class Paired:
             fa, qa = self._quantities(self.a, mids[0], h, n)
             fb, qb = self._quantities(self.b, mids[1], h, n)
             close = np.minimum(CT.closeness(qa), CT.closeness(qb))
+            ca, cb = CT.contact_of(fa, qa), CT.contact_of(fb, qb)
+            dot = np.clip(np.abs((qa["cn"] * qb["cn"]).sum(axis=1)), 0.0, 1.0)
+            angle = np.arccos(dot)
+            ia, ib = h - 1, n - 1
+            din = np.zeros(len(h))
+            for sl in (slice(0, 3), slice(3, 6), slice(6, 10), slice(10, 13)):
+                for idx in (ia, ib):
+                    da = np.abs(mids[0][sl][:, idx] - mids[1][sl][:, idx]).max(axis=0)
+                    sc = np.maximum(np.abs(mids[1][sl][:, idx]).max(axis=0), 1e-300)
+                    din = np.maximum(din, da / sc)
             for k, (p, q) in enumerate(new_roots):
+                if ca[k] == cb[k]:
+                    kind = "order"
+                elif close[k] <= MARGIN:
+                    kind = "threshold"
+                elif angle[k] > MIN_ANGLE:
+                    kind = "minimum"
+                else:
+                    kind = "other"
                 self.roots.append(dict(
-                    step=s, home=p, near=q, resolved=((p, q) in ga, (p, q) in gb),
+                    step=s, home=p, near=q, kind=kind, resolved=((p, q) in ga, (p, q) in gb),
                     flag=(int(fa[k]), int(fb[k])),
                     depth=(float(qa["depth"][k]), float(qb["depth"][k])),
                     fm1=(float(qa["fm1"][k]), float(qb["fm1"][k])),
                     fm2=(float(qa["fm2"][k]), float(qb["fm2"][k])),
                     nvrel=(float(qa["nvrel"][k]), float(qb["nvrel"][k])),
-                    closeness=float(close[k])))
+                    closeness=float(close[k]), normal_angle=float(angle[k]),
+                    input_diff=float(din[k])))
         newly = np.zeros_like(self.tainted)
         for (p, q) in diff:
             newly[p - 1] = newly[q - 1] = True
@@ -108,5 +147,8 @@ class Paired:
             "n_never_collided": int(free.sum()), "n_collided_same_history": int(same.sum()),
             "n_history_divergent": int(self.tainted.sum()),
             "first_divergent_collisions": self.roots,
-            "max_closeness": max((r["closeness"] for r in self.roots), default=0.0),
+            "kinds": {k: sum(r["kind"] == k for r in self.roots)
+                      for k in ("threshold", "minimum", "order", "other")},
+            "max_closeness_threshold_kind": max(
+                (r["closeness"] for r in self.roots if r["kind"] == "threshold"), default=0.0),
         }

@@ -502,6 +502,17 @@ def test_collision_advance_equals_collision_then_advance(c1, prec):
         b.close()
 
 
+def _ctx_rule(wl, rule, **kw):
+    """a context whose collision cells come from its cell rule (no DPMAX
+    given)"""
+    ctx = G.Context(wl.n, wl.L, wl.visc, wl.rhof, wl.dtime, scheme=wl.scheme, dpmax=0.0,
+                    cell_rule=rule, **kw)
+    ctx.bind_field(wl.field)
+    ctx.set_species(wl.species)
+    ctx.set_particles(wl.state, wl.species_id)
+    return ctx
+
+
 def test_cell_rule_guard_for_oblate_species():
     """All-oblate ensemble (APR 0.3: b = c = 3.3 a): the reference rule
     DPMAX = 2 max(EMAJ) (collision_initiation.f90:50-75) makes cells of
@@ -513,7 +524,7 @@ def test_cell_rule_guard_for_oblate_species():
     (brute force), which the reference rule misses some of."""
     wl = W.build_workload("oblate", 48, 3000, k0=3.0, seed=21, apr=0.3, emaj_dx=0.6, st=1.0)
     with pytest.raises(G.GpError) as e:
-        G.Context.from_workload(wl)
+        _ctx_rule(wl, 0)
     assert e.value.code == 8
     st = wl.state
     p = O.params(wl)
@@ -528,7 +539,7 @@ def test_cell_rule_guard_for_oblate_species():
     assert len(need) > 20
     found = {}
     for rule, dpmax in ((2, 2.0 * sp[W.SP_EMAJ]), (1, 2.0 * rmax)):
-        ctx = G.Context.from_workload(wl, cell_rule=rule)
+        ctx = _ctx_rule(wl, rule)
         h, n = ctx.collision_detection_connectivity()
         g = O.collision_grid(p, dpmax)
         con, ncon = O.connectivity(g)
@@ -548,7 +559,7 @@ def test_cell_rule_changes_nothing_when_safe(c1, prec):
     build the same grid and give the same steps bit for bit."""
     out = []
     for rule in (0, 1, 2):
-        ctx = G.Context.from_workload(c1, precision=prec, cell_rule=rule)
+        ctx = _ctx_rule(c1, rule, precision=prec)
         ctx.step(4)
         out.append(ctx.get_particles())
         ctx.close()
