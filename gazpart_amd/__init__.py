@@ -52,8 +52,9 @@ EXPORTS = (
     "gp_partfluid_stats", "gp_bind_field_device", "gp_bind_field_planar", "gp_set_dtime", "gp_dns_create", "gp_dns_destroy",
     "gp_dns_error_string", "gp_dns_set_velocity", "gp_dns_get_velocity", "gp_dns_get_vorticity",
     "gp_dns_step", "gp_dns_cfl", "gp_dns_info", "gp_dns_bind", "gp_get_contacts",
-    "gp_seed_net_gate", "gp_collision_advance",
+    "gp_seed_net_gate", "gp_collision_advance", "gp_interph_noalloc",
 )
+INTERPH_NOALLOC_MAX = 262144
 SEED_NET_NPARAM = 5252
 
 
@@ -152,6 +153,7 @@ def load_library(path: str = LIB_PATH):
     L.gp_get_fluid_at_particles.argtypes = [vp, C.c_int, dp]
     L.gp_interph.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, C.c_int, dp]
     L.gp_interph_field.argtypes = [vp, C.c_int, dp, C.c_int, dp, dp, dp, dp]
+    L.gp_interph_noalloc.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, C.c_int, dp]
     L.gp_adv_particle_ellp.argtypes = [vp, C.c_int]
     L.gp_collision_detection.argtypes = [vp, C.c_int64, ip, ip, C.POINTER(C.c_int64)]
     L.gp_contact_batch.argtypes = [vp, C.c_int64, ip, ip, vp, dp, dp, dp, ip]
@@ -328,6 +330,15 @@ class Context:
         out = np.empty_like(x)
         self._chk(self.lib.gp_interph(self.h, int_scheme, len(x), _ptr(x), _ptr(y), _ptr(z),
                                       comp, _ptr(out)))
+        return out
+
+    def interph_noalloc(self, int_scheme, comp, xint, yint, zint):
+        """gp_interph through the library's fixed staging (no device
+        allocation); at most INTERPH_NOALLOC_MAX points."""
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (xint, yint, zint))
+        out = np.empty_like(x)
+        self._chk(self.lib.gp_interph_noalloc(self.h, int_scheme, len(x), _ptr(x), _ptr(y),
+                                              _ptr(z), comp, _ptr(out)))
         return out
 
     def interph_field(self, int_scheme, uain, xint, yint, zint):

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1569,6 +1570,7 @@ struct gp_ctx {
   // flags (inlm) are set for the particles of plist only
   bool pipe_prev = false;
   int64_t batch_fallbacks = 0;  // gated seeded solves redone in the last gp_contact_batch
+  double* scratch = nullptr;    // this device's copy of the gp_interph_noalloc staging
   // collision grid
   CellGrid g{};
   int* cell = nullptr;
@@ -1632,6 +1634,37 @@ struct gp_ctx {
   // previous step's solve flags
   bool clear_plist = false;
 };
+
+// ---------------------------------------------------------------------------
+// Staging of gp_interph_noalloc: a file-scope device array, one copy per GPU
+// (held until the process ends).  Host code never uses its name as an
+// address: each device's copy is looked up once, with that device current,
+// when the first context on it is created, and kernels and copies take that
+// address.  Every call on a GPU shares its copy.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[4 * (size_t)GP_INTERPH_NOALLOC_MAX];
+constexpr int GP_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch[GP_MAX_DEVICES] = {};
+
+static int scratch_init(gp_ctx* ctx) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (ctx->device < 0 || ctx->device >= GP_MAX_DEVICES) {
+    ctx->err = "device index beyond the staging table";
+    return GP_ERR_HIP;
+  }
+  if (!g_scratch[ctx->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      ctx->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return GP_ERR_HIP;
+    }
+    g_scratch[ctx->device] = (double*)p;
+  }
+  ctx->scratch = g_scratch[ctx->device];
+  return GP_OK;
+}
 
 static void drop_graph(gp_ctx* ctx) {
   if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
@@ -2093,6 +2126,11 @@ int gp_create(const gp_params* p, gp_ctx** out) {
         delete ctx;
         return GP_ERR_HIP;
       }
+  // this device's copy of the gp_interph_noalloc staging (device current)
+  if (scratch_init(ctx) != GP_OK) {
+    gp_destroy(ctx);
+    return GP_ERR_HIP;
+  }
   hipDeviceSynchronize();
   *out = ctx;
   return GP_OK;
@@ -2403,45 +2441,86 @@ int gp_get_fluid_at_particles(gp_ctx* ctx, int np, double* fl) {
   return GP_OK;
 }
 
-int gp_interph(gp_ctx* ctx, int scheme, int nint, const double* x, const double* y,
-               const double* z, int comp, double* out) {
+// the INTERPH kernel on nint points staged at d (x | y | z | out)
+static int launch_interph(gp_ctx* ctx, int scheme, int comp, double* d, int nint) {
+  const int n = ctx->prm.n;
+  const double dx = ctx->sp.dx;
+  const double *x = d, *y = d + nint, *z = d + 2 * (size_t)nint;
+  double* o = d + 3 * (size_t)nint;
+  if (ctx->mixed) {
+    if (scheme == 3)
+      hipLaunchKernelGGL((k_interph<float, float, 4>), dim3(nblk(nint)), dim3(BLOCK), 0, ctx->st,
+                         (const float*)ctx->field, n, dx, x, y, z, comp, o, nint, ctx->d_err);
+    else
+      hipLaunchKernelGGL((k_interph<float, float, 5>), dim3(nblk(nint)), dim3(BLOCK), 0, ctx->st,
+                         (const float*)ctx->field, n, dx, x, y, z, comp, o, nint, ctx->d_err);
+  } else {
+    if (scheme == 3)
+      hipLaunchKernelGGL((k_interph<double, double, 4>), dim3(nblk(nint)), dim3(BLOCK), 0,
+                         ctx->st, (const double*)ctx->field, n, dx, x, y, z, comp, o, nint,
+                         ctx->d_err);
+    else
+      hipLaunchKernelGGL((k_interph<double, double, 5>), dim3(nblk(nint)), dim3(BLOCK), 0,
+                         ctx->st, (const double*)ctx->field, n, dx, x, y, z, comp, o, nint,
+                         ctx->d_err);
+  }
+  HIPCHK(hipGetLastError());
+  return GP_OK;
+}
+
+static int interph_args(gp_ctx* ctx, int scheme, int nint, int comp) {
   if (!ctx || nint < 0 || comp < 0 || comp > 5 || (scheme != 3 && scheme != 4)) return GP_ERR_ARG;
   if (!ctx->have_field) {
     ctx->err = "no field bound";
     return GP_ERR_STATE;
   }
-  if (nint == 0) return GP_OK;
-  double* d = nullptr;
-  HIPCHK(hipMalloc((void**)&d, (size_t)nint * 4 * sizeof(double)));
+  return GP_OK;
+}
+
+// points in, staging at d, result out (synchronous)
+static int interph_staged(gp_ctx* ctx, int scheme, int nint, const double* x, const double* y,
+                          const double* z, int comp, double* out, double* d) {
   HIPCHK(hipMemcpyAsync(d, x, (size_t)nint * 8, hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipMemcpyAsync(d + nint, y, (size_t)nint * 8, hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipMemcpyAsync(d + 2 * (size_t)nint, z, (size_t)nint * 8, hipMemcpyHostToDevice, ctx->st));
-  const int n = ctx->prm.n;
-  const double dx = ctx->sp.dx;
-  double* o = d + 3 * (size_t)nint;
-  if (ctx->mixed) {
-    if (scheme == 3)
-      hipLaunchKernelGGL((k_interph<float, float, 4>), dim3(nblk(nint)), dim3(BLOCK), 0, ctx->st,
-                         (const float*)ctx->field, n, dx, d, d + nint, d + 2 * (size_t)nint, comp,
-                         o, nint, ctx->d_err);
-    else
-      hipLaunchKernelGGL((k_interph<float, float, 5>), dim3(nblk(nint)), dim3(BLOCK), 0, ctx->st,
-                         (const float*)ctx->field, n, dx, d, d + nint, d + 2 * (size_t)nint, comp,
-                         o, nint, ctx->d_err);
-  } else {
-    if (scheme == 3)
-      hipLaunchKernelGGL((k_interph<double, double, 4>), dim3(nblk(nint)), dim3(BLOCK), 0,
-                         ctx->st, (const double*)ctx->field, n, dx, d, d + nint,
-                         d + 2 * (size_t)nint, comp, o, nint, ctx->d_err);
-    else
-      hipLaunchKernelGGL((k_interph<double, double, 5>), dim3(nblk(nint)), dim3(BLOCK), 0,
-                         ctx->st, (const double*)ctx->field, n, dx, d, d + nint,
-                         d + 2 * (size_t)nint, comp, o, nint, ctx->d_err);
-  }
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, o, (size_t)nint * 8, hipMemcpyDeviceToHost, ctx->st));
+  int rc = launch_interph(ctx, scheme, comp, d, nint);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(out, d + 3 * (size_t)nint, (size_t)nint * 8, hipMemcpyDeviceToHost,
+                        ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
+  return GP_OK;
+}
+
+int gp_interph(gp_ctx* ctx, int scheme, int nint, const double* x, const double* y,
+               const double* z, int comp, double* out) {
+  int rc = interph_args(ctx, scheme, nint, comp);
+  if (rc) return rc;
+  if (nint == 0) return GP_OK;
+  double* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, (size_t)nint * 4 * sizeof(double)));
+  rc = interph_staged(ctx, scheme, nint, x, y, z, comp, out, d);
   hipFree(d);
+  if (rc) return rc;
+  return check_err(ctx);
+}
+
+int gp_interph_noalloc(gp_ctx* ctx, int scheme, int nint, const double* x, const double* y,
+                       const double* z, int comp, double* out) {
+  int rc = interph_args(ctx, scheme, nint, comp);
+  if (rc) return rc;
+  if (nint > GP_INTERPH_NOALLOC_MAX) {
+    ctx->err = "gp_interph_noalloc: " + std::to_string(nint) + " points exceed the staging of " +
+               std::to_string(GP_INTERPH_NOALLOC_MAX) + " (use gp_interph)";
+    return GP_ERR_ARG;
+  }
+  if (nint == 0) return GP_OK;
+  if (!ctx->scratch) {
+    ctx->err = "no staging address for this device";
+    return GP_ERR_STATE;
+  }
+  // every point's result is written (zero where the stencil leaves UAIN,
+  // with the error set), so nothing of a previous call is read
+  if ((rc = interph_staged(ctx, scheme, nint, x, y, z, comp, out, ctx->scratch))) return rc;
   return check_err(ctx);
 }
 

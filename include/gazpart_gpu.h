@@ -188,6 +188,16 @@ int gp_get_fluid_at_particles(gp_ctx *ctx, int np, double *fl);
  * host positions; scheme 3 or 4.                                       */
 int gp_interph(gp_ctx *ctx, int scheme, int nint, const double *x, const double *y,
                const double *z, int comp, double *out);
+/* gp_interph without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under stream capture,
+ * serving loops, harnesses that forbid it).  The points and results are
+ * staged in a fixed device array of GP_INTERPH_NOALLOC_MAX points (8 MiB
+ * per GPU, held until the process ends); more points are refused with
+ * GP_ERR_ARG.  All contexts on one GPU share that array: calls that
+ * overlap on one GPU (from several host threads) race.                  */
+#define GP_INTERPH_NOALLOC_MAX 262144
+int gp_interph_noalloc(gp_ctx *ctx, int scheme, int nint, const double *x, const double *y,
+                       const double *z, int comp, double *out);
 /* INTERPH with the field passed in, as the reference passes UAIN: one
  * component UAIN(1:n,-1:n+2,-1:n+2) of the context's grid, FP64 gather.
  * (Copies the component to the device on every call; the step itself
