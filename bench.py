@@ -62,7 +62,7 @@ def advance_bytes(np_, n, field_bytes, taps, s_rw, s_par=16):
     return np_ * (s_rw + s_par) + 6.0 * n ** 3 * field_bytes * touched
 
 
-def run_gpu(wl, precision, steps, warmup, G, groups=None, theta=None, **kw):
+def run_gpu(wl, precision, steps, warmup, G, groups=None, theta=None, graph=1, **kw):
     """Timed steps of the captured hipGraph (8 steps, the next advance of the
     particles outside the contact solves beside those solves), then a
     separate eager run with HIP-event stage timers (per-kernel figures)."""
@@ -73,7 +73,7 @@ def run_gpu(wl, precision, steps, warmup, G, groups=None, theta=None, **kw):
         if theta is not None:
             c.seed_net_set(theta)
         return c
-    ctx = make(graph=True)
+    ctx = make(graph=graph)
     ctx.step(warmup)
     ctx.sync()
     t0 = time.perf_counter()
@@ -521,7 +521,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=9)
     ap.add_argument("--config", default="C2", choices=["C1", "C2", "C2L4", "C3", "C4", "C5"])
     ap.add_argument("--np", type=int, default=0)
     ap.add_argument("--n", type=int, default=0)
@@ -538,6 +538,9 @@ def main():
                     help="wavefronts of the contact-solve launch (0: the library "
                          "default, from the solve count: one per SIMD while "
                          "lanes sit idle, up to four per SIMD)")
+    ap.add_argument("--graph", type=int, default=0, choices=[0, 1, 2],
+                    help="steps without host synchronisation: 1 captured hipGraph replays, 2 "
+                         "direct launches, 0 (default) the faster of the two on this workload")
     ap.add_argument("--pairs", type=int, default=10_000_000,
                     help="C5: solver-labelled pairs the seed network is trained on")
     ap.add_argument("--epochs", type=int, default=3, help="C5: training epochs")
@@ -584,13 +587,21 @@ def main():
         print(json.dumps(dns_bench(wl, G, W, args.steps, args.warmup, not args.no_cpu)))
         return
 
-    kw = {"brick": args.brick, "lm_waves": args.lm_waves}
+    kw = {"brick": args.brick, "lm_waves": args.lm_waves, "graph": args.graph}
     extra = {}
     theta = None
+    from gazpart_amd.autotune import tune
     if cfg == "C4":
-        from gazpart_amd.autotune import tune
         tr = tune(wl, G.PREC_MIXED, groups=groups, log=log)
-        kw = {"brick": tr["best"]["brick"], "lm_waves": tr["best"]["lm_waves"]}
+        kw = {"brick": tr["best"]["brick"], "lm_waves": tr["best"]["lm_waves"],
+              "graph": tr["best"]["graph"]}
+        extra["autotune"] = tr
+    elif args.graph == 0:
+        # launch mode only (captured graph replays or direct launches), with
+        # the given visiting order and solve width
+        tr = tune(wl, G.PREC_MIXED, bricks=(args.brick,), lm_waves=(args.lm_waves,), steps=16,
+                  log=log)
+        kw["graph"] = tr["best"]["graph"]
         extra["autotune"] = tr
     if cfg == "C5":
         nctx = G.Context(wl.n, wl.L, wl.visc, wl.rhof, wl.dtime)

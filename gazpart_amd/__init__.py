@@ -210,7 +210,7 @@ class Context:
         p.prefilter, p.xcd_remap = prefilter, xcd_remap
         p.keep_fluid = 1 if keep_fluid else 0
         p.timing = 1 if timing else 0
-        p.graph = 1 if graph else 0
+        p.graph = int(graph) if graph in (0, 1, 2) else (1 if graph else 0)
         p.brick = brick
         p.lm_seed = lm_seed
         p.lm_waves = lm_waves

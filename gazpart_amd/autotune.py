@@ -11,11 +11,14 @@ out on the GPU, never what is computed:
 * ``lm_waves`` -- the width of the contact-solve launch (one pair per lane,
   pairs dealt lane-major over the wavefronts); 0 means one wavefront per
   SIMD.
+* ``graph`` -- how steps without host synchronisation are issued: 1
+  replays a captured hipGraph of 8 pipelined steps, 2 launches the same
+  kernels directly on the two streams.
 
-:func:`tune` times a few captured (hipGraph) steps of the caller's own
-workload from the same initial state for every combination and returns
-the fastest, with the whole table.  The GPU tests check that the knobs do
-not change results.
+:func:`tune` times a few steps (after a warm-up that leaves a captured
+graph in place) of the caller's own workload from the same initial state
+for every combination and returns the fastest, with the whole table.  The
+GPU tests check that the knobs do not change results.
 """
 from __future__ import annotations
 
@@ -25,29 +28,33 @@ from . import Context
 
 BRICKS = (-1, 2, 4, 8)
 LM_WAVES = (0, 2048, 4096)
+GRAPHS = (1, 2)
 
 
-def tune(wl, precision, *, steps: int = 6, warmup: int = 2, bricks=BRICKS, lm_waves=LM_WAVES,
-         groups=None, log=None, **ctx_kw) -> dict:
+def tune(wl, precision, *, steps: int = 8, warmup: int = 9, bricks=BRICKS, lm_waves=LM_WAVES,
+         graphs=GRAPHS, groups=None, log=None, **ctx_kw) -> dict:
     """Returns {"best": {...}, "table": [...]}; each entry has brick,
-    lm_waves and ms_per_step (graph-mode steps after `warmup` steps)."""
+    lm_waves, graph and ms_per_step (steps after `warmup` steps: the first
+    sizes the lists, the next 8 are one captured graph in mode 1)."""
     table = []
-    for b in bricks:
-        for w in lm_waves:
-            ctx = Context.from_workload(wl, precision=precision, graph=True, brick=b,
-                                        lm_waves=w, **ctx_kw)
-            if groups is not None:
-                ctx.set_groups(groups)
-            ctx.step(warmup)
-            ctx.sync()
-            t0 = time.perf_counter()
-            ctx.step(steps)
-            ctx.sync()
-            dt = time.perf_counter() - t0
-            ctx.close()
-            row = {"brick": b, "lm_waves": w, "ms_per_step": round(1e3 * dt / steps, 4)}
-            table.append(row)
-            if log:
-                log(f"autotune: {row}")
+    for g in graphs:
+        for b in bricks:
+            for w in lm_waves:
+                ctx = Context.from_workload(wl, precision=precision, graph=g, brick=b,
+                                            lm_waves=w, **ctx_kw)
+                if groups is not None:
+                    ctx.set_groups(groups)
+                ctx.step(warmup)
+                ctx.sync()
+                t0 = time.perf_counter()
+                ctx.step(steps)
+                ctx.sync()
+                dt = time.perf_counter() - t0
+                ctx.close()
+                row = {"brick": b, "lm_waves": w, "graph": g,
+                       "ms_per_step": round(1e3 * dt / steps, 4)}
+                table.append(row)
+                if log:
+                    log(f"autotune: {row}")
     best = min(table, key=lambda r: r["ms_per_step"])
     return {"best": best, "table": table}

@@ -2072,7 +2072,8 @@ int gp_create(const gp_params* p, gp_ctx** out) {
   *out = nullptr;
   if (p->n < 5 || (p->interp_scheme != 3 && p->interp_scheme != 4) ||
       (p->precision != GP_PREC_FP64 && p->precision != GP_PREC_MIXED) || !(p->L > 0.0) ||
-      !(p->dtime > 0.0) || !(p->visc > 0.0) || p->cell_rule < 0 || p->cell_rule > 2)
+      !(p->dtime > 0.0) || !(p->visc > 0.0) || p->cell_rule < 0 || p->cell_rule > 2 ||
+      p->graph < 0 || p->graph > 2)
     return GP_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return GP_ERR_HIP;
@@ -3055,20 +3056,17 @@ static void choose_lm_waves(gp_ctx* ctx, unsigned long long nlm) {
 }
 
 template <typename VT>
+static bool graph_matches(const gp_ctx* ctx) {
+  return ctx->gexec && ctx->g_cur == ctx->cur && ctx->g_perm == ctx->perm &&
+         ctx->g_perm_identity == ctx->perm_identity;
+}
+
+template <typename VT>
 static int graph_steps(gp_ctx* ctx, int nsteps) {
   int rc;
   const int unit = graph_unit(ctx);
-  const bool same = ctx->gexec && ctx->g_cur == ctx->cur && ctx->g_perm == ctx->perm &&
-                    ctx->g_perm_identity == ctx->perm_identity;
-  // fewer steps than a captured unit (a caller stepping once per call, as
-  // the reference's main loop does): launched directly, without capture and
-  // without host synchronisation, once the lists are sized
-  if (nsteps < unit && ctx->lists_sized) {
-    if ((rc = launch_steps<VT>(ctx, nsteps))) return rc;
-    ctx->contacts_valid = false;
-    return GP_OK;
-  }
-  if (!same) {
+  const bool direct = ctx->prm.graph == 2;
+  if (!ctx->lists_sized) {
     // one eager step sizes the lists; headroom for the candidate count to grow
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, ctx->d_cnt, 0ull,
                        1ll);
@@ -3077,18 +3075,32 @@ static int graph_steps(gp_ctx* ctx, int nsteps) {
     --nsteps;
     const unsigned long long need = 2ull * (unsigned long long)ctx->last_nclose + 4096ull;
     if ((rc = ensure_lists(ctx, need))) return rc;
-    // the solve's launch width is fixed in the graph: from this step's count
+    // the solve's launch width is fixed from here on: from this step's count
     HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     choose_lm_waves(ctx, ctx->h_cnt[7]);
-    if ((rc = capture_graph<VT>(ctx))) return rc;
+    drop_graph(ctx);
     ctx->lists_sized = true;
   }
-  for (; nsteps >= unit; nsteps -= unit) HIPCHK(hipGraphLaunch(ctx->gexec, ctx->st));
-  if (nsteps > 0) {
-    if ((rc = launch_steps<VT>(ctx, nsteps))) return rc;
+  // gp_params.graph = 2, or fewer steps than a captured unit (a caller
+  // stepping once per call, as the reference's main loop does): launched
+  // directly, without capture and without host synchronisation
+  if (direct || nsteps < unit) {
+    if (nsteps > 0 && (rc = launch_steps<VT>(ctx, nsteps))) return rc;
+    ctx->contacts_valid = false;
+    return GP_OK;
   }
+  if (!graph_matches<VT>(ctx) && ctx->gexec) {
+    // an odd number of steps since the capture left the buffers swapped:
+    // one direct step swaps them back
+    if ((rc = launch_steps<VT>(ctx, 1))) return rc;
+    --nsteps;
+  }
+  if (!graph_matches<VT>(ctx) && nsteps >= unit)
+    if ((rc = capture_graph<VT>(ctx))) return rc;
+  for (; nsteps >= unit; nsteps -= unit) HIPCHK(hipGraphLaunch(ctx->gexec, ctx->st));
+  if (nsteps > 0 && (rc = launch_steps<VT>(ctx, nsteps))) return rc;
   ctx->contacts_valid = false;   // pipelined steps reorder the particles
   return GP_OK;
 }

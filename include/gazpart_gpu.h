@@ -87,9 +87,13 @@ typedef struct {
                           (gp_get_fluid_at_particles) */
   int timing;          /* 1: accumulate per-stage HIP-event times (gp_stats) */
   int graph;           /* 1: gp_step replays a captured hipGraph of two full
-                          steps (no host synchronisation inside a step);
-                          buffers are sized from a first eager step with
-                          headroom, and an overflow is reported as
+                          steps (8 with overlap), no host synchronisation
+                          inside a step; 2: the same steps launched directly
+                          on the streams, without capture (faster than
+                          replaying on this stack, the 8-step pipelined
+                          graph costing ~60 us per step in node launches);
+                          in both, buffers are sized from a first eager step
+                          with headroom, and an overflow is reported as
                           GP_ERR_NCLOSE_OVERFLOW at the next gp_sync */
   int brick;           /* the advance visits particles grouped in bricks of
                           brick^3 collision cells (compact stencil sets per

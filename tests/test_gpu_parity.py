@@ -316,20 +316,21 @@ def test_host_handover_between_steps_changes_no_result(c1, prec):
             b.close()
 
 
-@pytest.mark.parametrize("overlap", [0, 1])
+@pytest.mark.parametrize("mode,overlap", [(1, 0), (1, 1), (2, 1)])
 @pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
-def test_graph_replays_equal_eager_steps(c1, prec, overlap):
+def test_graph_replays_equal_eager_steps(c1, prec, mode, overlap):
     """The captured hipGraph (2 steps, or 8 pipelined steps with the next
     advance beside the contact solves) replayed several times, with eager
     work and synchronisations in between, gives the eager result bit for
     bit after every call; on C1 and on the dense reference ensemble (many
-    contact solves per step)."""
+    contact solves per step).  Mode 2 launches the same pipelined steps
+    directly, without capture."""
     d = np.load(GOLD + "golden_trajectory.npz")
     # the dense ensemble stops at step 53 in the reference (NaN drag, see
     # test_error_at_the_reference_step): keep it below that
     for wl, counts in ((c1, (9, 16, 3, 17, 2)), (golden_workload(d), (9, 16, 3, 9, 2))):
         eager = G.Context.from_workload(wl, precision=prec)
-        graph = G.Context.from_workload(wl, precision=prec, graph=True, overlap=overlap)
+        graph = G.Context.from_workload(wl, precision=prec, graph=mode, overlap=overlap)
         for n in counts:
             eager.step(n)
             graph.step(n)
