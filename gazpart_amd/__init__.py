@@ -52,8 +52,11 @@ EXPORTS = (
     "gp_partfluid_stats", "gp_bind_field_device", "gp_bind_field_planar", "gp_set_dtime", "gp_dns_create", "gp_dns_destroy",
     "gp_dns_error_string", "gp_dns_set_velocity", "gp_dns_get_velocity", "gp_dns_get_vorticity",
     "gp_dns_step", "gp_dns_cfl", "gp_dns_info", "gp_dns_bind", "gp_get_contacts",
-    "gp_seed_net_gate", "gp_collision_advance", "gp_interph_noalloc",
+    "gp_seed_net_gate", "gp_collision_advance", "gp_interph_noalloc", "gp_particle_pdf",
+    "gp_lagrangian_origin", "gp_lagrangian_corr",
 )
+NPDF = 100
+NLGR = 16
 INTERPH_NOALLOC_MAX = 262144
 SEED_NET_NPARAM = 5252
 
@@ -166,6 +169,9 @@ def load_library(path: str = LIB_PATH):
     L.gp_get_collision_counts.argtypes = [vp, C.c_int, ip, C.c_int]
     L.gp_particle_stats.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
     L.gp_partfluid_stats.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
+    L.gp_particle_pdf.argtypes = [vp, C.c_int, dp]
+    L.gp_lagrangian_origin.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.gp_lagrangian_corr.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
     L.gp_set_collision_stats.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.gp_get_collision_stats.argtypes = [vp, dp, dp, C.POINTER(C.c_int64)]
     cp = C.c_char_p
@@ -304,6 +310,29 @@ class Context:
         out = np.zeros((self.nspec, 41))
         self._chk(self.lib.gp_particle_stats(self.h, self.nspec, npoly, int(bool(ref_norm)),
                                              _ptr(out)))
+        return out
+
+    def particle_pdf(self):
+        """MEAN_PART_PDF of STAT_PARTICLE (stat_particle.f90:536-645) over the
+        resident particles: (nspec, 12, NPDF) counts."""
+        out = np.empty((self.nspec, 12, NPDF))
+        self._chk(self.lib.gp_particle_pdf(self.h, self.nspec, _ptr(out)))
+        return out
+
+    def lagrangian_origin(self, slot, mean):
+        """Keep u - MEAN_PART(1:3) of every particle as the origin of
+        correlation slot `slot` (stat_particle.f90:760-770); mean (nspec, 3)."""
+        m = np.ascontiguousarray(mean, dtype=np.float64).reshape(self.nspec, 3)
+        self._chk(self.lib.gp_lagrangian_origin(self.h, slot, self.nspec, _ptr(m)))
+
+    def lagrangian_corr(self, slot, mean, npm):
+        """Per species row the sums of (u - mean) * u_t0 / npm (the
+        increments of RPX/RPY/RPZ_LOC, stat_particle.f90:790-805): (nspec, 3)."""
+        m = np.ascontiguousarray(mean, dtype=np.float64).reshape(self.nspec, 3)
+        n = np.ascontiguousarray(npm, dtype=np.float64).reshape(self.nspec)
+        out = np.empty((self.nspec, 3))
+        self._chk(self.lib.gp_lagrangian_corr(self.h, slot, self.nspec, _ptr(m), _ptr(n),
+                                              _ptr(out)))
         return out
 
     def partfluid_stats(self, npoly=None, ref_norm=True):

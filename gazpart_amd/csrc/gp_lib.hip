@@ -1571,6 +1571,10 @@ struct gp_ctx {
   bool pipe_prev = false;
   int64_t batch_fallbacks = 0;  // gated seeded solves redone in the last gp_contact_batch
   double* scratch = nullptr;    // this device's copy of the gp_interph_noalloc staging
+  // level-2 statistics: u - mean at the origin of each correlation slot,
+  // 3 x np doubles by reference index (allocated when a slot starts)
+  double* lag_u0[GP_NLGR] = {};
+  int lag_np = 0;
   // collision grid
   CellGrid g{};
   int* cell = nullptr;
@@ -2150,6 +2154,8 @@ int gp_destroy(gp_ctx* ctx) {
                   ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout,
                   ctx->cs_rec, ctx->cs_buf, ctx->cs_cnt, ctx->cs_cycle, ctx->cs_dev};
   for (void* b : bufs)
+    if (b) hipFree(b);
+  for (double* b : ctx->lag_u0)
     if (b) hipFree(b);
   for (auto& e : ctx->ev)
     if (e) hipEventDestroy(e);
@@ -3538,6 +3544,133 @@ int gp_particle_stats(gp_ctx* ctx, int nspec, int npoly, int ref_norm, double* o
   return check_err(ctx);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// STAT_PARTICLE's PDFs (LEVEL1_STPDF, stat_particle.f90:536-645): per species
+// row 12 histograms of GP_NPDF bins, bin = int((value - min) / width) + 1
+// clamped to [1, GP_NPDF]: u, v, w in [-2, 2]; body spin in [-2000, 2000];
+// lab spin R(q) omega in [-2000, 2000]; R(q) e_x (the body x axis in the lab
+// frame) in [-1, 1].  R and R omega are formed with the reference's
+// operation order and no contraction, so the bins are the reference's.
+// Counts go to LDS per workgroup when the rows are few, else to global
+// memory.
+// ---------------------------------------------------------------------------
+constexpr int PDF_Q = 12;
+constexpr int PDF_LDS_ROWS = 10;   // 10 x 12 x 100 counters = 48 KB
+
+__device__ __forceinline__ int pdf_bin(double v, double lo, double width) {
+  double t = __ddiv_rn(__dsub_rn(v, lo), width);
+  // int() truncates toward zero; out-of-range values clamp to the end bins
+  if (!(t < 1.0e9)) t = 1.0e9;
+  if (!(t > -1.0e9)) t = -1.0e9;
+  int b = (int)t + 1;
+  return b < 1 ? 0 : (b > GP_NPDF ? GP_NPDF - 1 : b - 1);
+}
+
+template <typename VT>
+__device__ __forceinline__ void pdf_bins(const PState<VT>& p, int i, int b[PDF_Q]) {
+  const double du = 4.0 / (double)GP_NPDF, dw = 4000.0 / (double)GP_NPDF,
+               dp = 2.0 / (double)GP_NPDF;
+  const double u[3] = {(double)p.u[i], (double)p.v[i], (double)p.w[i]};
+  const double w[3] = {(double)p.ox[i], (double)p.oy[i], (double)p.oz[i]};
+  const double a = p.qa[i], qb = p.qb[i], c = p.qc[i], d = p.qd[i];
+  // R of transform_basis.f90:44-54
+  double R[3][3];
+  R[0][0] = __dsub_rn(1.0, __dmul_rn(2.0, __dadd_rn(__dmul_rn(c, c), __dmul_rn(d, d))));
+  R[0][1] = __dmul_rn(2.0, __dsub_rn(__dmul_rn(qb, c), __dmul_rn(a, d)));
+  R[0][2] = __dmul_rn(2.0, __dadd_rn(__dmul_rn(qb, d), __dmul_rn(a, c)));
+  R[1][0] = __dmul_rn(2.0, __dadd_rn(__dmul_rn(qb, c), __dmul_rn(a, d)));
+  R[1][1] = __dsub_rn(1.0, __dmul_rn(2.0, __dadd_rn(__dmul_rn(qb, qb), __dmul_rn(d, d))));
+  R[1][2] = __dmul_rn(2.0, __dsub_rn(__dmul_rn(c, d), __dmul_rn(a, qb)));
+  R[2][0] = __dmul_rn(2.0, __dsub_rn(__dmul_rn(qb, d), __dmul_rn(a, c)));
+  R[2][1] = __dmul_rn(2.0, __dadd_rn(__dmul_rn(c, d), __dmul_rn(a, qb)));
+  R[2][2] = __dsub_rn(1.0, __dmul_rn(2.0, __dadd_rn(__dmul_rn(qb, qb), __dmul_rn(c, c))));
+  for (int k = 0; k < 3; ++k) {
+    b[k] = pdf_bin(u[k], -2.0, du);
+    b[3 + k] = pdf_bin(w[k], -2000.0, dw);
+    const double g = __dadd_rn(__dadd_rn(__dmul_rn(R[k][0], w[0]), __dmul_rn(R[k][1], w[1])),
+                               __dmul_rn(R[k][2], w[2]));
+    b[6 + k] = pdf_bin(g, -2000.0, dw);
+    b[9 + k] = pdf_bin(R[k][0], -1.0, dp);
+  }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(BLOCK) void k_ppdf(PState<VT> p, int np, int nspec,
+                                                unsigned* __restrict__ hist) {
+  __shared__ unsigned sh[PDF_LDS_ROWS * PDF_Q * GP_NPDF];
+  const bool lds = nspec <= PDF_LDS_ROWS;
+  const int nh = nspec * PDF_Q * GP_NPDF;
+  if (lds)
+    for (int k = threadIdx.x; k < nh; k += blockDim.x) sh[k] = 0u;
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+    int b[PDF_Q];
+    pdf_bins(p, i, b);
+    const int base = p.sp[i] * PDF_Q * GP_NPDF;
+    for (int k = 0; k < PDF_Q; ++k) {
+      if (lds) atomicAdd(&sh[base + k * GP_NPDF + b[k]], 1u);
+      else atomicAdd(&hist[base + k * GP_NPDF + b[k]], 1u);
+    }
+  }
+  if (!lds) return;
+  __syncthreads();
+  for (int k = threadIdx.x; k < nh; k += blockDim.x)
+    if (sh[k]) atomicAdd(&hist[k], sh[k]);
+}
+
+// ---------------------------------------------------------------------------
+// STAT_PARTICLE level 2 (stat_particle.f90:737-826): the particle velocity
+// fluctuation at the start of correlation slot NLGR, u - MEAN_PART(1:3, J,
+// IDP), kept per particle (by reference index, so that it survives the
+// reordering of the steps); at a later statistics cycle the sums over each
+// species row of (u - MEAN_PART(1:3)) * u_t0 / NPM(IDP), the increments of
+// RPX/RPY/RPZ_LOC(NTLGR, J, NLGR, IDP), formed per particle as the routine
+// forms them and summed in another order.
+// ---------------------------------------------------------------------------
+template <typename VT>
+__global__ __launch_bounds__(BLOCK) void k_lag_origin(PState<VT> p, int np,
+                                                      const double* __restrict__ mean,
+                                                      double* __restrict__ u0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int s = p.sp[i], id = p.id[i];
+  u0[id] = (double)p.u[i] - mean[3 * s];
+  u0[np + id] = (double)p.v[i] - mean[3 * s + 1];
+  u0[2 * (size_t)np + id] = (double)p.w[i] - mean[3 * s + 2];
+}
+
+constexpr int LAG_LDS_ROWS = 1024;
+
+template <typename VT>
+__global__ __launch_bounds__(BLOCK) void k_lag_corr(PState<VT> p, int np, int nspec,
+                                                    const double* __restrict__ mean,
+                                                    const double* __restrict__ npm,
+                                                    const double* __restrict__ u0,
+                                                    double* __restrict__ out) {
+  __shared__ double sh[3 * LAG_LDS_ROWS];
+  const bool lds = nspec <= LAG_LDS_ROWS;
+  if (lds)
+    for (int k = threadIdx.x; k < 3 * nspec; k += blockDim.x) sh[k] = 0.0;
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+    const int s = p.sp[i], id = p.id[i];
+    const double v[3] = {(double)p.u[i], (double)p.v[i], (double)p.w[i]};
+    for (int c = 0; c < 3; ++c) {
+      const double t = ((v[c] - mean[3 * s + c]) * u0[(size_t)c * np + id]) / npm[s];
+      if (lds) atomicAdd(&sh[3 * s + c], t);
+      else atomicAdd(&out[3 * s + c], t);
+    }
+  }
+  if (!lds) return;
+  __syncthreads();
+  for (int k = threadIdx.x; k < 3 * nspec; k += blockDim.x)
+    if (sh[k] != 0.0) atomicAdd(&out[k], sh[k]);
+}
+
+extern "C" {
+
 int gp_partfluid_stats(gp_ctx* ctx, int nspec, int npoly, int ref_norm, double* out) {
   int rc = ready(ctx);
   if (rc) return rc;
@@ -3600,6 +3733,104 @@ int gp_partfluid_stats(gp_ctx* ctx, int nspec, int npoly, int ref_norm, double* 
     ctx->err = std::string("particle-fluid statistics: ") + hipGetErrorString(e);
     return GP_ERR_HIP;
   }
+  return check_err(ctx);
+}
+
+int gp_particle_pdf(gp_ctx* ctx, int nspec, double* out) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (!out || nspec != ctx->nspec) return GP_ERR_ARG;
+  const size_t nh = (size_t)nspec * PDF_Q * GP_NPDF;
+  unsigned* hist = nullptr;
+  if (hipMalloc((void**)&hist, nh * sizeof(unsigned)) != hipSuccess) {
+    ctx->err = "particle PDFs: out of device memory";
+    return GP_ERR_NOMEM;
+  }
+  HIPCHK(hipMemsetAsync(hist, 0, nh * sizeof(unsigned), ctx->st));
+  const unsigned grid = std::max(1u, std::min(nblk(ctx->np), 1024u));
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_ppdf<float>, dim3(grid), dim3(BLOCK), 0, ctx->st,
+                       pstate<float>(ctx, ctx->cur), ctx->np, nspec, hist);
+  else
+    hipLaunchKernelGGL(k_ppdf<double>, dim3(grid), dim3(BLOCK), 0, ctx->st,
+                       pstate<double>(ctx, ctx->cur), ctx->np, nspec, hist);
+  std::vector<unsigned> h(nh);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h.data(), hist, nh * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
+  hipFree(hist);
+  if (e != hipSuccess) {
+    ctx->err = std::string("particle PDFs: ") + hipGetErrorString(e);
+    return GP_ERR_HIP;
+  }
+  // the routine's histograms start at ZERO (stat_particle.f90:164)
+  for (size_t k = 0; k < nh; ++k) out[k] = (double)h[k] + ZERO_R;
+  return check_err(ctx);
+}
+
+int gp_lagrangian_origin(gp_ctx* ctx, int slot, int nspec, const double* mean) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (slot < 0 || slot >= GP_NLGR || !mean || nspec != ctx->nspec) return GP_ERR_ARG;
+  const size_t n3 = 3 * (size_t)ctx->np;
+  if (ctx->lag_np != ctx->np) {
+    for (double*& b : ctx->lag_u0)
+      if (b) { hipFree(b); b = nullptr; }
+    ctx->lag_np = ctx->np;
+  }
+  if (!ctx->lag_u0[slot] && hipMalloc((void**)&ctx->lag_u0[slot], n3 * sizeof(double)) != hipSuccess) {
+    ctx->lag_u0[slot] = nullptr;
+    ctx->err = "Lagrangian origin: out of device memory";
+    return GP_ERR_NOMEM;
+  }
+  double* dm = nullptr;
+  HIPCHK(hipMalloc((void**)&dm, 3 * (size_t)nspec * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(dm, mean, 3 * (size_t)nspec * sizeof(double), hipMemcpyHostToDevice,
+                        ctx->st));
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_lag_origin<float>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st,
+                       pstate<float>(ctx, ctx->cur), ctx->np, (const double*)dm, ctx->lag_u0[slot]);
+  else
+    hipLaunchKernelGGL(k_lag_origin<double>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st,
+                       pstate<double>(ctx, ctx->cur), ctx->np, (const double*)dm,
+                       ctx->lag_u0[slot]);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  hipFree(dm);
+  return check_err(ctx);
+}
+
+int gp_lagrangian_corr(gp_ctx* ctx, int slot, int nspec, const double* mean, const double* npm,
+                       double* out) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (slot < 0 || slot >= GP_NLGR || !mean || !npm || !out || nspec != ctx->nspec)
+    return GP_ERR_ARG;
+  if (!ctx->lag_u0[slot] || ctx->lag_np != ctx->np) {
+    ctx->err = "Lagrangian slot " + std::to_string(slot) + " has no origin (gp_lagrangian_origin)";
+    return GP_ERR_STATE;
+  }
+  const size_t ns3 = 3 * (size_t)nspec;
+  double* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, (2 * ns3 + nspec) * sizeof(double)));
+  double *dm = d, *dn = d + ns3, *dout = d + ns3 + nspec;
+  HIPCHK(hipMemcpyAsync(dm, mean, ns3 * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(dn, npm, (size_t)nspec * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipMemsetAsync(dout, 0, ns3 * sizeof(double), ctx->st));
+  const unsigned grid = std::max(1u, std::min(nblk(ctx->np), 1024u));
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_lag_corr<float>, dim3(grid), dim3(BLOCK), 0, ctx->st,
+                       pstate<float>(ctx, ctx->cur), ctx->np, nspec, (const double*)dm,
+                       (const double*)dn, (const double*)ctx->lag_u0[slot], dout);
+  else
+    hipLaunchKernelGGL(k_lag_corr<double>, dim3(grid), dim3(BLOCK), 0, ctx->st,
+                       pstate<double>(ctx, ctx->cur), ctx->np, nspec, (const double*)dm,
+                       (const double*)dn, (const double*)ctx->lag_u0[slot], dout);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, ns3 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  hipFree(d);
   return check_err(ctx);
 }
 

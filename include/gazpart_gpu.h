@@ -283,6 +283,32 @@ int gp_particle_stats(gp_ctx *ctx, int nspec, int npoly, int ref_norm, double *o
  * TAUPF come from the rows' RHOP, EMAJ, APR columns as the routine forms
  * them.                                                                  */
 int gp_partfluid_stats(gp_ctx *ctx, int nspec, int npoly, int ref_norm, double *out);
+/* STAT_PARTICLE's PDFs (LEVEL1_STPDF, stat_particle.f90:536-645, on the
+ * cycles that are multiples of 500) over the resident particles:
+ * out[s][k][b] = MEAN_PART_PDF(k+1, J, IDP, b+1) (counts added to the
+ * routine's ZERO = 1e-33 start, :164), k = 0..11: u, v,
+ * w in [-2, 2]; body-frame spin x, y, z in [-2000, 2000]; lab-frame spin
+ * R(q) omega in [-2000, 2000]; the body x axis in the lab frame R(q) e_x
+ * in [-1, 1]; GP_NPDF bins, bin = int((value - min)/width) + 1 clamped to
+ * [1, GP_NPDF] as the routine bins them.  Species rows as for
+ * gp_particle_stats.  The caller adds these to MEAN_TIME_PART_PDF.       */
+#define GP_NPDF 100
+int gp_particle_pdf(gp_ctx *ctx, int nspec, double *out);
+/* STAT_PARTICLE level 2 (LEVEL2_STPAR, stat_particle.f90:737-826), the
+ * Lagrangian velocity autocorrelation.  gp_lagrangian_origin, at cycle
+ * NT0(NLGR) * FOUT2, keeps for correlation slot NLGR - 1 (0 <= slot <
+ * GP_NLGR) every particle's u - MEAN_PART(1:3, J, IDP) (UPT0..WPT0, :760-
+ * 770), mean[s][3] given per species row (MEAN_PART(1:3) of the same
+ * cycle's level 1: gp_particle_stats with ref_norm = 1, entries 1-3).
+ * gp_lagrangian_corr, on each later statistics cycle, returns per row
+ * out[s][3] = the sums over the row's particles of (u - mean) * u_t0 /
+ * npm[s] (:790-805), the increments of RPX/RPY/RPZ_LOC(NTLGR, J, NLGR,
+ * IDP), with npm[s] = NPM(IDP) as the routine holds it (the last class's
+ * count for size IDP).  Sums in another order than the particle loop.    */
+#define GP_NLGR 16
+int gp_lagrangian_origin(gp_ctx *ctx, int slot, int nspec, const double *mean);
+int gp_lagrangian_corr(gp_ctx *ctx, int slot, int nspec, const double *mean, const double *npm,
+                       double *out);
 /* zero the accumulated stage times                                       */
 int gp_reset_timers(gp_ctx *ctx);
 /* introspection for benchmarks: collision grid of the context           */

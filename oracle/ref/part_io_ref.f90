@@ -13,6 +13,14 @@
 !!   part_io_ref fread <name> <n> <out.raw> READ_MPIIO -> out.raw
 !!   part_io_ref pstat <in.raw> <out.raw>  classes from in.raw -> STAT_PARTICLE
 !!                                         (level 1) -> MEAN_PART(1:41, J, IDP)
+!!   part_io_ref ppdf <in.raw> <out.raw>   classes from in.raw -> STAT_PARTICLE
+!!                                         at NCYCLE = 500 with LEVEL1_STPDF ->
+!!                                         MEAN_PART_PDF(1:12, J, IDP, 1:NPDF)
+!!                                         (through MEAN_TIME_PART_PDF)
+!!   part_io_ref plag <in.raw> <out.raw>   three states per class (cycles 1, 2,
+!!                                         3; FOUT2 = 1, NT0(1) = 1) ->
+!!                                         STAT_PARTICLE levels 1 and 2 ->
+!!                                         RPX/RPY/RPZ_LOC(1:3, J, 1, IDP)
 !!
 !! in.raw for psave / pread (stream, native endianness): int32 nig, npoly,
 !! npart, n; double L; int32 npclass(npoly, nig); double rhop(npoly, nig),
@@ -46,7 +54,7 @@ program part_io_ref
   DEBUG = .false.
 
   select case (trim(mode))
-  case ('psave', 'pread', 'pstat')
+  case ('psave', 'pread', 'pstat', 'ppdf', 'plag')
     open(unit=40, file=trim(a1), access='stream', form='unformatted', status='old')
     read(40) ncls, npoly, npart, n
     read(40) L
@@ -69,6 +77,62 @@ program part_io_ref
       close(40)
       read(a2, *) ncycle
       call SAVE_PARTICLE(ncycle)
+    else if (trim(mode) == 'ppdf') then
+      do j = 1, ncls
+        read(40) st, idp
+        call ref_set_particles(j, npart, st, idp)
+      end do
+      close(40)
+      !! one PDF pass (the routine's NCYCLE multiple of 500); MEAN_TIME_PART_PDF
+      !! starts at zero, so it holds that pass's MEAN_PART_PDF
+      NSTAT = 100
+      LEVEL1_STPAR = .false.; LEVEL2_STPAR = .false.; LEVEL3_STPAR = .false.
+      LEVEL1_STPDF = .true.; STAT_TIME = .false.
+      if (allocated(MEAN_TIME_PART_PDF)) deallocate(MEAN_TIME_PART_PDF)
+      allocate(MEAN_TIME_PART_PDF(NSTAT, NIG, POLYDISPMAX, NPDF))
+      MEAN_TIME_PART_PDF = 0.0d0
+      NEVEN_PDF = 0
+      NCYCLEMAX = 1000
+      call STAT_PARTICLE(500, 0.0d0)
+      open(unit=41, file=trim(a2), access='stream', form='unformatted', status='replace')
+      do j = 1, ncls
+        do i = 1, npoly
+          write(41) MEAN_TIME_PART_PDF(1:12, j, i, 1:NPDF)
+        end do
+      end do
+      close(41)
+    else if (trim(mode) == 'plag') then
+      !! levels 1 and 2 on three consecutive statistics cycles; the state of
+      !! each class is replaced between the calls (PART's UPT0..WPT0 kept)
+      NSTAT = 100
+      LEVEL1_STPAR = .true.; LEVEL2_STPAR = .true.; LEVEL3_STPAR = .false.
+      LEVEL1_STPDF = .false.; STAT_TIME = .false.
+      FOUT2 = 1
+      NCYCLEMAX = 3
+      DIMLGR = 3
+      NT0(:) = 1000
+      NT0(1) = 1
+      if (allocated(TIME_LGR)) deallocate(TIME_LGR, RPX_LOC, RPY_LOC, RPZ_LOC)
+      allocate(TIME_LGR(DIMLGR, NBLGRMAX))
+      allocate(RPX_LOC(DIMLGR, NIG, NBLGRMAX, POLYDISPMAX))
+      allocate(RPY_LOC(DIMLGR, NIG, NBLGRMAX, POLYDISPMAX))
+      allocate(RPZ_LOC(DIMLGR, NIG, NBLGRMAX, POLYDISPMAX))
+      TIME_LGR = 0.0d0; RPX_LOC = 0.0d0; RPY_LOC = 0.0d0; RPZ_LOC = 0.0d0
+      do c = 1, 3
+        do j = 1, ncls
+          read(40) st, idp
+          call ref_set_particles(j, npart, st, idp)
+        end do
+        call STAT_PARTICLE(c, 0.1d0 * c)
+      end do
+      close(40)
+      open(unit=41, file=trim(a2), access='stream', form='unformatted', status='replace')
+      do j = 1, ncls
+        do i = 1, npoly
+          write(41) RPX_LOC(1:3, j, 1, i), RPY_LOC(1:3, j, 1, i), RPZ_LOC(1:3, j, 1, i)
+        end do
+      end do
+      close(41)
     else if (trim(mode) == 'pstat') then
       do j = 1, ncls
         read(40) st, idp

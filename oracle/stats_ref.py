@@ -186,3 +186,60 @@ def partfluid_stats(state: np.ndarray, fluid: np.ndarray, species_id: np.ndarray
         out[s] = acc[s] / n
         out[s, 35] = n
     return out
+
+
+# ---------------------------------------------------------------------------
+# STAT_PARTICLE's PDFs and level 2
+# ---------------------------------------------------------------------------
+NPDF = 100
+
+
+def _bins(v, lo, hi):
+    """int((v - lo) / ((hi - lo) / NPDF)) + 1 clamped to [1, NPDF]
+    (stat_particle.f90:562-645), 0-based"""
+    t = (v - lo) / ((hi - lo) / float(NPDF))
+    t = np.clip(np.trunc(t), -1e9, 1e9).astype(np.int64) + 1
+    return np.clip(t, 1, NPDF) - 1
+
+
+def particle_pdf(state: np.ndarray, species_id: np.ndarray, nspec: int) -> np.ndarray:
+    """MEAN_PART_PDF(1:12, J, IDP, 1:NPDF) of one call, (nspec, 12, NPDF),
+    counts added to the routine's ZERO start:
+    u, v, w in [-2, 2]; body spin in [-2000, 2000]; R(q) omega in [-2000,
+    2000]; R(q) e_x in [-1, 1] (stat_particle.f90:536-645).  R omega is
+    summed left to right as the routine's matmul does."""
+    R = rotation(state[6:10])
+    u, w = state[3:6], state[10:13]
+    g = [(R[:, k, 0] * w[0] + R[:, k, 1] * w[1]) + R[:, k, 2] * w[2] for k in range(3)]
+    vals = [(u[k], -2.0, 2.0) for k in range(3)] + [(w[k], -2000.0, 2000.0) for k in range(3)] + \
+        [(g[k], -2000.0, 2000.0) for k in range(3)] + [(R[:, k, 0], -1.0, 1.0) for k in range(3)]
+    out = np.full((nspec, 12, NPDF), 1.0e-33)     # MEAN_PART_PDF = ZERO (:164)
+    for k, (v, lo, hi) in enumerate(vals):
+        b = _bins(v, lo, hi)
+        np.add.at(out, (species_id, k, b), 1.0)
+    return out
+
+
+def lagrangian_origin(state, species_id, mean):
+    """UPT0..WPT0 (stat_particle.f90:760-770): u - MEAN_PART(1:3, J, IDP),
+    (3, np)"""
+    return state[3:6] - mean[species_id].T
+
+
+def lagrangian_corr(state, species_id, mean, npm, u0):
+    """The sums over each species row of (u - MEAN_PART(1:3)) * u_t0 /
+    NPM(IDP) (stat_particle.f90:790-805), (nspec, 3)"""
+    nspec = mean.shape[0]
+    t = ((state[3:6] - mean[species_id].T) * u0) / npm[species_id]
+    out = np.zeros((nspec, 3))
+    for c in range(3):
+        out[:, c] = np.bincount(species_id, weights=t[c], minlength=nspec)
+    return out
+
+
+def npm_of(species_id, nspec, npoly):
+    """NPM(IDP) as STAT_PARTICLE holds it after its class loop: the count of
+    the last class's row of size IDP, per species row"""
+    counts = np.bincount(species_id, minlength=nspec).astype(np.float64)
+    last = counts[nspec - npoly:]
+    return np.tile(last, nspec // npoly)
