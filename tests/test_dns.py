@@ -8,7 +8,10 @@ exact Navier-Stokes solutions whose nonlinear term the projection removes
 (2D Taylor-Green vortex, ABC flow, a shear wave: exact exp(-nu k^2 t)
 decay), the projection's k . U = 0, the filter's bands, second-order
 convergence in time on a nonlinear flow, and the Park-Miller minimal
-standard's published check value for the forcing's generator.
+standard's published check value for the forcing's generator.  The
+forcing's RAN1 and Gauss themselves compile without P3DFFT: the
+restatement equals the reference's own functions bit for bit
+(tests/golden/golden_rng.npz).
 
 Tolerances
   oracle, exact solutions : 1e-13 of the amplitude.
@@ -20,6 +23,8 @@ Tolerances
                             returns the node value) up to 1e-14.
 """
 import math
+
+import os
 
 import numpy as np
 import pytest
@@ -127,6 +132,31 @@ def test_oracle_random_generator():
         vals.append(v)
     v = np.array(vals)
     assert abs(v.mean()) < 0.03 and abs(v.std() - 1.0) < 0.03
+
+
+def test_oracle_random_generator_against_reference_fixture():
+    """RAN1 and Gauss of the restatement equal the reference's own functions
+    (adv_forcing.f90, compiled here; tests/golden/make_rng_golden.py) bit
+    for bit: the first 2048 values of each from four seeds, and the SHA-256
+    of the first 100,000 (the Bays-Durham shuffle index, the warm-up of
+    NTAB + 8 draws and Gauss's cached second deviate all show there)."""
+    import hashlib
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden_rng.npz"))
+    n = int(d["n"])
+    for s in d["seeds"]:
+        s = int(s)
+        tag = f"m{-s}" if s < 0 else f"p{s}"
+        for kind in ("ran1", "gauss"):
+            g = D.Ran1()
+            idum = s
+            out = np.empty(n)
+            f = g.ran1 if kind == "ran1" else g.gauss
+            for i in range(n):
+                out[i], idum = f(idum)
+            head = d[f"{kind}_{tag}"]
+            np.testing.assert_array_equal(out[:len(head)], head)
+            assert hashlib.sha256(out.astype("<f8").tobytes()).hexdigest() == \
+                str(d[f"{kind}_sha_{tag}"]), (kind, s)
 
 
 def test_oracle_forcing_modes_and_statistics():
