@@ -534,6 +534,9 @@ def main():
     ap.add_argument("--brick", type=int, default=0,
                     help="advance visiting order: bricks of BRICK^3 collision cells "
                          "(0: the library default)")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="advance visiting order: field tiles of TILE^3 cells staged in LDS "
+                         "(0: off; overrides --brick)")
     ap.add_argument("--lm-waves", type=int, default=0,
                     help="wavefronts of the contact-solve launch (0: the library "
                          "default, from the solve count: one per SIMD while "
@@ -541,6 +544,9 @@ def main():
     ap.add_argument("--graph", type=int, default=0, choices=[0, 1, 2],
                     help="steps without host synchronisation: 1 captured hipGraph replays, 2 "
                          "direct launches, 0 (default) the faster of the two on this workload")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="no autotuner: the launch shape given by --brick, --tile, --lm-waves "
+                         "and --graph (--graph 0: eager steps)")
     ap.add_argument("--pairs", type=int, default=10_000_000,
                     help="C5: solver-labelled pairs the seed network is trained on")
     ap.add_argument("--epochs", type=int, default=3, help="C5: training epochs")
@@ -587,20 +593,22 @@ def main():
         print(json.dumps(dns_bench(wl, G, W, args.steps, args.warmup, not args.no_cpu)))
         return
 
-    kw = {"brick": args.brick, "lm_waves": args.lm_waves, "graph": args.graph}
+    kw = {"brick": args.brick, "tile": args.tile, "lm_waves": args.lm_waves,
+          "graph": args.graph}
     extra = {}
     theta = None
     from gazpart_amd.autotune import tune
-    if cfg == "C4":
+    if cfg == "C4" and not args.no_tune:
         tr = tune(wl, G.PREC_MIXED, groups=groups, log=log)
-        kw = {"brick": tr["best"]["brick"], "lm_waves": tr["best"]["lm_waves"],
-              "graph": tr["best"]["graph"]}
+        kw = {"brick": tr["best"]["brick"], "tile": tr["best"]["tile"],
+              "lm_waves": tr["best"]["lm_waves"], "graph": tr["best"]["graph"]}
         extra["autotune"] = tr
-    elif args.graph == 0:
+    elif args.graph == 0 and not args.no_tune:
         # launch mode only (captured graph replays or direct launches), with
         # the given visiting order and solve width
-        tr = tune(wl, G.PREC_MIXED, bricks=(args.brick,), lm_waves=(args.lm_waves,), steps=16,
-                  log=log)
+        tr = tune(wl, G.PREC_MIXED, bricks=(args.brick,) if args.tile <= 0 else (),
+                  tiles=(args.tile,) if args.tile > 0 else (), lm_waves=(args.lm_waves,),
+                  steps=16, log=log)
         kw["graph"] = tr["best"]["graph"]
         extra["autotune"] = tr
     if cfg == "C5":

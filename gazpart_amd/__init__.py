@@ -99,7 +99,7 @@ class Params(C.Structure):
                 ("xcd_remap", C.c_int), ("keep_fluid", C.c_int), ("timing", C.c_int),
                 ("graph", C.c_int), ("brick", C.c_int), ("lm_seed", C.c_int),
                 ("lm_waves", C.c_int), ("overlap", C.c_int), ("cell_rule", C.c_int),
-                ("reserved", C.c_int * 2)]
+                ("tile", C.c_int), ("reserved", C.c_int * 1)]
 
 
 class Stats(C.Structure):
@@ -208,7 +208,7 @@ class Context:
     def __init__(self, n, L, visc, rhof, dtime, *, scheme=3, precision=PREC_FP64, ecw=1.0,
                  muw=0.0, dpmax=0.0, prefilter=2, xcd_remap=1, renorm_quat=0,
                  keep_fluid=False, timing=False, graph=False, brick=0, lm_seed=0,
-                 lm_waves=0, overlap=1, cell_rule=0):
+                 lm_waves=0, overlap=1, cell_rule=0, tile=0):
         self.lib = load_library()
         p = Params()
         p.n, p.interp_scheme, p.precision, p.renorm_quat = n, scheme, precision, renorm_quat
@@ -222,6 +222,7 @@ class Context:
         p.lm_waves = lm_waves
         p.overlap = overlap
         p.cell_rule = cell_rule
+        p.tile = tile
         self.params = p
         h = C.c_void_p()
         rc = self.lib.gp_create(C.byref(p), C.byref(h))

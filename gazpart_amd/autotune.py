@@ -1,6 +1,6 @@
 """Autotuner for the launch shapes that leave every result unchanged.
 
-Two knobs of :class:`gazpart_amd.Context` decide only how the work is laid
+Four knobs of :class:`gazpart_amd.Context` decide only how the work is laid
 out on the GPU, never what is computed:
 
 * ``brick`` -- the advance visits particles grouped in bricks of brick^3
@@ -8,6 +8,10 @@ out on the GPU, never what is computed:
   <= 0, in the reference's cell order.  The collision cell itself stays at
   the reference rule (collision_initiation.f90:50-102) because it fixes
   the candidate-pair order that the one-collision-per-particle pass follows.
+* ``tile`` -- the advance visits particles in tiles of tile^3 field cells,
+  one workgroup per tile that stages the field nodes its stencils reach in
+  LDS (k_advance_tiled; overrides brick).  It reads less of HBM and pays
+  off where a tile holds a wavefront's worth of particles or more.
 * ``lm_waves`` -- the width of the contact-solve launch (one pair per lane,
   pairs dealt lane-major over the wavefronts); 0 means one wavefront per
   SIMD.
@@ -18,7 +22,8 @@ out on the GPU, never what is computed:
 :func:`tune` times a few steps (after a warm-up that leaves a captured
 graph in place) of the caller's own workload from the same initial state
 for every combination and returns the fastest, with the whole table.  The
-GPU tests check that the knobs do not change results.
+visiting orders tried are the bricks, then the tiles.  The GPU tests check
+that the knobs do not change results.
 """
 from __future__ import annotations
 
@@ -27,20 +32,22 @@ import time
 from . import Context
 
 BRICKS = (-1, 2, 4, 8)
+TILES = (6, 8)
 LM_WAVES = (0, 2048, 4096)
 GRAPHS = (1, 2)
 
 
-def tune(wl, precision, *, steps: int = 8, warmup: int = 9, bricks=BRICKS, lm_waves=LM_WAVES,
-         graphs=GRAPHS, groups=None, log=None, **ctx_kw) -> dict:
-    """Returns {"best": {...}, "table": [...]}; each entry has brick,
+def tune(wl, precision, *, steps: int = 8, warmup: int = 9, bricks=BRICKS, tiles=TILES,
+         lm_waves=LM_WAVES, graphs=GRAPHS, groups=None, log=None, **ctx_kw) -> dict:
+    """Returns {"best": {...}, "table": [...]}; each entry has brick, tile,
     lm_waves, graph and ms_per_step (steps after `warmup` steps: the first
     sizes the lists, the next 8 are one captured graph in mode 1)."""
+    orders = [(b, 0) for b in bricks] + [(0, t) for t in tiles]
     table = []
     for g in graphs:
-        for b in bricks:
+        for b, t in orders:
             for w in lm_waves:
-                ctx = Context.from_workload(wl, precision=precision, graph=g, brick=b,
+                ctx = Context.from_workload(wl, precision=precision, graph=g, brick=b, tile=t,
                                             lm_waves=w, **ctx_kw)
                 if groups is not None:
                     ctx.set_groups(groups)
@@ -51,7 +58,7 @@ def tune(wl, precision, *, steps: int = 8, warmup: int = 9, bricks=BRICKS, lm_wa
                 ctx.sync()
                 dt = time.perf_counter() - t0
                 ctx.close()
-                row = {"brick": b, "lm_waves": w, "graph": g,
+                row = {"brick": b, "tile": t, "lm_waves": w, "graph": g,
                        "ms_per_step": round(1e3 * dt / steps, 4)}
                 table.append(row)
                 if log:

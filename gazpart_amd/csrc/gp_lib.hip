@@ -62,9 +62,15 @@ struct CellGrid {
 };
 
 // bricks of b x b x b collision cells, row-major: the order in which the
-// advance visits the particles (a compact set of stencils per workgroup)
+// advance visits the particles (a compact set of stencils per workgroup).
+// With tile = 1 the bricks are tiles of b x b x b field cells instead (by
+// the stencil's base node, floor(x / dx)), each advanced by one workgroup
+// that stages the tile's field nodes in LDS (k_advance_tiled).
 struct BrickGrid {
   int b, nx, ny, nz, n;
+  int tile;     // 1: field tiles
+  int nn;       // field nodes per axis (tiles)
+  double dx;    // field spacing (tiles)
 };
 
 struct ActiveRec {   // a contact that is approaching (flag and NVREL < ZERO)
@@ -177,27 +183,46 @@ template <typename FT, int S> struct Row<FT, S, 1> {
 // well (index period n, the reference's x wrap of interp_lag3.f90:132-154),
 // so the stencil is branch-free.  For lag4 the reference's FYP2 row reads
 // J+1 (interp_lag4.f90:196,222,248,274,300): that row gets weight FYP1+FYP2.
-template <typename FT, typename CT, int S, int NC = 6>
-__device__ __forceinline__ bool gather6(const FT* __restrict__ f, int n, double dx, double x,
-                                        double y, double z, CT out[NC]) {
+// the stencil of a position: the reference's 1-based node left of it on
+// each axis (i0, j0, k0) and the Lagrange weights.  False outside the
+// interpolation range (NaN included).
+template <typename CT, int S>
+struct Stencil {
+  int i0, j0, k0;
+  CT fx[S], fz[S], wy[4];
+};
+
+template <typename CT, int S>
+__device__ __forceinline__ bool stencil_at(int n, double dx, double x, double y, double z,
+                                           Stencil<CT, S>& w) {
   if (!(x == x && y == y && z == z)) return false;
   const double tx = x / dx, ty = y / dx, tz = z / dx;
   if (!(tx > -1.0 && ty > -1.0 && tz > -1.0 && tx < n && ty < n && tz < n)) return false;
-  const int i0 = (int)tx + 1, j0 = (int)ty + 1, k0 = (int)tz + 1;
-  if (i0 < 1 || i0 > n || j0 < 1 || j0 > n || k0 < 1 || k0 > n) return false;
-  CT fx[S], fy[S], fz[S];
-  lweights<CT, S>(CT((x - (double)(i0 - 1) * dx) / dx), fx);
-  lweights<CT, S>(CT((y - (double)(j0 - 1) * dx) / dx), fy);
-  lweights<CT, S>(CT((z - (double)(k0 - 1) * dx) / dx), fz);
-  constexpr int lo = (S == 4) ? -1 : -2;
-  constexpr int NY = (S == 4) ? 4 : 4;   // lag4: rows j0-2..j0+1 (J+2 never read)
-  CT wy[4];
+  w.i0 = (int)tx + 1; w.j0 = (int)ty + 1; w.k0 = (int)tz + 1;
+  if (w.i0 < 1 || w.i0 > n || w.j0 < 1 || w.j0 > n || w.k0 < 1 || w.k0 > n) return false;
+  CT fy[S];
+  lweights<CT, S>(CT((x - (double)(w.i0 - 1) * dx) / dx), w.fx);
+  lweights<CT, S>(CT((y - (double)(w.j0 - 1) * dx) / dx), fy);
+  lweights<CT, S>(CT((z - (double)(w.k0 - 1) * dx) / dx), w.fz);
+  // lag4: rows j0-2..j0+1 (J+2 never read)
   if constexpr (S == 4) {
-    wy[0] = fy[0]; wy[1] = fy[1]; wy[2] = fy[2]; wy[3] = fy[3];
+    w.wy[0] = fy[0]; w.wy[1] = fy[1]; w.wy[2] = fy[2]; w.wy[3] = fy[3];
   } else {
-    wy[0] = fy[0]; wy[1] = fy[1]; wy[2] = fy[2]; wy[3] = fy[3] + fy[4];
+    w.wy[0] = fy[0]; w.wy[1] = fy[1]; w.wy[2] = fy[2]; w.wy[3] = fy[3] + fy[4];
   }
-  const int n4 = n + 4;
+  return true;
+}
+
+// first stencil node relative to i0 (and j0, k0): i0-1..i0+2 (lag3),
+// i0-2..i0+2 (lag4); rows j0+lo..j0+lo+3 in both
+template <int S> constexpr int stencil_lo() { return (S == 4) ? -1 : -2; }
+
+// the weighted sum over the stencil; row(jy, kz) points at the S nodes x NC
+// values of row jy of plane kz (the same arithmetic, in the same order,
+// wherever the rows are read from)
+template <typename FT, typename CT, int S, int NC, typename RowAt>
+__device__ __forceinline__ void stencil_sum(const Stencil<CT, S>& w, const RowAt& row,
+                                            CT out[NC]) {
   CT acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = CT(0);
@@ -207,24 +232,87 @@ __device__ __forceinline__ bool gather6(const FT* __restrict__ f, int n, double 
 #pragma unroll
     for (int c = 0; c < NC; ++c) sy[c] = CT(0);
 #pragma unroll
-    for (int jy = 0; jy < NY; ++jy) {
+    for (int jy = 0; jy < 4; ++jy) {
       Row<FT, S, NC> r;
-      r.load(f + node_index(n4, i0 + lo, j0 + lo + jy, k0 + lo + kz) * NC);
+      r.load(row(jy, kz));
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         CT sx = 0;
 #pragma unroll
-        for (int ix = 0; ix < S; ++ix) sx += fx[ix] * CT(r.v[ix * NC + c]);
-        sy[c] += wy[jy] * sx;
+        for (int ix = 0; ix < S; ++ix) sx += w.fx[ix] * CT(r.v[ix * NC + c]);
+        sy[c] += w.wy[jy] * sx;
       }
     }
 #pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] += fz[kz] * sy[c];
+    for (int c = 0; c < NC; ++c) acc[c] += w.fz[kz] * sy[c];
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) out[c] = acc[c];
+}
+
+template <typename FT, typename CT, int S, int NC>
+__device__ __forceinline__ void stencil_sum_global(const FT* __restrict__ f, int n,
+                                                   const Stencil<CT, S>& w, CT out[NC]) {
+  constexpr int lo = stencil_lo<S>();
+  const int n4 = n + 4;
+  stencil_sum<FT, CT, S, NC>(
+      w,
+      [&](int jy, int kz) {
+        return f + node_index(n4, w.i0 + lo, w.j0 + lo + jy, w.k0 + lo + kz) * NC;
+      },
+      out);
+}
+
+template <typename FT, typename CT, int S, int NC = 6>
+__device__ __forceinline__ bool gather6(const FT* __restrict__ f, int n, double dx, double x,
+                                        double y, double z, CT out[NC]) {
+  Stencil<CT, S> w;
+  if (!stencil_at<CT, S>(n, dx, x, y, z, w)) return false;
+  stencil_sum_global<FT, CT, S, NC>(f, n, w, out);
   return true;
 }
+
+// the six fields from the field in HBM
+template <typename FT, typename CT, int S>
+struct GlobalGather {
+  const FT* __restrict__ f;
+  int n;
+  double dx;
+  __device__ __forceinline__ bool operator()(double x, double y, double z, CT out[6]) const {
+    return gather6<FT, CT, S>(f, n, dx, x, y, z, out);
+  }
+};
+
+// the staged field block of k_advance_tiled (dynamic LDS)
+extern __shared__ __align__(16) unsigned char tile_lds[];
+
+// the six fields from a tile's staged nodes in LDS: block of Ex x Ey x Ez
+// nodes whose first node is the stencil's first node for base node
+// (tbx, tby, tbz) (the tile's first cell); positions whose base node lies
+// outside the tile read HBM
+template <typename FT, typename CT, int S>
+struct TileGather {
+  const FT* __restrict__ f;
+  int n;
+  double dx;
+  int tbx, tby, tbz, T, Ex, Ey;
+  __device__ __forceinline__ bool operator()(double x, double y, double z, CT out[6]) const {
+    Stencil<CT, S> w;
+    if (!stencil_at<CT, S>(n, dx, x, y, z, w)) return false;
+    const int bx = w.i0 - 1 - tbx, by = w.j0 - 1 - tby, bz = w.k0 - 1 - tbz;
+    if ((unsigned)bx < (unsigned)T && (unsigned)by < (unsigned)T && (unsigned)bz < (unsigned)T)
+      stencil_sum<FT, CT, S, 6>(
+          w,
+          [&](int jy, int kz) {
+            return reinterpret_cast<const FT*>(tile_lds) +
+                   (((bz + kz) * Ey + (by + jy)) * Ex + bx) * 6;
+          },
+          out);
+    else
+      stencil_sum_global<FT, CT, S, 6>(f, n, w, out);
+    return true;
+  }
+};
 
 // collision cell of a position (collision_detection_connectivity.f90:74-78):
 // IEEE (x - xmin) / dx truncated, 1-based, as a 0-based box.  A position
@@ -256,6 +344,24 @@ __device__ __forceinline__ int cell_of(const CellGrid& g, double x, double y, do
 __device__ __forceinline__ int brick_of(const CellGrid& g, const BrickGrid& bg, int c) {
   const int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny);
   return i / bg.b + bg.nx * (j / bg.b + bg.ny * (k / bg.b));
+}
+
+// field tile along one axis: the stencil's base node floor(x / dx) (as
+// stencil_at takes it) over the tile edge; positions outside the
+// interpolation range go to tile 0 (their advance reports the error)
+__device__ __forceinline__ int tile_axis(double x, const BrickGrid& bg, int nt) {
+  const double t = x / bg.dx;
+  const int b = (t > -1.0 && t < (double)bg.nn) ? (int)t : 0;
+  return min(b / bg.b, nt - 1);
+}
+
+// the brick of a particle at (x, y, z) in collision cell c
+__device__ __forceinline__ int brick_at(const CellGrid& g, const BrickGrid& bg, int c, double x,
+                                        double y, double z) {
+  if (bg.tile)
+    return tile_axis(x, bg, bg.nx) +
+           bg.nx * (tile_axis(y, bg, bg.ny) + bg.ny * tile_axis(z, bg, bg.nz));
+  return brick_of(g, bg, c);
 }
 
 // float not below the double (the bounding-sphere test stays conservative)
@@ -423,9 +529,9 @@ __device__ __forceinline__ int grouped_slot(int* cnt, int key) {
 // one particle: dst[i] = advance(src[s]), its cell and slot (and brick
 // when bcnt; grouped = every lane of the wavefront that is still here
 // calls together), error bits and wall count
-template <typename FT, typename VT, typename CT, int S>
+template <typename VT, typename CT, typename Gather>
 __device__ __forceinline__ void advance_one(
-    const StepParams& P, const FT* __restrict__ f, const Species* __restrict__ spec,
+    const StepParams& P, const Gather& gather, const Species* __restrict__ spec,
     const PState<VT>& src, const PState<VT>& dst, int i, int s, int np, const CellGrid& g,
     int* __restrict__ cell, int* __restrict__ slot, int* __restrict__ cnt, const BrickGrid& bg,
     int* __restrict__ brick, int* __restrict__ bslot, int* __restrict__ bcnt, bool grouped,
@@ -436,7 +542,7 @@ __device__ __forceinline__ void advance_one(
   const int pid = src.id[s];
   CT fl[6];
   unsigned e = 0;
-  if (!gather6<FT, CT, S>(f, P.n, P.dx, st[0], st[1], st[2], fl)) {
+  if (!gather(st[0], st[1], st[2], fl)) {
     e |= EB_INTERP_RANGE;
     for (int c = 0; c < 6; ++c) fl[c] = CT(0);
   }
@@ -459,7 +565,7 @@ __device__ __forceinline__ void advance_one(
     slot[i] = atomicAdd(&cnt[c], 1);
   }
   if (bcnt) {
-    const int b = brick_of(g, bg, c);
+    const int b = brick_at(g, bg, c, st[0], st[1], st[2]);
     brick[i] = b;
     bslot[i] = grouped ? grouped_slot(bcnt, b) : atomicAdd(&bcnt[b], 1);
   }
@@ -481,8 +587,107 @@ __global__ __launch_bounds__(BLOCK) void k_advance(
   if (i >= np) return;
   const int s = perm ? perm[i] : i;
   if (skip && skip[s]) return;
-  advance_one<FT, VT, CT, S>(P, f, spec, src, dst, i, s, np, g, cell, slot, cnt, bg, brick, bslot,
-                             bcnt, true, err, nwall, flout);
+  advance_one<VT, CT>(P, GlobalGather<FT, CT, S>{f, P.n, P.dx}, spec, src, dst, i, s, np, g, cell,
+                      slot, cnt, bg, brick, bslot, bcnt, true, err, nwall, flout);
+}
+
+// quotient and remainder by a small divisor d (dividends below 2^22): a
+// float reciprocal estimate, corrected by one step (its error is below one)
+struct SmallDiv {
+  int d;
+  float rcp;
+  __device__ explicit SmallDiv(int d_) : d(d_), rcp(1.0f / (float)d_) {}
+  __device__ __forceinline__ void divmod(int u, int& q, int& r) const {
+    q = (int)((float)u * rcp);
+    r = u - q * d;
+    if (r < 0) { --q; r += d; }
+    if (r >= d) { ++q; r -= d; }
+  }
+};
+
+// k_advance in field tiles (gp_params.tile): workgroup t (one wavefront)
+// advances the particles at visiting positions boff[t]..boff[t+1] (those
+// whose stencil base node lay in tile t when they were binned).  It first
+// copies the block of nodes that every such stencil reaches, tile + S - 1
+// nodes in x and z and tile + 3 rows in y, from HBM into LDS (rows of 8-byte
+// words, contiguous and coalesced), then gathers from LDS.  A particle whose
+// base node is outside the tile (none, unless positions changed after the
+// binning) gathers from HBM; results are those of k_advance either way.
+constexpr int TILE_WG = 64;
+
+template <int S>
+__host__ __device__ constexpr int tile_extent_x(int T) { return T + S - 1; }
+__host__ __device__ constexpr int tile_extent_y(int T) { return T + 3; }
+
+template <typename FT, typename VT, typename CT, int S>
+__global__ __launch_bounds__(TILE_WG) void k_advance_tiled(
+    StepParams P, const FT* __restrict__ f, const Species* __restrict__ spec, PState<VT> src,
+    PState<VT> dst, const int* __restrict__ trav, const int* __restrict__ boff, int np,
+    CellGrid g, int* __restrict__ cell, int* __restrict__ slot, int* __restrict__ cnt,
+    BrickGrid bg, int* __restrict__ brick, int* __restrict__ bslot, int* __restrict__ bcnt,
+    unsigned* err, unsigned long long* nwall, double* __restrict__ flout, int xcd,
+    const int* __restrict__ skip) {
+  const int t = block_index(xcd);
+  if (t >= bg.n) return;
+  const int p0 = boff[t], p1 = boff[t + 1];
+  if (p0 >= p1) return;
+  const int T = bg.b;
+  const int tx = t % bg.nx, ty = (t / bg.nx) % bg.ny, tz = t / (bg.nx * bg.ny);
+  const int tbx = tx * T, tby = ty * T, tbz = tz * T;
+  const int Ex = tile_extent_x<S>(T), Ey = tile_extent_y(T), Ez = Ex;
+  // storage index (ghosts included) of the block's first node on each
+  // axis: node_index of the stencil's first node for base node tb
+  constexpr int off0 = 2 + stencil_lo<S>();
+  const int n4 = P.n + 4;
+  // nodes past storage index n + 3 exist in no stencil (base nodes stop at
+  // n - 1): a last, partial tile stages fewer
+  const int ex = min(Ex, n4 - (tbx + off0)), ey = min(Ey, n4 - (tby + off0)),
+            ez = min(Ez, n4 - (tbz + off0));
+  constexpr int WPN = 6 * (int)sizeof(FT) / 8;   // 8-byte words per node
+  const int wpr = ex * WPN;                       // words per staged row
+  const int lrow = Ex * WPN;                      // words per LDS row
+  unsigned long long* l64 = reinterpret_cast<unsigned long long*>(tile_lds);
+  {
+    // lane-strided words, TILE_BATCH loads in flight per lane before their
+    // LDS writes (branch-free index arithmetic, so that the loads of a
+    // batch issue back to back)
+    constexpr int TILE_BATCH = 16;
+    const SmallDiv dw(wpr), dy(ey);
+    const int total = wpr * ey * ez;
+    for (int u0 = (int)threadIdx.x; u0 < total; u0 += TILE_WG * TILE_BATCH) {
+      unsigned long long v[TILE_BATCH];
+      int at[TILE_BATCH];
+#pragma unroll
+      for (int u = 0; u < TILE_BATCH; ++u) {
+        // past the end: reload the last word (straight-line code), not written
+        const int idx = u0 + u * TILE_WG, idc = min(idx, total - 1);
+        int r, w, kz, jy;
+        dw.divmod(idc, r, w);
+        dy.divmod(r, kz, jy);
+        const long long node =
+            ((long long)(tbz + off0 + kz) * n4 + (tby + off0 + jy)) * n4 + (tbx + off0);
+        v[u] = reinterpret_cast<const unsigned long long*>(f + node * 6)[w];
+        at[u] = idx < total ? (kz * Ey + jy) * lrow + w : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < TILE_BATCH; ++u)
+        if (at[u] >= 0) l64[at[u]] = v[u];
+    }
+  }
+  __syncthreads();
+  const TileGather<FT, CT, S> gather{f, P.n, P.dx, tbx, tby, tbz, T, Ex, Ey};
+  for (int i = p0 + (int)threadIdx.x; i < p1; i += TILE_WG) {
+    const int s = trav[i];
+    if (skip && skip[s]) continue;
+    advance_one<VT, CT>(P, gather, spec, src, dst, i, s, np, g, cell, slot, cnt, bg, brick, bslot,
+                        bcnt, true, err, nwall, flout);
+  }
+}
+
+template <typename FT, int S>
+static size_t tile_lds_bytes(int T) {
+  const size_t e = (size_t)tile_extent_x<S>(T);
+  return e * e * (size_t)tile_extent_y(T) * 6 * sizeof(FT);
 }
 
 // the particles left out above: storage indices in list[0, *nlist), each
@@ -504,8 +709,8 @@ __global__ __launch_bounds__(ADVL_BLOCK) void k_advance_listed(
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nl;
        t += (long long)gridDim.x * blockDim.x) {
     const int s = list[t];
-    advance_one<FT, VT, CT, S>(P, f, spec, src, dst, inv[s], s, np, g, cell, slot, cnt, bg, brick,
-                               bslot, bcnt, false, err, nwall, flout);
+    advance_one<VT, CT>(P, GlobalGather<FT, CT, S>{f, P.n, P.dx}, spec, src, dst, inv[s], s, np, g,
+                        cell, slot, cnt, bg, brick, bslot, bcnt, false, err, nwall, flout);
   }
 }
 
@@ -524,7 +729,7 @@ __global__ __launch_bounds__(BLOCK) void k_cells(PState<VT> p, int np, CellGrid 
   cell[i] = c;
   slot[i] = atomicAdd(&cnt[c], 1);
   if (bcnt) {
-    const int b = brick_of(g, bg, c);
+    const int b = brick_at(g, bg, c, p.x[i], p.y[i], p.z[i]);
     brick[i] = b;
     bslot[i] = grouped_slot(bcnt, b);
   }
@@ -2052,7 +2257,27 @@ static void grid_init(gp_ctx* ctx) {
   bg.ny = (g.ny + bg.b - 1) / bg.b;
   bg.nz = (g.nz + bg.b - 1) / bg.b;
   bg.n = bg.nx * bg.ny * bg.nz;
+  bg.tile = 0;
+  if (ctx->prm.tile > 0) {
+    // field tiles: base nodes 0..n-1 on each axis
+    const int T = ctx->prm.tile, nn = ctx->prm.n;
+    ctx->use_bricks = true;
+    bg.tile = 1;
+    bg.b = T;
+    bg.nn = nn;
+    bg.dx = ctx->sp.dx;
+    bg.nx = bg.ny = bg.nz = (nn + T - 1) / T;
+    bg.n = bg.nx * bg.ny * bg.nz;
+  }
 }
+
+// LDS of one k_advance_tiled workgroup for tile edge T
+static size_t tile_bytes(const gp_params* p, int T) {
+  if (p->precision == GP_PREC_MIXED)
+    return p->interp_scheme == 3 ? tile_lds_bytes<float, 4>(T) : tile_lds_bytes<float, 5>(T);
+  return p->interp_scheme == 3 ? tile_lds_bytes<double, 4>(T) : tile_lds_bytes<double, 5>(T);
+}
+constexpr size_t TILE_LDS_MAX = 65536;
 
 static int ensure_grid_buffers(gp_ctx* ctx) {
   const CellGrid& g = ctx->g;
@@ -2077,7 +2302,8 @@ int gp_create(const gp_params* p, gp_ctx** out) {
   if (p->n < 5 || (p->interp_scheme != 3 && p->interp_scheme != 4) ||
       (p->precision != GP_PREC_FP64 && p->precision != GP_PREC_MIXED) || !(p->L > 0.0) ||
       !(p->dtime > 0.0) || !(p->visc > 0.0) || p->cell_rule < 0 || p->cell_rule > 2 ||
-      p->graph < 0 || p->graph > 2)
+      p->graph < 0 || p->graph > 2 || p->tile < 0 ||
+      (p->tile > 0 && tile_bytes(p, p->tile) > TILE_LDS_MAX))
     return GP_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return GP_ERR_HIP;
@@ -2613,6 +2839,26 @@ static void launch_advance_kernel(gp_ctx* ctx, hipStream_t st, const int* perm, 
   int* bcnt = bin && ctx->use_bricks ? ctx->bcnt : nullptr;
   int* cell = bin ? ctx->cell : nullptr;
   const int xcd = ctx->prm.xcd_remap;
+  if (ctx->bg.tile && perm == ctx->trav) {
+    // one workgroup per field tile (the visiting order is the tiles')
+    unsigned nt = (unsigned)ctx->bg.n;
+    if (xcd) nt = (nt + 7) / 8 * 8;
+    const size_t lds = tile_bytes(&ctx->prm, ctx->bg.b);
+#define GP_ADVT(FT, CT, S)                                                                      \
+  hipLaunchKernelGGL((k_advance_tiled<FT, VT, CT, S>), dim3(nt), dim3(TILE_WG), lds, st,       \
+                     ctx->sp, (const FT*)ctx->field, ctx->spec, ps, pd, ctx->trav, ctx->boff,   \
+                     np, ctx->g, cell, ctx->slot, ctx->cnt, ctx->bg, ctx->brick, ctx->bslot,    \
+                     bcnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd, skip)
+    if (ctx->mixed) {
+      if (ctx->prm.interp_scheme == 3) GP_ADVT(float, float, 4);
+      else GP_ADVT(float, float, 5);
+    } else {
+      if (ctx->prm.interp_scheme == 3) GP_ADVT(double, double, 4);
+      else GP_ADVT(double, double, 5);
+    }
+#undef GP_ADVT
+    return;
+  }
   unsigned nb = nblk(np);
   if (xcd) nb = (nb + 7) / 8 * 8;
   const dim3 gr(nb), bl(BLOCK);

@@ -13,8 +13,8 @@ module gazpart_gpu
     real(c_double) :: L, visc, rhof, dtime, ecw, muw, dpmax
     integer(c_int) :: prefilter, xcd_remap
     integer(c_int) :: keep_fluid, timing, graph, brick, lm_seed, lm_waves, overlap
-    integer(c_int) :: cell_rule
-    integer(c_int) :: reserved(2)
+    integer(c_int) :: cell_rule, tile
+    integer(c_int) :: reserved(1)
   end type gp_params
 
   type, bind(C) :: gp_stats

@@ -111,6 +111,7 @@ contains
       p%lm_waves = 0
       p%overlap = 0
       p%cell_rule = GPU_CELL_RULE
+      p%tile = 0
       p%reserved = 0
       GPU_CTX(J) = c_null_ptr
       rc = gp_create(p, GPU_CTX(J))
@@ -381,6 +382,7 @@ contains
       p%lm_waves = 0
       p%overlap = 0
       p%cell_rule = 2
+      p%tile = 0
       p%reserved = 0
       rc = gp_create(p, GPU_FIELD_CTX)
       call GPU_CHECK(rc, GPU_FIELD_CTX, 'gp_create')

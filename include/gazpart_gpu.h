@@ -125,7 +125,16 @@ typedef struct {
                           an oblate semi-axis is the largest); 2: the
                           reference rule unchecked (the reference's lists,
                           lost pairs included) */
-  int reserved[2];
+  int tile;            /* > 0: the advance visits particles grouped in tiles
+                          of tile^3 field cells, one workgroup per tile that
+                          stages the field nodes its stencils reach in LDS
+                          (each node read from HBM once per tile instead of
+                          once per stencil); overrides brick.  The staged
+                          block, (tile+S-1)^2 (tile+3) nodes of 6 values
+                          (S = 4 lag3, 5 lag4), must fit 64 KiB: tile <= 10
+                          in the mixed form, 8 (lag3) or 7 (lag4) in FP64;
+                          else GP_ERR_ARG.  0: off.  No effect on results */
+  int reserved[1];
 } gp_params;
 
 typedef struct {

@@ -70,3 +70,34 @@ def test_invalid_arguments_rejected_before_device(lib):
     h = C.c_void_p()
     assert lib.gp_create(C.byref(p), C.byref(h)) == 10
     assert lib.gp_create(None, C.byref(h)) == 10
+
+
+def test_tile_lds_limit_checked_before_device(lib):
+    """gp_params.tile: the staged block, (tile+S-1)^2 (tile+3) nodes of six
+    values, must fit 64 KiB of LDS; larger tiles are refused (GP_ERR_ARG)
+    before any device call, smaller ones get as far as the device."""
+    import ctypes as C
+    import torch
+    ok = [(G.PREC_MIXED, 3, 10), (G.PREC_MIXED, 4, 10), (G.PREC_FP64, 3, 8),
+          (G.PREC_FP64, 4, 7)]
+    bad = [(G.PREC_MIXED, 3, 11), (G.PREC_MIXED, 4, 11), (G.PREC_FP64, 3, 9),
+           (G.PREC_FP64, 4, 8)]
+    for (prec, scheme, tile), refused in [(c, False) for c in ok] + [(c, True) for c in bad]:
+        # closed form of the block's bytes
+        S = 4 if scheme == 3 else 5
+        nbytes = (tile + S - 1) ** 2 * (tile + 3) * 6 * (4 if prec == G.PREC_MIXED else 8)
+        assert (nbytes > 65536) == refused, (prec, scheme, tile, nbytes)
+        p = G.Params()
+        p.n, p.interp_scheme, p.precision, p.L, p.visc, p.dtime = 16, scheme, prec, 6.28, 0.01, 1e-3
+        p.tile = tile
+        h = C.c_void_p()
+        rc = lib.gp_create(C.byref(p), C.byref(h))
+        if refused:
+            assert rc == 10, (prec, scheme, tile)
+        elif torch.cuda.is_available():
+            assert rc == 0
+            lib.gp_destroy(h)
+        else:
+            assert rc == 12
+    p.tile = -1
+    assert lib.gp_create(C.byref(p), C.byref(h)) == 10

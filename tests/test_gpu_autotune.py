@@ -39,8 +39,10 @@ def test_contact_launch_width_changes_no_result(dense, prec):
 
 
 def test_autotune_picks_the_fastest_of_its_table(dense):
-    r = tune(dense, G.PREC_MIXED, steps=2, warmup=1, bricks=(-1, 4), lm_waves=(0, 256))
-    assert len(r["table"]) == 8
+    r = tune(dense, G.PREC_MIXED, steps=2, warmup=1, bricks=(-1, 4), tiles=(8,),
+             lm_waves=(0, 256))
+    assert len(r["table"]) == 12
+    assert {t["tile"] for t in r["table"]} == {0, 8}
     assert {t["graph"] for t in r["table"]} == {1, 2}
     assert r["best"] == min(r["table"], key=lambda t: t["ms_per_step"])
     assert all(t["ms_per_step"] > 0 for t in r["table"])

@@ -261,14 +261,15 @@ def test_prefilters_change_no_result(prec):
 
 @pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
 def test_visiting_order_changes_no_result(c1, prec):
-    """The advance's visiting order (reference cell order, or bricks of 2,
-    8 or 16 cells a side) decides only where particles are stored: the
-    state after 6 steps is the same bit for bit, dense ensemble and C1."""
+    """The advance's visiting order (reference cell order, bricks of 2, 8 or
+    16 collision cells a side, or field tiles of 3, 7 or 8 cells staged in
+    LDS) decides only where particles are stored: the state after 6 steps is
+    the same bit for bit, dense ensemble and C1."""
     d = np.load(GOLD + "golden_trajectory.npz")
     for wl in (golden_workload(d), c1):
         out = []
-        for brick in (-1, 2, 0, 16):
-            ctx = G.Context.from_workload(wl, precision=prec, brick=brick)
+        for brick, tile in ((-1, 0), (2, 0), (0, 0), (16, 0), (0, 3), (0, 7), (0, 8)):
+            ctx = G.Context.from_workload(wl, precision=prec, brick=brick, tile=tile)
             ctx.step(3)
             ctx.adv_particle_ellp(4)
             ctx.collision(4)
