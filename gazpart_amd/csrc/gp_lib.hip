@@ -65,7 +65,7 @@ struct CellGrid {
 // advance visits the particles (a compact set of stencils per workgroup).
 // With tile = 1 the bricks are tiles of b x b x b field cells instead (by
 // the stencil's base node, floor(x / dx)), each advanced by one workgroup
-// that stages the tile's field nodes in LDS (k_advance_tiled).
+// that stages the tile's field nodes in LDS (k_gather_tiled).
 struct BrickGrid {
   int b, nx, ny, nz, n;
   int tile;     // 1: field tiles
@@ -272,6 +272,21 @@ __device__ __forceinline__ bool gather6(const FT* __restrict__ f, int n, double 
   return true;
 }
 
+// fluid at the particles from a buffer filled before the advance
+// (k_gather_tiled): the six fields of visiting position i at c * np + i.
+// Positions outside the interpolation range were reported by that kernel
+// and hold zeros, as the direct gather leaves them.
+template <typename CT>
+struct BufferGather {
+  const CT* __restrict__ fl;
+  int np, i;
+  __device__ __forceinline__ bool operator()(double, double, double, CT out[6]) const {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) out[c] = fl[(long long)c * np + i];
+    return true;
+  }
+};
+
 // the six fields from the field in HBM
 template <typename FT, typename CT, int S>
 struct GlobalGather {
@@ -283,7 +298,7 @@ struct GlobalGather {
   }
 };
 
-// the staged field block of k_advance_tiled (dynamic LDS)
+// the staged field block of k_gather_tiled (dynamic LDS)
 extern __shared__ __align__(16) unsigned char tile_lds[];
 
 // the six fields from a tile's staged nodes in LDS: block of Ex x Ey x Ez
@@ -591,6 +606,56 @@ __global__ __launch_bounds__(BLOCK) void k_advance(
                       slot, cnt, bg, brick, bslot, bcnt, true, err, nwall, flout);
 }
 
+// the advance of the field-tile mode, with the fluid at the particles
+// already gathered (k_gather_tiled), one body for both of its callers:
+// list == nullptr, visiting position i from the (XCD-remapped) grid,
+// particle perm[i], skip flags honoured; list != nullptr, the listed
+// particles list[0, *nlist) to their visiting positions inv[s], grid-stride.
+// (The same routine compiled into two kernels can round differently, the
+// compiler choosing fused multiply-adds per kernel: one kernel keeps the
+// eager and the pipelined steps of this mode bit-identical.)
+template <typename VT, typename CT>
+__global__ __launch_bounds__(BLOCK) void k_advance_fl(
+    StepParams P, const CT* __restrict__ fl, const Species* __restrict__ spec, PState<VT> src,
+    PState<VT> dst, const int* __restrict__ perm, int np, CellGrid g, int* __restrict__ cell,
+    int* __restrict__ slot, int* __restrict__ cnt, BrickGrid bg, int* __restrict__ brick,
+    int* __restrict__ bslot, int* __restrict__ bcnt, unsigned* err, unsigned long long* nwall,
+    double* __restrict__ flout, int xcd, const int* __restrict__ skip,
+    const int* __restrict__ list, const unsigned long long* nlist, const int* __restrict__ inv) {
+  const bool listed = list != nullptr;
+  const long long n = listed ? (long long)min(*nlist, (unsigned long long)np) : (long long)np;
+  const long long stride = listed ? (long long)gridDim.x * blockDim.x : n;
+  for (long long t = (long long)(listed ? (int)blockIdx.x : block_index(xcd)) * blockDim.x +
+                     threadIdx.x;
+       t < n; t += stride) {
+    int i, s;
+    if (listed) {
+      s = list[t];
+      i = inv[s];
+    } else {
+      i = (int)t;
+      s = perm[i];
+      if (skip && skip[s]) continue;
+    }
+    advance_one<VT, CT>(P, BufferGather<CT>{fl, np, i}, spec, src, dst, i, s, np, g, cell, slot,
+                        cnt, bg, brick, bslot, bcnt, !listed, err, nwall, flout);
+  }
+}
+
+// the field tiles (gp_params.tile): workgroup t (one wavefront) takes the
+// particles at visiting positions boff[t]..boff[t+1], those whose stencil
+// base node lay in tile t when they were binned.  stage_tile copies the
+// block of nodes every such stencil reaches, tile + S - 1 nodes in x and z
+// and tile + 3 rows in y, from HBM into LDS (rows of 8-byte words,
+// contiguous and coalesced); TileGather then reads the stencils from LDS
+// (a particle whose base node is outside the tile, none unless positions
+// changed after the binning, reads HBM).
+constexpr int TILE_WG = 64;
+
+template <int S>
+__host__ __device__ constexpr int tile_extent_x(int T) { return T + S - 1; }
+__host__ __device__ constexpr int tile_extent_y(int T) { return T + 3; }
+
 // quotient and remainder by a small divisor d (dividends below 2^22): a
 // float reciprocal estimate, corrected by one step (its error is below one)
 struct SmallDiv {
@@ -605,32 +670,10 @@ struct SmallDiv {
   }
 };
 
-// k_advance in field tiles (gp_params.tile): workgroup t (one wavefront)
-// advances the particles at visiting positions boff[t]..boff[t+1] (those
-// whose stencil base node lay in tile t when they were binned).  It first
-// copies the block of nodes that every such stencil reaches, tile + S - 1
-// nodes in x and z and tile + 3 rows in y, from HBM into LDS (rows of 8-byte
-// words, contiguous and coalesced), then gathers from LDS.  A particle whose
-// base node is outside the tile (none, unless positions changed after the
-// binning) gathers from HBM; results are those of k_advance either way.
-constexpr int TILE_WG = 64;
-
-template <int S>
-__host__ __device__ constexpr int tile_extent_x(int T) { return T + S - 1; }
-__host__ __device__ constexpr int tile_extent_y(int T) { return T + 3; }
-
-template <typename FT, typename VT, typename CT, int S>
-__global__ __launch_bounds__(TILE_WG) void k_advance_tiled(
-    StepParams P, const FT* __restrict__ f, const Species* __restrict__ spec, PState<VT> src,
-    PState<VT> dst, const int* __restrict__ trav, const int* __restrict__ boff, int np,
-    CellGrid g, int* __restrict__ cell, int* __restrict__ slot, int* __restrict__ cnt,
-    BrickGrid bg, int* __restrict__ brick, int* __restrict__ bslot, int* __restrict__ bcnt,
-    unsigned* err, unsigned long long* nwall, double* __restrict__ flout, int xcd,
-    const int* __restrict__ skip) {
-  const int t = block_index(xcd);
-  if (t >= bg.n) return;
-  const int p0 = boff[t], p1 = boff[t + 1];
-  if (p0 >= p1) return;
+template <typename FT, typename CT, int S>
+__device__ __forceinline__ TileGather<FT, CT, S> stage_tile(const FT* __restrict__ f,
+                                                            const StepParams& P,
+                                                            const BrickGrid& bg, int t) {
   const int T = bg.b;
   const int tx = t % bg.nx, ty = (t / bg.nx) % bg.ny, tz = t / (bg.nx * bg.ny);
   const int tbx = tx * T, tby = ty * T, tbz = tz * T;
@@ -675,12 +718,31 @@ __global__ __launch_bounds__(TILE_WG) void k_advance_tiled(
     }
   }
   __syncthreads();
-  const TileGather<FT, CT, S> gather{f, P.n, P.dx, tbx, tby, tbz, T, Ex, Ey};
+  return TileGather<FT, CT, S>{f, P.n, P.dx, tbx, tby, tbz, T, Ex, Ey};
+}
+
+// the six fields at every particle of tile t (skipped ones included: a
+// contact pass changes velocities, not positions), by visiting position
+template <typename FT, typename VT, typename CT, int S>
+__global__ __launch_bounds__(TILE_WG) void k_gather_tiled(
+    StepParams P, const FT* __restrict__ f, PState<VT> src, const int* __restrict__ trav,
+    const int* __restrict__ boff, int np, BrickGrid bg, CT* __restrict__ fl, unsigned* err,
+    int xcd) {
+  const int t = block_index(xcd);
+  if (t >= bg.n) return;
+  const int p0 = boff[t], p1 = boff[t + 1];
+  if (p0 >= p1) return;
+  const TileGather<FT, CT, S> gather = stage_tile<FT, CT, S>(f, P, bg, t);
   for (int i = p0 + (int)threadIdx.x; i < p1; i += TILE_WG) {
     const int s = trav[i];
-    if (skip && skip[s]) continue;
-    advance_one<VT, CT>(P, gather, spec, src, dst, i, s, np, g, cell, slot, cnt, bg, brick, bslot,
-                        bcnt, true, err, nwall, flout);
+    CT v[6];
+    if (!gather(src.x[s], src.y[s], src.z[s], v)) {
+      atomicOr(err, EB_INTERP_RANGE);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) v[c] = CT(0);
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) fl[(long long)c * np + i] = v[c];
   }
 }
 
@@ -1746,6 +1808,13 @@ struct gp_ctx {
   // pipelined steps (gp_params.overlap)
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // field tiles: the fluid at the particles of the next advance, by
+  // visiting position (6 x np of the compute type), gathered before it;
+  // flpre_valid: the last advance launch filled it (the listed advance of
+  // the same step reads it, after ev_gather when it was filled on st2)
+  void* flpre = nullptr;
+  bool flpre_valid = false, flpre_st2 = false;
+  hipEvent_t ev_gather = nullptr;
   int* inlm = nullptr;          // per particle: in a contact solve of this step
   int* plist = nullptr;         // those particles (storage indices)
   int* inv = nullptr;           // storage index -> visiting position of the next advance
@@ -2271,7 +2340,7 @@ static void grid_init(gp_ctx* ctx) {
   }
 }
 
-// LDS of one k_advance_tiled workgroup for tile edge T
+// LDS of one k_gather_tiled workgroup for tile edge T
 static size_t tile_bytes(const gp_params* p, int T) {
   if (p->precision == GP_PREC_MIXED)
     return p->interp_scheme == 3 ? tile_lds_bytes<float, 4>(T) : tile_lds_bytes<float, 5>(T);
@@ -2337,7 +2406,8 @@ int gp_create(const gp_params* p, gp_ctx** out) {
   if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->st2, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_gather, hipEventDisableTiming) != hipSuccess) {
     delete ctx;
     return GP_ERR_HIP;
   }
@@ -2377,7 +2447,7 @@ int gp_destroy(gp_ctx* ctx) {
                   ctx->work, ctx->lmwork, ctx->brick, ctx->bslot, ctx->bcnt, ctx->boff, ctx->trav,
                   ctx->pos4, ctx->csort, ctx->inlm, ctx->plist, ctx->inv, ctx->ccount, ctx->mlp_th, ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, ctx->mlp_wb,
                   ctx->mlp_bias, ctx->mlp_part, ctx->mlp_lpart, ctx->mlp_loss, ctx->seeds,
-                  ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout,
+                  ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout, ctx->flpre,
                   ctx->cs_rec, ctx->cs_buf, ctx->cs_cnt, ctx->cs_cycle, ctx->cs_dev};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -2391,6 +2461,7 @@ int gp_destroy(gp_ctx* ctx) {
   hipStreamSynchronize(ctx->st2);
   hipEventDestroy(ctx->ev_fork);
   hipEventDestroy(ctx->ev_join);
+  hipEventDestroy(ctx->ev_gather);
   hipStreamDestroy(ctx->st2);
   hipStreamDestroy(ctx->st);
   delete ctx;
@@ -2586,6 +2657,14 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   if ((rc = dalloc(ctx, &ctx->ccount, (size_t)np))) return rc;
   HIPCHK(hipMemset(ctx->inlm, 0, (size_t)np * sizeof(int)));
   HIPCHK(hipMemset(ctx->ccount, 0, (size_t)np * sizeof(int)));
+  if (ctx->prm.tile > 0) {
+    if (ctx->mixed) {
+      if ((rc = dalloc(ctx, (float**)&ctx->flpre, (size_t)np * 6))) return rc;
+    } else {
+      if ((rc = dalloc(ctx, (double**)&ctx->flpre, (size_t)np * 6))) return rc;
+    }
+  }
+  ctx->flpre_valid = false;
   if (ctx->prm.keep_fluid) {
     if ((rc = dalloc(ctx, &ctx->flout, (size_t)np * 6))) return rc;
   }
@@ -2839,24 +2918,40 @@ static void launch_advance_kernel(gp_ctx* ctx, hipStream_t st, const int* perm, 
   int* bcnt = bin && ctx->use_bricks ? ctx->bcnt : nullptr;
   int* cell = bin ? ctx->cell : nullptr;
   const int xcd = ctx->prm.xcd_remap;
+  ctx->flpre_valid = false;
   if (ctx->bg.tile && perm == ctx->trav) {
-    // one workgroup per field tile (the visiting order is the tiles')
+    // the fluid at every particle first, one workgroup per field tile (the
+    // visiting order is the tiles'), then the advance at full occupancy
     unsigned nt = (unsigned)ctx->bg.n;
     if (xcd) nt = (nt + 7) / 8 * 8;
     const size_t lds = tile_bytes(&ctx->prm, ctx->bg.b);
-#define GP_ADVT(FT, CT, S)                                                                      \
-  hipLaunchKernelGGL((k_advance_tiled<FT, VT, CT, S>), dim3(nt), dim3(TILE_WG), lds, st,       \
-                     ctx->sp, (const FT*)ctx->field, ctx->spec, ps, pd, ctx->trav, ctx->boff,   \
-                     np, ctx->g, cell, ctx->slot, ctx->cnt, ctx->bg, ctx->brick, ctx->bslot,    \
-                     bcnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd, skip)
+    unsigned nb = nblk(np);
+    if (xcd) nb = (nb + 7) / 8 * 8;
+#define GP_GATHER(FT, CT, S)                                                                    \
+  hipLaunchKernelGGL((k_gather_tiled<FT, VT, CT, S>), dim3(nt), dim3(TILE_WG), lds, st,        \
+                     ctx->sp, (const FT*)ctx->field, ps, ctx->trav, ctx->boff, np, ctx->bg,     \
+                     (CT*)ctx->flpre, ctx->d_err, xcd)
+#define GP_ADVP(CT)                                                                             \
+  hipLaunchKernelGGL((k_advance_fl<VT, CT>), dim3(nb), dim3(BLOCK), 0, st, ctx->sp,            \
+                     (const CT*)ctx->flpre, ctx->spec, ps, pd, perm, np, ctx->g, cell, ctx->slot, \
+                     ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,                \
+                     ctx->d_cnt + 0, ctx->flout, xcd, skip, (const int*)nullptr,                \
+                     (const unsigned long long*)nullptr, (const int*)nullptr)
     if (ctx->mixed) {
-      if (ctx->prm.interp_scheme == 3) GP_ADVT(float, float, 4);
-      else GP_ADVT(float, float, 5);
+      if (ctx->prm.interp_scheme == 3) GP_GATHER(float, float, 4);
+      else GP_GATHER(float, float, 5);
     } else {
-      if (ctx->prm.interp_scheme == 3) GP_ADVT(double, double, 4);
-      else GP_ADVT(double, double, 5);
+      if (ctx->prm.interp_scheme == 3) GP_GATHER(double, double, 4);
+      else GP_GATHER(double, double, 5);
     }
-#undef GP_ADVT
+    // the listed advance on the main stream reads the same buffer
+    ctx->flpre_st2 = st != ctx->st;
+    if (ctx->flpre_st2) (void)hipEventRecord(ctx->ev_gather, st);
+    if (ctx->mixed) GP_ADVP(float);
+    else GP_ADVP(double);
+#undef GP_GATHER
+#undef GP_ADVP
+    ctx->flpre_valid = true;
     return;
   }
   unsigned nb = nblk(np);
@@ -2884,6 +2979,21 @@ static void launch_advance_listed(gp_ctx* ctx, hipStream_t st) {
   auto ps = pstate<VT>(ctx, ctx->cur), pd = pstate<VT>(ctx, 1 - ctx->cur);
   int* bcnt = ctx->use_bricks ? ctx->bcnt : nullptr;
   const dim3 gr(1024), bl(ADVL_BLOCK);
+  if (ctx->flpre_valid) {
+    // field tiles: the fluid gathered with the tiles of this advance (on the
+    // second stream), the same advance kernel as the tiles' own
+    if (ctx->flpre_st2) (void)hipStreamWaitEvent(st, ctx->ev_gather, 0);
+#define GP_ADVF(CT)                                                                              \
+  hipLaunchKernelGGL((k_advance_fl<VT, CT>), gr, bl, 0, st, ctx->sp, (const CT*)ctx->flpre,    \
+                     ctx->spec, ps, pd, (const int*)nullptr, np, ctx->g, ctx->cell, ctx->slot,  \
+                     ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,                \
+                     ctx->d_cnt + 0, ctx->flout, 0, (const int*)nullptr, ctx->plist,            \
+                     ctx->d_cnt + 10, ctx->inv)
+    if (ctx->mixed) GP_ADVF(float);
+    else GP_ADVF(double);
+#undef GP_ADVF
+    return;
+  }
 #define GP_ADVL(FT, CT, S)                                                                       \
   hipLaunchKernelGGL((k_advance_listed<FT, VT, CT, S>), gr, bl, 0, st, ctx->sp,                 \
                      (const FT*)ctx->field, ctx->spec, ps, pd, ctx->plist, ctx->d_cnt + 10,     \
