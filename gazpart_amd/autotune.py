@@ -1,7 +1,8 @@
 """Autotuner for the launch shapes that leave every result unchanged.
 
-Four knobs of :class:`gazpart_amd.Context` decide only how the work is laid
-out on the GPU, never what is computed:
+Four knobs of :class:`gazpart_amd.Context` decide how the work is laid out
+on the GPU, not what is computed (``tile`` changes the rounding of the
+advance, nothing else):
 
 * ``brick`` -- the advance visits particles grouped in bricks of brick^3
   collision cells (compact sets of stencils per workgroup) or, for values
@@ -10,8 +11,10 @@ out on the GPU, never what is computed:
   the candidate-pair order that the one-collision-per-particle pass follows.
 * ``tile`` -- the advance visits particles in tiles of tile^3 field cells,
   one workgroup per tile that stages the field nodes its stencils reach in
-  LDS (k_advance_tiled; overrides brick).  It reads less of HBM and pays
-  off where a tile holds a wavefront's worth of particles or more.
+  LDS and writes the fluid at its particles, then the tile mode's advance
+  kernel reads them (overrides brick).  It reads less of HBM and pays off
+  where a tile holds a wavefront's worth of particles or more; its results
+  equal the direct path's to rounding.
 * ``lm_waves`` -- the width of the contact-solve launch (one pair per lane,
   pairs dealt lane-major over the wavefronts); 0 means one wavefront per
   SIMD.

@@ -126,14 +126,22 @@ typedef struct {
                           reference rule unchecked (the reference's lists,
                           lost pairs included) */
   int tile;            /* > 0: the advance visits particles grouped in tiles
-                          of tile^3 field cells, one workgroup per tile that
+                          of tile^3 field cells: one workgroup per tile
                           stages the field nodes its stencils reach in LDS
-                          (each node read from HBM once per tile instead of
-                          once per stencil); overrides brick.  The staged
-                          block, (tile+S-1)^2 (tile+3) nodes of 6 values
-                          (S = 4 lag3, 5 lag4), must fit 64 KiB: tile <= 10
-                          in the mixed form, 8 (lag3) or 7 (lag4) in FP64;
-                          else GP_ERR_ARG.  0: off.  No effect on results */
+                          and writes the fluid at its particles (each node
+                          read from HBM once per tile instead of once per
+                          stencil), then the advance reads those values;
+                          overrides brick.  Pays off where a tile holds a
+                          wavefront's worth of particles (many ensembles on
+                          one field).  The staged block, (tile+S-1)^2
+                          (tile+3) nodes of 6 values (S = 4 lag3, 5 lag4),
+                          must fit 64 KiB: tile <= 10 in the mixed form, 8
+                          (lag3) or 7 (lag4) in FP64; else GP_ERR_ARG.
+                          Results: identical for every tile size and launch
+                          mode; against tile = 0 they differ by rounding
+                          only (the advance runs in a kernel of its own,
+                          whose fused multiply-adds the compiler chooses
+                          separately).  0: off */
   int reserved[1];
 } gp_params;
 
