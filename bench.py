@@ -14,12 +14,15 @@ setup excluded).
 value        = updates/s of the mixed form (FP32 field, gather and drag;
                FP64 positions, quaternions, contact solve and impulses)
 fp64         = the same for the plain FP64 form
-roofline     = the fused advance kernel (the HBM-bound gather): algorithmic
-               bytes per launch / its average duration (HIP events on the
-               library's stream over timed eager steps) against 8 TB/s;
-               traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE +
-               WRITE_SIZE scaled by a copy calibration, when a summary for
-               this configuration is committed
+roofline     = the advance (the HBM-bound gather; with field tiles the
+               tiled gather and the tile mode's advance kernel together):
+               algorithmic bytes per advance / its average duration (HIP
+               events on the library's stream over timed eager steps)
+               against 8 TB/s; traffic = HBM bytes per advance from
+               rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy
+               calibration, the median over the dispatches of a profiled
+               run (steady state: sorted particles), when a summary for
+               this configuration and kernel is committed
 cpu_baseline = the FP64 CPU oracle (a restatement of the reference path,
                OpenMP over particles) on the host cores, one full step of
                the same workload (C2), or the compiled reference Fortran on
@@ -225,7 +228,14 @@ def cpu_baseline_reference_c2(wl):
              f"{st.shape[1]} particles of C2's state with x < L/16")
 
 
-def roofline(wl, res, scheme, cfg):
+def advance_kernel(launch):
+    """The kernel(s) of one advance in a launch shape: the fused gather +
+    advance, or with field tiles the tiled gather and the mode's advance."""
+    return "k_gather_tiled+k_advance_fl" if launch.get("tile", 0) > 0 else "k_advance"
+
+
+def roofline(wl, res, scheme, cfg, launch):
+    kernel = advance_kernel(launch)
     roof = {}
     for name, fb, srw in (("fp64", 8, 208), ("mixed", 4, 160)):
         if name not in res:
@@ -236,13 +246,14 @@ def roofline(wl, res, scheme, cfg):
         ach = byt / t_adv / 1e9
         roof[name] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                       "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                      "kernel": "k_advance", "bytes_per_launch": int(byt),
+                      "kernel": kernel, "bytes_per_launch": int(byt),
                       "avg_launch_ms": round(t_adv * 1e3, 4)}
     tr_file = os.path.join(ROOT, "profiles", f"traffic_k_advance_{cfg}.json")
     if os.path.exists(tr_file):
         tr = json.load(open(tr_file))
         for name in roof:
-            if name in tr:
+            # a committed measurement of this configuration's kernel(s)
+            if name in tr and tr.get("kernel", "k_advance") == kernel:
                 roof[name]["traffic"] = tr[name]
                 roof[name]["traffic_source"] = tr.get("source", os.path.relpath(tr_file, ROOT))
     return roof
@@ -635,7 +646,7 @@ def main():
         shown = {k: v for k, v in st.items() if k != "collision_counts"}
         log(f"{name}: {ms:.3f} ms/step, {wl.np_ * args.steps / dt:.3e} updates/s, stats {shown}")
 
-    roof = roofline(wl, res, scheme, cfg)
+    roof = roofline(wl, res, scheme, cfg, kw)
     cpu, cpu_ref = None, None
     if not args.no_cpu:
         if cfg == "C1":

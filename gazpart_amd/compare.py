@@ -7,22 +7,25 @@ far two FP64 runs from inputs 1e-12 apart drift (the contact solve's own
 floor).  Each run steps eagerly (ADV_PARTICLE_ELLP then COLLISION); after
 each step the resolved collisions of both runs (gp_get_contacts) are
 compared.  A particle becomes history-divergent when its collision differs
-between the runs, or when it later collides with such a particle in either
-run.  The first divergent collision of every such chain (both particles
-still undivergent) is listed with the decision quantities of each run on
-its own state (gazpart_amd.contacts): depth, both margins, NVREL, the
-pair's closeness to the threshold, the angle between the two runs' contact
-normals and the relative difference of the pair's inputs, and is put in
-one of three classes:
+between the runs (a pair resolved in one run only, or resolved in both with
+contact normals more than MIN_ANGLE apart: the two solves ended at
+different minima, and the impulses differ accordingly), or when it later
+collides with such a particle in either run.  The first divergent collision
+of every such chain (both particles still undivergent) is listed with the
+decision quantities of each run on its own state (gazpart_amd.contacts):
+depth, both margins, NVREL, the pair's closeness to the threshold, the
+angle between the two runs' contact normals and the relative difference of
+the pair's inputs, and is put in one of three classes:
 
   threshold  the contact decision (flag and NVREL < ZERO) differs and, in
              one run or the other, a test sits within MARGIN of its
              switching point;
-  minimum    the contact decision differs and the two solves ended at
-             different minima of their residual (normals more than
-             MIN_ANGLE apart): the reference's solve is discontinuous in
-             its inputs there, and the FP64 form started from inputs 1e-12
-             apart shows such flips too;
+  minimum    the two solves ended at different minima of their residual
+             (normals more than MIN_ANGLE apart), with the contact decision
+             differing or not (a pair resolved in both runs with different
+             normals): the reference's solve is discontinuous in its inputs
+             there, and the FP64 form started from inputs 1e-12 apart shows
+             such flips too;
   order      the contact decision is the same in both runs; the pair was
              resolved in one run only because the one-collision pass had
              given one of its particles to another pair first.
@@ -88,6 +91,16 @@ class Paired:
         ga, gb = sets
         self.ncoll.append((len(ga), len(gb)))
         diff = ga ^ gb
+        # pairs resolved in both runs, by undivergent particles, whose solves
+        # ended at different minima (their impulses differ)
+        both = sorted(p for p in ga & gb if not (self.tainted[p[0] - 1] or self.tainted[p[1] - 1]))
+        if both:
+            h = np.array([p for p, _ in both], np.int32)
+            n = np.array([q for _, q in both], np.int32)
+            _, e1a, e2a, cna, _ = self.a.ellipsoid_contact_detection(h, n)
+            _, e1b, e2b, cnb, _ = self.b.ellipsoid_contact_detection(h, n)
+            ang = np.arccos(np.clip(np.abs((cna * cnb).sum(axis=1)), 0.0, 1.0))
+            diff |= {both[k] for k in np.nonzero(ang > MIN_ANGLE)[0]}
         new_roots = [(p, q) for (p, q) in sorted(diff)
                      if not (self.tainted[p - 1] or self.tainted[q - 1])]
         if new_roots:
@@ -110,7 +123,9 @@ class Paired:
                     sc = np.maximum(np.abs(mids[1][sl][:, idx]).max(axis=0), 1e-300)
                     din = np.maximum(din, da / sc)
             for k, (p, q) in enumerate(new_roots):
-                if ca[k] == cb[k]:
+                if ca[k] == cb[k] and angle[k] > MIN_ANGLE:
+                    kind = "minimum"
+                elif ca[k] == cb[k]:
                     kind = "order"
                 elif close[k] <= MARGIN:
                     kind = "threshold"
