@@ -91,7 +91,7 @@ def run_gpu(wl, precision, steps, warmup, G, groups=None, theta=None, graph=1, *
     # call, the previous cycle's collision pass with this cycle's advance
     # (gp_collision_advance), on a context as GPU_INIT makes it
     if groups is None:
-        dctx = make(overlap=0)
+        dctx = make(overlap=1)
         dctx.step(warmup)
         dctx.adv_particle_ellp(warmup + 1)
         dctx.collision_advance(warmup + 1)   # sizes the lists

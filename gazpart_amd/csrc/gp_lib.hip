@@ -84,6 +84,50 @@ struct WorkRec {     // a candidate pair that passed the prefilter
   int i, j;
 };
 
+// Solving ahead.  The contact solves of a step are, nearly all of them,
+// pairs that were solved in the step before too (near contacts persist for
+// many steps).  So every pair that survived the prefilters of step s is
+// listed by reference index, and at the start of step s+1 (its particles
+// all advanced) those pairs are solved again from the step-s+1 state, beside
+// that step's sort and prefilters; step s+1's filter then sends a pair found
+// in that list to the results instead of to its own solve.  A pair's solve
+// depends only on the two particles' state, so every result is the one the
+// step's own solve would give.
+struct EarlyRec {     // a contact solved ahead
+  int i, j;           // storage slots of the step it was solved for
+  int approach, nfev;
+  double r1[3], r2[3], cn[3];
+};
+
+struct MergeRec {     // a surviving pair whose solve was done ahead
+  long long k;        // its key in this step's list
+  int e, pad;         // its record among the results solved ahead
+};
+
+// k_contact's mode for solving ahead: pairs by reference index
+struct EarlyIn {
+  const int* pid = nullptr;                       // home, near per pair
+  const unsigned long long* n = nullptr;
+  const int* pos = nullptr;                       // reference index -> slot
+  EarlyRec* res = nullptr;                        // null: the step's own solve
+};
+
+// the filter's side of it: this step's surviving pairs listed (and put in
+// a hash table) for the next step, and looked up in the previous step's
+struct EarlyTab {
+  const unsigned long long* pkeys = nullptr;      // previous step's table
+  const int* pvals = nullptr;
+  unsigned long long* nkeys = nullptr;            // this step's (cleared)
+  int* nvals = nullptr;
+  unsigned long long mask = 0;                    // table size - 1
+  int* pid = nullptr;                             // this step's pairs
+  int* slot = nullptr;                            // their table slots
+  unsigned long long* npid = nullptr;
+  MergeRec* merge = nullptr;                      // found in the previous
+  unsigned long long* nmerge = nullptr;
+  unsigned long long cap = 0;
+};
+
 // a contact (flag true) of a statistics step, with the quantities
 // collision_interaction.f90:330-345 sums for it when the ordered pass
 // reaches it with both particles still free
@@ -550,7 +594,8 @@ __device__ __forceinline__ void advance_one(
     const PState<VT>& src, const PState<VT>& dst, int i, int s, int np, const CellGrid& g,
     int* __restrict__ cell, int* __restrict__ slot, int* __restrict__ cnt, const BrickGrid& bg,
     int* __restrict__ brick, int* __restrict__ bslot, int* __restrict__ bcnt, bool grouped,
-    unsigned* err, unsigned long long* nwall, double* __restrict__ flout) {
+    unsigned* err, unsigned long long* nwall, double* __restrict__ flout,
+    int* __restrict__ posid) {
   double st[13];
   load_state(src, s, st);
   const int sid = src.sp[s];
@@ -571,6 +616,7 @@ __device__ __forceinline__ void advance_one(
   dst.sp[i] = sid;
   dst.id[i] = pid;
   dst.grp[i] = src.grp[s];
+  if (posid) posid[pid] = i;   // reference index -> slot of the new state
   int c = 0;
   if (cell) {
     bool oob;
@@ -597,13 +643,13 @@ __global__ __launch_bounds__(BLOCK) void k_advance(
     PState<VT> dst, const int* __restrict__ perm, int np, CellGrid g, int* __restrict__ cell,
     int* __restrict__ slot, int* __restrict__ cnt, BrickGrid bg, int* __restrict__ brick,
     int* __restrict__ bslot, int* __restrict__ bcnt, unsigned* err, unsigned long long* nwall,
-    double* __restrict__ flout, int xcd, const int* __restrict__ skip) {
+    double* __restrict__ flout, int xcd, const int* __restrict__ skip, int* __restrict__ posid) {
   const int i = block_index(xcd) * BLOCK + threadIdx.x;
   if (i >= np) return;
   const int s = perm ? perm[i] : i;
   if (skip && skip[s]) return;
   advance_one<VT, CT>(P, GlobalGather<FT, CT, S>{f, P.n, P.dx}, spec, src, dst, i, s, np, g, cell,
-                      slot, cnt, bg, brick, bslot, bcnt, true, err, nwall, flout);
+                      slot, cnt, bg, brick, bslot, bcnt, true, err, nwall, flout, posid);
 }
 
 // the advance of the field-tile mode, with the fluid at the particles
@@ -621,7 +667,8 @@ __global__ __launch_bounds__(BLOCK) void k_advance_fl(
     int* __restrict__ slot, int* __restrict__ cnt, BrickGrid bg, int* __restrict__ brick,
     int* __restrict__ bslot, int* __restrict__ bcnt, unsigned* err, unsigned long long* nwall,
     double* __restrict__ flout, int xcd, const int* __restrict__ skip,
-    const int* __restrict__ list, const unsigned long long* nlist, const int* __restrict__ inv) {
+    const int* __restrict__ list, const unsigned long long* nlist, const int* __restrict__ inv,
+    int* __restrict__ posid) {
   const bool listed = list != nullptr;
   const long long n = listed ? (long long)min(*nlist, (unsigned long long)np) : (long long)np;
   const long long stride = listed ? (long long)gridDim.x * blockDim.x : n;
@@ -638,7 +685,7 @@ __global__ __launch_bounds__(BLOCK) void k_advance_fl(
       if (skip && skip[s]) continue;
     }
     advance_one<VT, CT>(P, BufferGather<CT>{fl, np, i}, spec, src, dst, i, s, np, g, cell, slot,
-                        cnt, bg, brick, bslot, bcnt, !listed, err, nwall, flout);
+                        cnt, bg, brick, bslot, bcnt, !listed, err, nwall, flout, posid);
   }
 }
 
@@ -766,13 +813,13 @@ __global__ __launch_bounds__(ADVL_BLOCK) void k_advance_listed(
     const int* __restrict__ inv, int np, CellGrid g,
     int* __restrict__ cell, int* __restrict__ slot, int* __restrict__ cnt, BrickGrid bg,
     int* __restrict__ brick, int* __restrict__ bslot, int* __restrict__ bcnt, unsigned* err,
-    unsigned long long* nwall, double* __restrict__ flout) {
+    unsigned long long* nwall, double* __restrict__ flout, int* __restrict__ posid) {
   const long long nl = (long long)min(*nlist, (unsigned long long)np);
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nl;
        t += (long long)gridDim.x * blockDim.x) {
     const int s = list[t];
     advance_one<VT, CT>(P, GlobalGather<FT, CT, S>{f, P.n, P.dx}, spec, src, dst, inv[s], s, np, g,
-                        cell, slot, cnt, bg, brick, bslot, bcnt, false, err, nwall, flout);
+                        cell, slot, cnt, bg, brick, bslot, bcnt, false, err, nwall, flout, posid);
   }
 }
 
@@ -810,6 +857,12 @@ struct SideClear {
   long long nzi2 = 0;
   unsigned long long* zc = nullptr;
   int nzc = 0;
+  unsigned long long* zc2 = nullptr;
+  int nzc2 = 0;
+  unsigned long long* zl = nullptr;          // keys[slot[t]] = 0 for t < *nslot
+  const int* slot = nullptr;
+  const unsigned long long* nslot = nullptr;
+  unsigned long long nslot_cap = 0;
   const int* list = nullptr;
   const unsigned long long* nlist = nullptr;
   int* flag = nullptr;
@@ -823,6 +876,15 @@ __device__ __forceinline__ void side_clear(const SideClear& sc) {
   for (long long t = t0; t < sc.nzi2; t += st) sc.zi2[t] = 0;
   if (sc.zc)
     for (long long t = t0; t < sc.nzc; t += st) sc.zc[t] = 0ull;
+  if (sc.zc2)
+    for (long long t = t0; t < sc.nzc2; t += st) sc.zc2[t] = 0ull;
+  if (sc.zl) {
+    const long long ns = (long long)min(*sc.nslot, sc.nslot_cap);
+    for (long long t = t0; t < ns; t += st) {
+      const int h = sc.slot[t];
+      if (h >= 0) sc.zl[h] = 0ull;
+    }
+  }
   if (sc.list) {
     const long long nl = (long long)min(*sc.nlist, (unsigned long long)sc.np);
     for (long long t = t0; t < nl; t += st) sc.flag[sc.list[t]] = 0;
@@ -1085,6 +1147,53 @@ __global__ __launch_bounds__(BLOCK) void k_prefilter(
   }
 }
 
+// open-addressing table of pair keys (pid_home + 1) << 32 | (pid_near + 1),
+// 0 = empty; sized at least twice the pairs it holds, so probes end
+__device__ __forceinline__ unsigned long long pair_hash(unsigned long long k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+__device__ __forceinline__ int table_put(unsigned long long* keys, int* vals,
+                                         unsigned long long mask, unsigned long long key, int v) {
+  unsigned long long h = pair_hash(key) & mask;
+  for (unsigned long long probe = 0; probe <= mask; ++probe) {
+    const unsigned long long old = atomicCAS(&keys[h], 0ull, key);
+    if (old == 0ull || old == key) {
+      vals[h] = v;
+      return (int)h;
+    }
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+__device__ __forceinline__ int table_get(const unsigned long long* keys, const int* vals,
+                                         unsigned long long mask, unsigned long long key) {
+  unsigned long long h = pair_hash(key) & mask;
+  for (unsigned long long probe = 0; probe <= mask; ++probe) {
+    const unsigned long long k = keys[h];
+    if (k == key) return vals[h];
+    if (k == 0ull) return -1;
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+
+// one slot per lane that asks, one atomic per wavefront
+__device__ __forceinline__ unsigned long long wave_append(bool want, unsigned long long* count) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long m = __ballot(want);
+  if (!m) return 0;
+  const int leader = __ffsll((long long)m) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  return base + __popcll(m & ((1ull << lane) - 1ull));
+}
+
 // exact disjointness test (Perram-Wertheim) on the bounding-sphere
 // survivors; pairs not proven disjoint go to the LM list in any order (the
 // list index k keeps the reference order).  use_pw = 0 passes every pair.
@@ -1095,7 +1204,7 @@ __global__ __launch_bounds__(BLOCK) void k_pw_filter(
     PState<VT> p, const Species* __restrict__ spec, const WorkRec* __restrict__ work,
     const unsigned long long* nwork, unsigned long long cap, double L, int use_pw,
     WorkRec* __restrict__ lm, unsigned long long* nlm, int* __restrict__ flag,
-    int* __restrict__ plist, unsigned long long* nplist, SideClear sc) {
+    int* __restrict__ plist, unsigned long long* nplist, SideClear sc, EarlyTab et) {
   side_clear(sc);
   const unsigned long long nw = min(*nwork, cap);
   const int lane = threadIdx.x & 63;
@@ -1125,14 +1234,37 @@ __global__ __launch_bounds__(BLOCK) void k_pw_filter(
         }
       }
     }
+    // solving ahead: list the pair for the next step, and take it out of
+    // this step's solves when the previous step listed it
+    bool solve = pass;
+    if (et.nkeys) {
+      unsigned long long key = 0;
+      int found = -1;
+      if (pass) {
+        const int pi = p.id[r.i], pj = p.id[r.j];
+        key = ((unsigned long long)(pi + 1) << 32) | (unsigned long long)(pj + 1);
+        found = table_get(et.pkeys, et.pvals, et.mask, key);
+      }
+      const unsigned long long e = wave_append(pass, et.npid);
+      if (pass && e < et.cap) {
+        et.pid[2 * e] = (int)(key >> 32) - 1;
+        et.pid[2 * e + 1] = (int)(key & 0xffffffffull) - 1;
+        et.slot[e] = table_put(et.nkeys, et.nvals, et.mask, key, (int)e);
+      }
+      const unsigned long long g = wave_append(found >= 0, et.nmerge);
+      if (found >= 0 && g < et.cap) {
+        et.merge[g] = MergeRec{r.k, found, 0};
+        solve = false;
+      }
+    }
     // one atomic per wavefront: ballot + prefix
-    const unsigned long long m = __ballot(pass);
+    const unsigned long long m = __ballot(solve);
     if (m) {
       const int leader = __ffsll((long long)m) - 1;
       unsigned long long base = 0;
       if (lane == leader) base = atomicAdd(nlm, (unsigned long long)__popcll(m));
       base = __shfl(base, leader);
-      if (pass) lm[base + __popcll(m & ((1ull << lane) - 1ull))] = r;
+      if (solve) lm[base + __popcll(m & ((1ull << lane) - 1ull))] = r;
     }
     // flag (once) and list the particles of the solves, whose advance of the
     // next step waits for this step's collisions
@@ -1197,8 +1329,11 @@ __global__ __launch_bounds__(LM_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) v
     ActiveRec* __restrict__ act, unsigned long long* nact, unsigned long long acap,
     unsigned long long* nfev_sum, unsigned long long* ncontact, unsigned* err,
     const double* __restrict__ seeds, SeedGate gate, unsigned long long* nfallback,
-    const CStat* __restrict__ csp) {
-  const unsigned long long nw = min(*nwork, cap);
+    const CStat* __restrict__ csp, EarlyIn ein) {
+  // ein.res: solving ahead (pairs by reference index, results to ein.res;
+  // the same solve in the same kernel, so the same bits)
+  const bool ahead = ein.res != nullptr;
+  const unsigned long long nw = min(ahead ? *ein.n : *nwork, cap);
   const unsigned long long waves = gridDim.x;
   const int lane = threadIdx.x;
   unsigned long long nfev = 0, nfb = 0;
@@ -1209,7 +1344,14 @@ __global__ __launch_bounds__(LM_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) v
     bool approach = false;
     ActiveRec ar;
     if (w < nw) {
-      const WorkRec r = work[w];
+      WorkRec r;
+      if (ahead) {
+        r.k = 0;
+        r.i = ein.pos[ein.pid[2 * w]];
+        r.j = ein.pos[ein.pid[2 * w + 1]];
+      } else {
+        r = work[w];
+      }
       double bi[3], bj[3];
       min_image(p, r.i, r.j, L, bi, bj);
       const double qi[4] = {p.qa[r.i], p.qb[r.i], p.qc[r.i], p.qd[r.i]};
@@ -1251,6 +1393,19 @@ __global__ __launch_bounds__(LM_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) v
                        v2[1] - v1[1], v2[2] - v1[2], o.e1[0] - o.e2[0], o.e1[1] - o.e2[1],
                        o.e1[2] - o.e2[2], err);
       }
+      if (ahead) {
+        EarlyRec& e = ein.res[w];
+        e.i = r.i;
+        e.j = r.j;
+        e.approach = approach ? 1 : 0;
+        e.nfev = nf;
+        for (int c = 0; c < 3; ++c) {
+          e.r1[c] = approach ? ar.r1[c] : 0.0;
+          e.r2[c] = approach ? ar.r2[c] : 0.0;
+          e.cn[c] = approach ? ar.cn[c] : 0.0;
+        }
+        approach = false;
+      }
     }
     const unsigned long long m = __ballot(approach);
     if (m) {
@@ -1273,8 +1428,53 @@ __global__ __launch_bounds__(LM_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) v
     nfev += __shfl_xor(nfev, o);
     nfb += __shfl_xor(nfb, o);
   }
+  if (ahead) return;   // counted when the results are used
   if (lane == 0 && nfev) atomicAdd(nfev_sum, nfev);
   if (lane == 0 && nfb) atomicAdd(nfallback, nfb);
+}
+
+// the surviving pairs of this step that were solved ahead: their active
+// contacts (with this step's list keys) join those of the step's own solve,
+// and their evaluations and pairs are counted
+__global__ __launch_bounds__(BLOCK) void k_merge_early(
+    const MergeRec* __restrict__ merge, const unsigned long long* nmerge, unsigned long long cap,
+    const EarlyRec* __restrict__ res, ActiveRec* __restrict__ act, unsigned long long* nact,
+    unsigned long long acap, unsigned long long* nfev_sum, unsigned long long* ncontact,
+    unsigned long long* nlm, unsigned* err) {
+  const unsigned long long n = min(*nmerge, cap);
+  unsigned long long nfev = 0, npair = 0;
+  for (unsigned long long b0 = (unsigned long long)blockIdx.x * blockDim.x; b0 < n;
+       b0 += (unsigned long long)gridDim.x * blockDim.x) {
+    const unsigned long long t = b0 + threadIdx.x;
+    bool approach = false;
+    ActiveRec ar;
+    if (t < n) {
+      const MergeRec mr = merge[t];
+      const EarlyRec& e = res[mr.e];
+      nfev += (unsigned long long)e.nfev;
+      ++npair;
+      if (e.approach) {
+        approach = true;
+        ar.k = mr.k; ar.i = e.i; ar.j = e.j;
+        for (int c = 0; c < 3; ++c) { ar.r1[c] = e.r1[c]; ar.r2[c] = e.r2[c]; ar.cn[c] = e.cn[c]; }
+      }
+    }
+    const unsigned long long a = wave_append(approach, nact);
+    if (approach) {
+      if (a < acap) act[a] = ar;
+      else atomicOr(err, EB_ACTIVE_OVERFLOW);
+    }
+    const unsigned long long ma = __ballot(approach);
+    if ((threadIdx.x & 63) == 0 && ma) atomicAdd(ncontact, (unsigned long long)__popcll(ma));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    nfev += __shfl_xor(nfev, o);
+    npair += __shfl_xor(npair, o);
+  }
+  if ((threadIdx.x & 63) == 0 && npair) {
+    atomicAdd(nfev_sum, nfev);
+    atomicAdd(nlm, npair);
+  }
 }
 
 // network forward for the 64 lanes of a one-wavefront workgroup: features
@@ -1911,6 +2111,26 @@ struct gp_ctx {
   // set around a sort inside pipelined steps: its scatter clears the
   // previous step's solve flags
   bool clear_plist = false;
+  // solving ahead (pipelined steps, gp_params.overlap = 1): per step parity
+  // p, the filter lists its surviving pairs in ep_pid[p] (count d_cnt[13 +
+  // p]) with the hash table ep_keys/ep_vals[p], and looks pairs up in the
+  // other parity's, solved from this step's state by k_contact on st3 into
+  // ep_res; ep_merge takes the pairs found (count d_cnt[12])
+  hipStream_t st3 = nullptr;
+  hipEvent_t ev_efork = nullptr, ev_edone = nullptr;
+  unsigned long long* ep_keys[2] = {nullptr, nullptr};
+  int* ep_vals[2] = {nullptr, nullptr};
+  int* ep_pid[2] = {nullptr, nullptr};
+  int* ep_slot[2] = {nullptr, nullptr};
+  int* ep_pos[2] = {nullptr, nullptr};   // reference index -> slot, per buffer
+  bool ep_pos_valid[2] = {false, false};
+  EarlyRec* ep_res = nullptr;
+  MergeRec* ep_merge = nullptr;
+  unsigned long long ep_tab = 0;   // table size (a power of two)
+  unsigned long long ep_cap = 0;   // pairs per list
+  int ep_par = 0;
+  bool ep_step = false;            // this step solves ahead
+  unsigned long long last_nlm = 0; // last known count of contact solves
 };
 
 // ---------------------------------------------------------------------------
@@ -2371,7 +2591,7 @@ int gp_create(const gp_params* p, gp_ctx** out) {
   if (p->n < 5 || (p->interp_scheme != 3 && p->interp_scheme != 4) ||
       (p->precision != GP_PREC_FP64 && p->precision != GP_PREC_MIXED) || !(p->L > 0.0) ||
       !(p->dtime > 0.0) || !(p->visc > 0.0) || p->cell_rule < 0 || p->cell_rule > 2 ||
-      p->graph < 0 || p->graph > 2 || p->tile < 0 ||
+      p->graph < 0 || p->graph > 2 || p->overlap < 0 || p->overlap > 2 || p->tile < 0 ||
       (p->tile > 0 && tile_bytes(p, p->tile) > TILE_LDS_MAX))
     return GP_ERR_ARG;
   int ndev = 0;
@@ -2407,7 +2627,10 @@ int gp_create(const gp_params* p, gp_ctx** out) {
       hipStreamCreateWithFlags(&ctx->st2, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->ev_gather, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&ctx->ev_gather, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->st3, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_efork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_edone, hipEventDisableTiming) != hipSuccess) {
     delete ctx;
     return GP_ERR_HIP;
   }
@@ -2441,6 +2664,7 @@ int gp_destroy(gp_ctx* ctx) {
   if (!ctx) return GP_ERR_ARG;
   hipStreamSynchronize(ctx->st);
   hipStreamSynchronize(ctx->st2);
+  hipStreamSynchronize(ctx->st3);
   void* bufs[] = {ctx->field, ctx->spec, ctx->pbuf[0], ctx->pbuf[1], ctx->perm, ctx->cell,
                   ctx->slot, ctx->cnt, ctx->off, ctx->tmp, ctx->sorted, ctx->pcount,
                   ctx->poff, ctx->scan_scr, ctx->mark, ctx->used, ctx->pos_of_id,
@@ -2448,6 +2672,10 @@ int gp_destroy(gp_ctx* ctx) {
                   ctx->pos4, ctx->csort, ctx->inlm, ctx->plist, ctx->inv, ctx->ccount, ctx->mlp_th, ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, ctx->mlp_wb,
                   ctx->mlp_bias, ctx->mlp_part, ctx->mlp_lpart, ctx->mlp_loss, ctx->seeds,
                   ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout, ctx->flpre,
+                  ctx->ep_keys[0], ctx->ep_keys[1], ctx->ep_vals[0], ctx->ep_vals[1],
+                  ctx->ep_pid[0], ctx->ep_pid[1], ctx->ep_slot[0], ctx->ep_slot[1],
+                  ctx->ep_pos[0], ctx->ep_pos[1],
+                  ctx->ep_res, ctx->ep_merge,
                   ctx->cs_rec, ctx->cs_buf, ctx->cs_cnt, ctx->cs_cycle, ctx->cs_dev};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -2462,6 +2690,10 @@ int gp_destroy(gp_ctx* ctx) {
   hipEventDestroy(ctx->ev_fork);
   hipEventDestroy(ctx->ev_join);
   hipEventDestroy(ctx->ev_gather);
+  hipStreamSynchronize(ctx->st3);
+  hipEventDestroy(ctx->ev_efork);
+  hipEventDestroy(ctx->ev_edone);
+  hipStreamDestroy(ctx->st3);
   hipStreamDestroy(ctx->st2);
   hipStreamDestroy(ctx->st);
   delete ctx;
@@ -2611,10 +2843,17 @@ int gp_set_species(gp_ctx* ctx, int nspec, const double* rows) {
 int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* species) {
   if (!ctx || np < 1 || !state || !species) return GP_ERR_ARG;
   hipStreamSynchronize(ctx->st);
+  hipStreamSynchronize(ctx->st3);
   drop_graph(ctx);
   ctx->contacts_valid = false;
   ctx->lists_sized = false;
   ctx->pipe_prev = false;
+  // pairs listed for solving ahead name particles of the old set: lists
+  // and tables emptied together (a table entry names a list position)
+  if (ctx->d_cnt) HIPCHK(hipMemset(ctx->d_cnt + 13, 0, 2 * sizeof(unsigned long long)));
+  for (int b = 0; b < 2; ++b)
+    if (ctx->ep_keys[b])
+      HIPCHK(hipMemset(ctx->ep_keys[b], 0, (size_t)ctx->ep_tab * sizeof(unsigned long long)));
   if (!ctx->spec) {
     ctx->err = "gp_set_species must precede gp_set_particles";
     return GP_ERR_STATE;
@@ -2655,6 +2894,9 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   if ((rc = dalloc(ctx, &ctx->plist, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->inv, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->ccount, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->ep_pos[0], (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->ep_pos[1], (size_t)np))) return rc;
+  ctx->ep_pos_valid[0] = ctx->ep_pos_valid[1] = false;
   HIPCHK(hipMemset(ctx->inlm, 0, (size_t)np * sizeof(int)));
   HIPCHK(hipMemset(ctx->ccount, 0, (size_t)np * sizeof(int)));
   if (ctx->prm.tile > 0) {
@@ -2919,6 +3161,9 @@ static void launch_advance_kernel(gp_ctx* ctx, hipStream_t st, const int* perm, 
   int* cell = bin ? ctx->cell : nullptr;
   const int xcd = ctx->prm.xcd_remap;
   ctx->flpre_valid = false;
+  // reference index -> slot of the new state, for solving ahead
+  int* posid = ctx->ep_pos[1 - ctx->cur];
+  ctx->ep_pos_valid[1 - ctx->cur] = posid != nullptr;
   if (ctx->bg.tile && perm == ctx->trav) {
     // the fluid at every particle first, one workgroup per field tile (the
     // visiting order is the tiles'), then the advance at full occupancy
@@ -2936,7 +3181,7 @@ static void launch_advance_kernel(gp_ctx* ctx, hipStream_t st, const int* perm, 
                      (const CT*)ctx->flpre, ctx->spec, ps, pd, perm, np, ctx->g, cell, ctx->slot, \
                      ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,                \
                      ctx->d_cnt + 0, ctx->flout, xcd, skip, (const int*)nullptr,                \
-                     (const unsigned long long*)nullptr, (const int*)nullptr)
+                     (const unsigned long long*)nullptr, (const int*)nullptr, posid)
     if (ctx->mixed) {
       if (ctx->prm.interp_scheme == 3) GP_GATHER(float, float, 4);
       else GP_GATHER(float, float, 5);
@@ -2961,7 +3206,7 @@ static void launch_advance_kernel(gp_ctx* ctx, hipStream_t st, const int* perm, 
   hipLaunchKernelGGL((k_advance<FT, VT, CT, S>), gr, bl, 0, st, ctx->sp, (const FT*)ctx->field, \
                      ctx->spec, ps, pd, perm, np, ctx->g, cell, ctx->slot, ctx->cnt, ctx->bg,    \
                      ctx->brick, ctx->bslot, bcnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd,  \
-                     skip)
+                     skip, posid)
   if (ctx->mixed) {
     if (ctx->prm.interp_scheme == 3) GP_ADV(float, float, 4);
     else GP_ADV(float, float, 5);
@@ -2988,7 +3233,7 @@ static void launch_advance_listed(gp_ctx* ctx, hipStream_t st) {
                      ctx->spec, ps, pd, (const int*)nullptr, np, ctx->g, ctx->cell, ctx->slot,  \
                      ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,                \
                      ctx->d_cnt + 0, ctx->flout, 0, (const int*)nullptr, ctx->plist,            \
-                     ctx->d_cnt + 10, ctx->inv)
+                     ctx->d_cnt + 10, ctx->inv, ctx->ep_pos[1 - ctx->cur])
     if (ctx->mixed) GP_ADVF(float);
     else GP_ADVF(double);
 #undef GP_ADVF
@@ -2998,7 +3243,8 @@ static void launch_advance_listed(gp_ctx* ctx, hipStream_t st) {
   hipLaunchKernelGGL((k_advance_listed<FT, VT, CT, S>), gr, bl, 0, st, ctx->sp,                 \
                      (const FT*)ctx->field, ctx->spec, ps, pd, ctx->plist, ctx->d_cnt + 10,     \
                      ctx->inv, np, ctx->g, ctx->cell, ctx->slot, ctx->cnt, ctx->bg,              \
-                     ctx->brick, ctx->bslot, bcnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout)
+                     ctx->brick, ctx->bslot, bcnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout,       \
+                     ctx->ep_pos[1 - ctx->cur])
   if (ctx->mixed) {
     if (ctx->prm.interp_scheme == 3) GP_ADVL(float, float, 4);
     else GP_ADVL(float, float, 5);
@@ -3064,7 +3310,18 @@ static int launch_sort(gp_ctx* ctx, bool offsets = true) {
     sc_flags.np = np;
   }
   sc_cnt.zc = ctx->d_cnt + 1;
-  sc_cnt.nzc = 11;
+  sc_cnt.nzc = 12;
+  if (ctx->ep_step) {
+    // this step's list and table for solving ahead: the scatter empties the
+    // table slots its last filling used, the ranking then zeroes the count
+    const int b = ctx->ep_par;
+    sc_flags.zl = ctx->ep_keys[b];
+    sc_flags.slot = ctx->ep_slot[b];
+    sc_flags.nslot = ctx->d_cnt + 13 + b;
+    sc_flags.nslot_cap = ctx->ep_cap;
+    sc_cnt.zc2 = ctx->d_cnt + 13 + b;
+    sc_cnt.nzc2 = 1;
+  }
   hipLaunchKernelGGL(k_scatter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->slot,
                      ctx->off, ctx->tmp, np, sc_flags);
   const int* grp = ctx->ngroups > 1 ? ps.grp : nullptr;
@@ -3123,6 +3380,30 @@ static int ensure_lists(gp_ctx* ctx, unsigned long long need) {
     if ((rc = dalloc(ctx, &ctx->lmwork, (size_t)cap))) return rc;
     if ((rc = dalloc(ctx, &ctx->seeds, (size_t)cap * 2))) return rc;
     ctx->work_cap = cap;
+  }
+  // solving ahead: lists of an eighth of the work capacity (the survivors
+  // of both filters are a small part of the bounding-sphere survivors; a
+  // pair past the capacity is simply solved in its own step), tables of
+  // twice that (a power of two), keys and list counts cleared
+  const unsigned long long ecap =
+      std::min(ctx->work_cap, std::max<unsigned long long>(ctx->work_cap / 8, 65536));
+  if (ecap > ctx->ep_cap) {
+    HIPCHK(hipStreamSynchronize(ctx->st3));
+    drop_graph(ctx);
+    unsigned long long t = 1024;
+    while (t < 2 * ecap) t <<= 1;
+    for (int b = 0; b < 2; ++b) {
+      if ((rc = dalloc(ctx, &ctx->ep_keys[b], (size_t)t))) return rc;
+      if ((rc = dalloc(ctx, &ctx->ep_vals[b], (size_t)t))) return rc;
+      if ((rc = dalloc(ctx, &ctx->ep_pid[b], (size_t)ecap * 2))) return rc;
+      if ((rc = dalloc(ctx, &ctx->ep_slot[b], (size_t)ecap))) return rc;
+      HIPCHK(hipMemset(ctx->ep_keys[b], 0, (size_t)t * sizeof(unsigned long long)));
+    }
+    if ((rc = dalloc(ctx, &ctx->ep_res, (size_t)ecap))) return rc;
+    if ((rc = dalloc(ctx, &ctx->ep_merge, (size_t)ecap))) return rc;
+    HIPCHK(hipMemset(ctx->d_cnt + 12, 0, 3 * sizeof(unsigned long long)));
+    ctx->ep_tab = t;
+    ctx->ep_cap = ecap;
   }
   const unsigned long long acap = std::max<unsigned long long>(ctx->work_cap / 4, 4096);
   if (acap > ctx->act_cap) {
@@ -3207,9 +3488,25 @@ static int launch_contacts_filter(gp_ctx* ctx, bool list_lm, bool clear_bins = f
     sc.zc = c;
     sc.nzc = 1;
   }
+  EarlyTab et;
+  if (ctx->ep_step) {
+    const int b = ctx->ep_par;
+    et.pkeys = ctx->ep_keys[1 - b];
+    et.pvals = ctx->ep_vals[1 - b];
+    et.nkeys = ctx->ep_keys[b];
+    et.nvals = ctx->ep_vals[b];
+    et.mask = ctx->ep_tab - 1;
+    et.pid = ctx->ep_pid[b];
+    et.slot = ctx->ep_slot[b];
+    et.npid = c + 13 + b;
+    et.merge = ctx->ep_merge;
+    et.nmerge = c + 12;
+    et.cap = ctx->ep_cap;
+  }
   hipLaunchKernelGGL(k_pw_filter<VT>, dim3(fgrid), dim3(BLOCK), 0, ctx->st, ps, ctx->spec,
                      ctx->work, c + 1, ctx->work_cap, ctx->prm.L, ctx->prm.prefilter >= 2 ? 1 : 0,
-                     ctx->lmwork, c + 7, list_lm ? ctx->inlm : nullptr, ctx->plist, c + 10, sc);
+                     ctx->lmwork, c + 7, list_lm ? ctx->inlm : nullptr, ctx->plist, c + 10, sc,
+                     et);
   HIPCHK(hipGetLastError());
   return GP_OK;
 }
@@ -3231,7 +3528,16 @@ static int launch_contacts_solve(gp_ctx* ctx) {
   hipLaunchKernelGGL(k_contact<VT>, dim3(ctx->lm_waves), dim3(LM_BLOCK), 0, ctx->st, ps, ctx->spec,
                      ctx->lmwork, c + 7, ctx->work_cap, ctx->prm.L, ctx->act, c + 2, ctx->act_cap,
                      c + 3, c + 4, ctx->d_err, net ? ctx->seeds : nullptr, ctx->gate, c + 9,
-                     cs.rec ? ctx->cs_dev : nullptr);
+                     cs.rec ? ctx->cs_dev : nullptr, EarlyIn{});
+  if (ctx->ep_step) {
+    // the pairs solved ahead, once that solve is done
+    HIPCHK(hipStreamWaitEvent(ctx->st, ctx->ev_edone, 0));
+    const unsigned mgrid =
+        (unsigned)std::min<unsigned long long>((ctx->ep_cap + BLOCK - 1) / BLOCK, 1024ull);
+    hipLaunchKernelGGL(k_merge_early, dim3(mgrid), dim3(BLOCK), 0, ctx->st, ctx->ep_merge,
+                       c + 12, ctx->ep_cap, ctx->ep_res, ctx->act, c + 2, ctx->act_cap, c + 3,
+                       c + 4, c + 7, ctx->d_err);
+  }
   mark(ctx, 3);
   hipLaunchKernelGGL(k_greedy_mark, dim3(GREEDY_GRID), dim3(BLOCK), 0, ctx->st, ctx->act, c + 2,
                      ctx->act_cap, ctx->mark);
@@ -3283,6 +3589,56 @@ static int launch_collide(gp_ctx* ctx) {
   return GP_OK;
 }
 
+// solving ahead, at the start of a pipelined step (every particle advanced):
+// the pairs the previous step's filter listed, solved from this step's state
+// on the third stream beside the step's sort and filters.  Only for the
+// plain centre-line solve without statistics records (a pair solved ahead
+// leaves no record, and the seeded solve keeps its own seeds).
+static bool early_on(const gp_ctx* ctx) {
+  // latency-bound solves only (fewer than one per lane of one wavefront per
+  // SIMD, the last count known): there the step waits on its slowest solve,
+  // and solving ahead takes that wait off the sort and filters.  Throughput-
+  // bound solves gain nothing from the overlap and pay the second pass.
+  return ctx->prm.overlap == 1 && ctx->prm.lm_seed == 0 && ctx->cs_fout2 <= 0 && ctx->ep_cap > 0 &&
+         !ctx->timing && ctx->last_nlm < (unsigned long long)ctx->nsimd * 64ull;
+}
+
+template <typename VT>
+static int launch_early(gp_ctx* ctx) {
+  ctx->ep_step = early_on(ctx);
+  if (!ctx->ep_step) return GP_OK;
+  const int b = 1 - ctx->ep_par;   // the previous step's list
+  auto ps = pstate<VT>(ctx, ctx->cur);
+  HIPCHK(hipEventRecord(ctx->ev_efork, ctx->st));
+  HIPCHK(hipStreamWaitEvent(ctx->st3, ctx->ev_efork, 0));
+  // the advance that made this state wrote its map (else built here)
+  int* pos = ctx->ep_pos[ctx->cur];
+  if (!ctx->ep_pos_valid[ctx->cur]) {
+    hipLaunchKernelGGL(k_pos_of_id<VT>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st3, ps, pos,
+                       ctx->np);
+    ctx->ep_pos_valid[ctx->cur] = true;
+  }
+  EarlyIn ein;
+  ein.pid = ctx->ep_pid[b];
+  ein.n = ctx->d_cnt + 13 + b;
+  ein.pos = pos;
+  ein.res = ctx->ep_res;
+  unsigned long long* c = ctx->d_cnt;
+  hipLaunchKernelGGL(k_contact<VT>, dim3(ctx->lm_waves), dim3(LM_BLOCK), 0, ctx->st3, ps,
+                     ctx->spec, ctx->lmwork, c + 7, ctx->ep_cap, ctx->prm.L, ctx->act, c + 2,
+                     ctx->act_cap, c + 3, c + 4, ctx->d_err, (const double*)nullptr, ctx->gate,
+                     c + 9, (const CStat*)nullptr, ein);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev_edone, ctx->st3));
+  return GP_OK;
+}
+
+// end of a step that solved ahead: the next one lists into the other parity
+static void end_early(gp_ctx* ctx) {
+  if (ctx->ep_step) ctx->ep_par ^= 1;
+  ctx->ep_step = false;
+}
+
 // COLLISION of the advanced particles, with the next ADV_PARTICLE_ELLP of
 // every particle outside the contact solves on the second stream beside the
 // solves (as inside pipelined gp_step), then the solves' particles advanced
@@ -3290,6 +3646,7 @@ static int launch_collide(gp_ctx* ctx) {
 template <typename VT>
 static int launch_collide_advance(gp_ctx* ctx) {
   int rc;
+  if ((rc = launch_early<VT>(ctx))) return rc;
   if (!ctx->pipe_prev)
     hipLaunchKernelGGL(k_fill<int>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st, ctx->inlm, 0,
                        (long long)ctx->np);
@@ -3314,6 +3671,7 @@ static int launch_collide_advance(gp_ctx* ctx) {
   ctx->cells_valid = true;
   ctx->contacts_valid = false;
   ctx->pipe_prev = true;
+  end_early(ctx);
   return GP_OK;
 }
 
@@ -3336,6 +3694,8 @@ static int launch_steps(gp_ctx* ctx, int k) {
   if ((rc = launch_advance<VT>(ctx, true))) return rc;
   for (int s = 0; s < k; ++s) {
     const bool next = s + 1 < k;
+    // every particle is advanced: the previous step's pairs solved ahead
+    if (pipe && (rc = launch_early<VT>(ctx))) return rc;
     // from the second step on, the sort clears the previous step's flags
     ctx->clear_plist = pipe && s > 0;
     rc = launch_sort<VT>(ctx, false);
@@ -3343,6 +3703,7 @@ static int launch_steps(gp_ctx* ctx, int k) {
     if (rc) return rc;
     if (!(pipe && next)) {
       if ((rc = launch_contacts<VT>(ctx))) return rc;
+      end_early(ctx);
       if (next) {
         hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0, ctx->st, c, 0ull, 1ll);
         if ((rc = launch_advance<VT>(ctx, true))) return rc;
@@ -3369,6 +3730,7 @@ static int launch_steps(gp_ctx* ctx, int k) {
     ctx->cur = 1 - ctx->cur;
     ctx->perm_identity = true;
     ctx->cells_valid = true;
+    end_early(ctx);
   }
   return GP_OK;
 }
@@ -3412,6 +3774,7 @@ static int capture_graph(gp_ctx* ctx) {
 // bound and more wavefronts hide each other's latency (2 per SIMD fit the
 // register budget; 4 queue behind them).  No effect on results.
 static void choose_lm_waves(gp_ctx* ctx, unsigned long long nlm) {
+  ctx->last_nlm = nlm;
   if (ctx->prm.lm_waves > 0) return;
   const unsigned long long lanes = (unsigned long long)ctx->nsimd * 64ull;
   ctx->lm_waves = ctx->nsimd * (nlm >= 4 * lanes ? 4 : nlm >= lanes ? 2 : 1);

@@ -109,7 +109,8 @@ contains
       p%brick = 0
       p%lm_seed = 0
       p%lm_waves = 0
-      p%overlap = 0
+      ! resident mode: each collision pass solves its pairs ahead
+      p%overlap = 1
       p%cell_rule = GPU_CELL_RULE
       p%tile = 0
       p%reserved = 0

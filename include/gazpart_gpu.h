@@ -110,10 +110,17 @@ typedef struct {
                           last known count of solves (one per SIMD while they
                           leave lanes idle, two or four per SIMD when each
                           lane has several) */
-  int overlap;         /* 1: in graph-mode gp_step, the next step's advance
-                          of the particles outside the contact solves runs on
-                          a second stream beside the solves (8 steps per
-                          captured graph); 0: steps run one after another.
+  int overlap;         /* 1: in graph-mode gp_step (and gp_collision_advance),
+                          the next step's advance of the particles outside
+                          the contact solves runs on a second stream beside
+                          the solves (8 steps per captured graph), and each
+                          step's contact pairs that were also the previous
+                          step's are solved ahead on a third stream, beside
+                          the step's sort and filters (while the solves are
+                          latency-bound: fewer than the lanes of one
+                          wavefront per SIMD; not with lm_seed = 1 or
+                          collision statistics); 2: the same without solving
+                          ahead; 0: steps run one after another.
                           No effect on results */
   int cell_rule;       /* collision cell size when dpmax <= 0.  0: the
                           reference rule DPMAX = 2 max(EMAJ), DXBOX = 2 DPMAX

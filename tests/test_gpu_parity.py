@@ -331,8 +331,8 @@ def test_host_handover_between_steps_changes_no_result(c1, prec):
             b.close()
 
 
-@pytest.mark.parametrize("mode,overlap,tile", [(1, 0, 0), (1, 1, 0), (2, 1, 0), (1, 1, 4),
-                                               (2, 1, 4)])
+@pytest.mark.parametrize("mode,overlap,tile", [(1, 0, 0), (1, 1, 0), (2, 1, 0), (2, 2, 0),
+                                               (1, 1, 4), (2, 1, 4)])
 @pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
 def test_graph_replays_equal_eager_steps(c1, prec, mode, overlap, tile):
     """The captured hipGraph (2 steps, or 8 pipelined steps with the next
@@ -342,7 +342,8 @@ def test_graph_replays_equal_eager_steps(c1, prec, mode, overlap, tile):
     contact solves per step).  Mode 2 launches the same pipelined steps
     directly, without capture.  With field tiles the fluid at every particle
     is gathered beside the solves and the solves' particles read it after
-    the collision pass."""
+    the collision pass.  overlap = 1 also solves each step's contact pairs
+    ahead, from the pairs of the step before (2: pipelined without it)."""
     d = np.load(GOLD + "golden_trajectory.npz")
     # the dense ensemble stops at step 53 in the reference (NaN drag, see
     # test_error_at_the_reference_step): keep it below that
