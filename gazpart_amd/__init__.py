@@ -110,7 +110,7 @@ class Stats(C.Structure):
                 ("t_sort_ms", C.c_double), ("t_contact_ms", C.c_double),
                 ("t_greedy_ms", C.c_double), ("ntimed", C.c_int64), ("nlm", C.c_int64),
                 ("seed_fallbacks", C.c_int64), ("lm_waves", C.c_int64),
-                ("t_solve_ms", C.c_double)]
+                ("t_solve_ms", C.c_double), ("nahead", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

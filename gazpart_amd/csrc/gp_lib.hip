@@ -87,12 +87,14 @@ struct WorkRec {     // a candidate pair that passed the prefilter
 // Solving ahead.  The contact solves of a step are, nearly all of them,
 // pairs that were solved in the step before too (near contacts persist for
 // many steps).  So every pair that survived the prefilters of step s is
-// listed by reference index, and at the start of step s+1 (its particles
-// all advanced) those pairs are solved again from the step-s+1 state, beside
-// that step's sort and prefilters; step s+1's filter then sends a pair found
-// in that list to the results instead of to its own solve.  A pair's solve
-// depends only on the two particles' state, so every result is the one the
-// step's own solve would give.
+// listed by the slots its two particles take in the next state (the sort of
+// step s fixes the visiting order of the next advance, and with it those
+// slots), and at the start of step s+1 (its particles all advanced) those
+// pairs are solved again from the step-s+1 state, beside that step's sort
+// and prefilters; step s+1's filter then sends a pair found in that list,
+// (home, near) slots equal, to the results instead of to its own solve.  A
+// pair's solve depends only on the two particles' state, so every result is
+// the one the step's own solve would give, whatever the list holds.
 struct EarlyRec {     // a contact solved ahead
   int i, j;           // storage slots of the step it was solved for
   int approach, nfev;
@@ -104,11 +106,10 @@ struct MergeRec {     // a surviving pair whose solve was done ahead
   int e, pad;         // its record among the results solved ahead
 };
 
-// k_contact's mode for solving ahead: pairs by reference index
+// k_contact's mode for solving ahead: pairs by slot
 struct EarlyIn {
-  const int* pid = nullptr;                       // home, near per pair
+  const int* pair = nullptr;                      // home, near slots per pair
   const unsigned long long* n = nullptr;
-  const int* pos = nullptr;                       // reference index -> slot
   EarlyRec* res = nullptr;                        // null: the step's own solve
 };
 
@@ -120,9 +121,10 @@ struct EarlyTab {
   unsigned long long* nkeys = nullptr;            // this step's (cleared)
   int* nvals = nullptr;
   unsigned long long mask = 0;                    // table size - 1
-  int* pid = nullptr;                             // this step's pairs
+  const int* inv = nullptr;                       // slot -> slot in the next state
+  int* pair = nullptr;                            // this step's pairs (next slots)
   int* slot = nullptr;                            // their table slots
-  unsigned long long* npid = nullptr;
+  unsigned long long* npair = nullptr;
   MergeRec* merge = nullptr;                      // found in the previous
   unsigned long long* nmerge = nullptr;
   unsigned long long cap = 0;
@@ -594,8 +596,7 @@ __device__ __forceinline__ void advance_one(
     const PState<VT>& src, const PState<VT>& dst, int i, int s, int np, const CellGrid& g,
     int* __restrict__ cell, int* __restrict__ slot, int* __restrict__ cnt, const BrickGrid& bg,
     int* __restrict__ brick, int* __restrict__ bslot, int* __restrict__ bcnt, bool grouped,
-    unsigned* err, unsigned long long* nwall, double* __restrict__ flout,
-    int* __restrict__ posid) {
+    unsigned* err, unsigned long long* nwall, double* __restrict__ flout) {
   double st[13];
   load_state(src, s, st);
   const int sid = src.sp[s];
@@ -616,7 +617,6 @@ __device__ __forceinline__ void advance_one(
   dst.sp[i] = sid;
   dst.id[i] = pid;
   dst.grp[i] = src.grp[s];
-  if (posid) posid[pid] = i;   // reference index -> slot of the new state
   int c = 0;
   if (cell) {
     bool oob;
@@ -643,13 +643,13 @@ __global__ __launch_bounds__(BLOCK) void k_advance(
     PState<VT> dst, const int* __restrict__ perm, int np, CellGrid g, int* __restrict__ cell,
     int* __restrict__ slot, int* __restrict__ cnt, BrickGrid bg, int* __restrict__ brick,
     int* __restrict__ bslot, int* __restrict__ bcnt, unsigned* err, unsigned long long* nwall,
-    double* __restrict__ flout, int xcd, const int* __restrict__ skip, int* __restrict__ posid) {
+    double* __restrict__ flout, int xcd, const int* __restrict__ skip) {
   const int i = block_index(xcd) * BLOCK + threadIdx.x;
   if (i >= np) return;
   const int s = perm ? perm[i] : i;
   if (skip && skip[s]) return;
   advance_one<VT, CT>(P, GlobalGather<FT, CT, S>{f, P.n, P.dx}, spec, src, dst, i, s, np, g, cell,
-                      slot, cnt, bg, brick, bslot, bcnt, true, err, nwall, flout, posid);
+                      slot, cnt, bg, brick, bslot, bcnt, true, err, nwall, flout);
 }
 
 // the advance of the field-tile mode, with the fluid at the particles
@@ -667,8 +667,7 @@ __global__ __launch_bounds__(BLOCK) void k_advance_fl(
     int* __restrict__ slot, int* __restrict__ cnt, BrickGrid bg, int* __restrict__ brick,
     int* __restrict__ bslot, int* __restrict__ bcnt, unsigned* err, unsigned long long* nwall,
     double* __restrict__ flout, int xcd, const int* __restrict__ skip,
-    const int* __restrict__ list, const unsigned long long* nlist, const int* __restrict__ inv,
-    int* __restrict__ posid) {
+    const int* __restrict__ list, const unsigned long long* nlist, const int* __restrict__ inv) {
   const bool listed = list != nullptr;
   const long long n = listed ? (long long)min(*nlist, (unsigned long long)np) : (long long)np;
   const long long stride = listed ? (long long)gridDim.x * blockDim.x : n;
@@ -685,7 +684,7 @@ __global__ __launch_bounds__(BLOCK) void k_advance_fl(
       if (skip && skip[s]) continue;
     }
     advance_one<VT, CT>(P, BufferGather<CT>{fl, np, i}, spec, src, dst, i, s, np, g, cell, slot,
-                        cnt, bg, brick, bslot, bcnt, !listed, err, nwall, flout, posid);
+                        cnt, bg, brick, bslot, bcnt, !listed, err, nwall, flout);
   }
 }
 
@@ -813,13 +812,13 @@ __global__ __launch_bounds__(ADVL_BLOCK) void k_advance_listed(
     const int* __restrict__ inv, int np, CellGrid g,
     int* __restrict__ cell, int* __restrict__ slot, int* __restrict__ cnt, BrickGrid bg,
     int* __restrict__ brick, int* __restrict__ bslot, int* __restrict__ bcnt, unsigned* err,
-    unsigned long long* nwall, double* __restrict__ flout, int* __restrict__ posid) {
+    unsigned long long* nwall, double* __restrict__ flout) {
   const long long nl = (long long)min(*nlist, (unsigned long long)np);
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nl;
        t += (long long)gridDim.x * blockDim.x) {
     const int s = list[t];
     advance_one<VT, CT>(P, GlobalGather<FT, CT, S>{f, P.n, P.dx}, spec, src, dst, inv[s], s, np, g,
-                        cell, slot, cnt, bg, brick, bslot, bcnt, false, err, nwall, flout, posid);
+                        cell, slot, cnt, bg, brick, bslot, bcnt, false, err, nwall, flout);
   }
 }
 
@@ -1147,8 +1146,11 @@ __global__ __launch_bounds__(BLOCK) void k_prefilter(
   }
 }
 
-// open-addressing table of pair keys (pid_home + 1) << 32 | (pid_near + 1),
+// open-addressing table of pair keys (home slot + 1) << 32 | (near slot + 1),
 // 0 = empty; sized at least twice the pairs it holds, so probes end
+__device__ __forceinline__ unsigned long long pair_key(int i, int j) {
+  return ((unsigned long long)(unsigned)(i + 1) << 32) | (unsigned long long)(unsigned)(j + 1);
+}
 __device__ __forceinline__ unsigned long long pair_hash(unsigned long long k) {
   k ^= k >> 33;
   k *= 0xff51afd7ed558ccdull;
@@ -1238,18 +1240,17 @@ __global__ __launch_bounds__(BLOCK) void k_pw_filter(
     // this step's solves when the previous step listed it
     bool solve = pass;
     if (et.nkeys) {
-      unsigned long long key = 0;
-      int found = -1;
+      int found = -1, ni = 0, nj = 0;
       if (pass) {
-        const int pi = p.id[r.i], pj = p.id[r.j];
-        key = ((unsigned long long)(pi + 1) << 32) | (unsigned long long)(pj + 1);
-        found = table_get(et.pkeys, et.pvals, et.mask, key);
+        found = table_get(et.pkeys, et.pvals, et.mask, pair_key(r.i, r.j));
+        ni = et.inv[r.i];
+        nj = et.inv[r.j];
       }
-      const unsigned long long e = wave_append(pass, et.npid);
+      const unsigned long long e = wave_append(pass, et.npair);
       if (pass && e < et.cap) {
-        et.pid[2 * e] = (int)(key >> 32) - 1;
-        et.pid[2 * e + 1] = (int)(key & 0xffffffffull) - 1;
-        et.slot[e] = table_put(et.nkeys, et.nvals, et.mask, key, (int)e);
+        et.pair[2 * e] = ni;
+        et.pair[2 * e + 1] = nj;
+        et.slot[e] = table_put(et.nkeys, et.nvals, et.mask, pair_key(ni, nj), (int)e);
       }
       const unsigned long long g = wave_append(found >= 0, et.nmerge);
       if (found >= 0 && g < et.cap) {
@@ -1330,7 +1331,7 @@ __global__ __launch_bounds__(LM_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) v
     unsigned long long* nfev_sum, unsigned long long* ncontact, unsigned* err,
     const double* __restrict__ seeds, SeedGate gate, unsigned long long* nfallback,
     const CStat* __restrict__ csp, EarlyIn ein) {
-  // ein.res: solving ahead (pairs by reference index, results to ein.res;
+  // ein.res: solving ahead (pairs by slot, results to ein.res;
   // the same solve in the same kernel, so the same bits)
   const bool ahead = ein.res != nullptr;
   const unsigned long long nw = min(ahead ? *ein.n : *nwork, cap);
@@ -1347,8 +1348,8 @@ __global__ __launch_bounds__(LM_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) v
       WorkRec r;
       if (ahead) {
         r.k = 0;
-        r.i = ein.pos[ein.pid[2 * w]];
-        r.j = ein.pos[ein.pid[2 * w + 1]];
+        r.i = ein.pair[2 * w];
+        r.j = ein.pair[2 * w + 1];
       } else {
         r = work[w];
       }
@@ -1969,6 +1970,8 @@ struct DevBuf {
   size_t n = 0;
 };
 
+constexpr int NCNT_HOST = 13;   // device counters mirrored to the host (16 allocated)
+
 struct gp_ctx {
   gp_params prm{};
   StepParams sp{};
@@ -2075,7 +2078,8 @@ struct gp_ctx {
   unsigned* d_err = nullptr;
   unsigned long long* d_cnt = nullptr;   // 0 nwall 1 nwork 2 nact 3 nfev 4 ncontact 5 ncoll 6 rounds 7 nlm 8 undecided
                                          // 9 seed fallbacks 10 particles in solves 11 candidates
-  unsigned long long* h_cnt = nullptr;   // pinned mirror
+                                         // 12 pairs solved ahead 13, 14 lists to solve ahead
+  unsigned long long* h_cnt = nullptr;   // pinned mirror of the first NCNT_HOST
   long long* h_total = nullptr;
   double* flout = nullptr;
   gp_stats stats{};
@@ -2112,7 +2116,7 @@ struct gp_ctx {
   // previous step's solve flags
   bool clear_plist = false;
   // solving ahead (pipelined steps, gp_params.overlap = 1): per step parity
-  // p, the filter lists its surviving pairs in ep_pid[p] (count d_cnt[13 +
+  // p, the filter lists its surviving pairs in ep_pair[p] (count d_cnt[13 +
   // p]) with the hash table ep_keys/ep_vals[p], and looks pairs up in the
   // other parity's, solved from this step's state by k_contact on st3 into
   // ep_res; ep_merge takes the pairs found (count d_cnt[12])
@@ -2120,10 +2124,8 @@ struct gp_ctx {
   hipEvent_t ev_efork = nullptr, ev_edone = nullptr;
   unsigned long long* ep_keys[2] = {nullptr, nullptr};
   int* ep_vals[2] = {nullptr, nullptr};
-  int* ep_pid[2] = {nullptr, nullptr};
+  int* ep_pair[2] = {nullptr, nullptr};
   int* ep_slot[2] = {nullptr, nullptr};
-  int* ep_pos[2] = {nullptr, nullptr};   // reference index -> slot, per buffer
-  bool ep_pos_valid[2] = {false, false};
   EarlyRec* ep_res = nullptr;
   MergeRec* ep_merge = nullptr;
   unsigned long long ep_tab = 0;   // table size (a power of two)
@@ -2673,8 +2675,7 @@ int gp_destroy(gp_ctx* ctx) {
                   ctx->mlp_bias, ctx->mlp_part, ctx->mlp_lpart, ctx->mlp_loss, ctx->seeds,
                   ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout, ctx->flpre,
                   ctx->ep_keys[0], ctx->ep_keys[1], ctx->ep_vals[0], ctx->ep_vals[1],
-                  ctx->ep_pid[0], ctx->ep_pid[1], ctx->ep_slot[0], ctx->ep_slot[1],
-                  ctx->ep_pos[0], ctx->ep_pos[1],
+                  ctx->ep_pair[0], ctx->ep_pair[1], ctx->ep_slot[0], ctx->ep_slot[1],
                   ctx->ep_res, ctx->ep_merge,
                   ctx->cs_rec, ctx->cs_buf, ctx->cs_cnt, ctx->cs_cycle, ctx->cs_dev};
   for (void* b : bufs)
@@ -2848,8 +2849,9 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   ctx->contacts_valid = false;
   ctx->lists_sized = false;
   ctx->pipe_prev = false;
-  // pairs listed for solving ahead name particles of the old set: lists
-  // and tables emptied together (a table entry names a list position)
+  // pairs listed for solving ahead name slots of the old set (a smaller
+  // set has fewer): lists and tables emptied together (a table entry names
+  // a list position)
   if (ctx->d_cnt) HIPCHK(hipMemset(ctx->d_cnt + 13, 0, 2 * sizeof(unsigned long long)));
   for (int b = 0; b < 2; ++b)
     if (ctx->ep_keys[b])
@@ -2894,9 +2896,6 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   if ((rc = dalloc(ctx, &ctx->plist, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->inv, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->ccount, (size_t)np))) return rc;
-  if ((rc = dalloc(ctx, &ctx->ep_pos[0], (size_t)np))) return rc;
-  if ((rc = dalloc(ctx, &ctx->ep_pos[1], (size_t)np))) return rc;
-  ctx->ep_pos_valid[0] = ctx->ep_pos_valid[1] = false;
   HIPCHK(hipMemset(ctx->inlm, 0, (size_t)np * sizeof(int)));
   HIPCHK(hipMemset(ctx->ccount, 0, (size_t)np * sizeof(int)));
   if (ctx->prm.tile > 0) {
@@ -3161,9 +3160,6 @@ static void launch_advance_kernel(gp_ctx* ctx, hipStream_t st, const int* perm, 
   int* cell = bin ? ctx->cell : nullptr;
   const int xcd = ctx->prm.xcd_remap;
   ctx->flpre_valid = false;
-  // reference index -> slot of the new state, for solving ahead
-  int* posid = ctx->ep_pos[1 - ctx->cur];
-  ctx->ep_pos_valid[1 - ctx->cur] = posid != nullptr;
   if (ctx->bg.tile && perm == ctx->trav) {
     // the fluid at every particle first, one workgroup per field tile (the
     // visiting order is the tiles'), then the advance at full occupancy
@@ -3181,7 +3177,7 @@ static void launch_advance_kernel(gp_ctx* ctx, hipStream_t st, const int* perm, 
                      (const CT*)ctx->flpre, ctx->spec, ps, pd, perm, np, ctx->g, cell, ctx->slot, \
                      ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,                \
                      ctx->d_cnt + 0, ctx->flout, xcd, skip, (const int*)nullptr,                \
-                     (const unsigned long long*)nullptr, (const int*)nullptr, posid)
+                     (const unsigned long long*)nullptr, (const int*)nullptr)
     if (ctx->mixed) {
       if (ctx->prm.interp_scheme == 3) GP_GATHER(float, float, 4);
       else GP_GATHER(float, float, 5);
@@ -3206,7 +3202,7 @@ static void launch_advance_kernel(gp_ctx* ctx, hipStream_t st, const int* perm, 
   hipLaunchKernelGGL((k_advance<FT, VT, CT, S>), gr, bl, 0, st, ctx->sp, (const FT*)ctx->field, \
                      ctx->spec, ps, pd, perm, np, ctx->g, cell, ctx->slot, ctx->cnt, ctx->bg,    \
                      ctx->brick, ctx->bslot, bcnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout, xcd,  \
-                     skip, posid)
+                     skip)
   if (ctx->mixed) {
     if (ctx->prm.interp_scheme == 3) GP_ADV(float, float, 4);
     else GP_ADV(float, float, 5);
@@ -3233,7 +3229,7 @@ static void launch_advance_listed(gp_ctx* ctx, hipStream_t st) {
                      ctx->spec, ps, pd, (const int*)nullptr, np, ctx->g, ctx->cell, ctx->slot,  \
                      ctx->cnt, ctx->bg, ctx->brick, ctx->bslot, bcnt, ctx->d_err,                \
                      ctx->d_cnt + 0, ctx->flout, 0, (const int*)nullptr, ctx->plist,            \
-                     ctx->d_cnt + 10, ctx->inv, ctx->ep_pos[1 - ctx->cur])
+                     ctx->d_cnt + 10, ctx->inv)
     if (ctx->mixed) GP_ADVF(float);
     else GP_ADVF(double);
 #undef GP_ADVF
@@ -3243,8 +3239,7 @@ static void launch_advance_listed(gp_ctx* ctx, hipStream_t st) {
   hipLaunchKernelGGL((k_advance_listed<FT, VT, CT, S>), gr, bl, 0, st, ctx->sp,                 \
                      (const FT*)ctx->field, ctx->spec, ps, pd, ctx->plist, ctx->d_cnt + 10,     \
                      ctx->inv, np, ctx->g, ctx->cell, ctx->slot, ctx->cnt, ctx->bg,              \
-                     ctx->brick, ctx->bslot, bcnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout,       \
-                     ctx->ep_pos[1 - ctx->cur])
+                     ctx->brick, ctx->bslot, bcnt, ctx->d_err, ctx->d_cnt + 0, ctx->flout)
   if (ctx->mixed) {
     if (ctx->prm.interp_scheme == 3) GP_ADVL(float, float, 4);
     else GP_ADVL(float, float, 5);
@@ -3395,7 +3390,7 @@ static int ensure_lists(gp_ctx* ctx, unsigned long long need) {
     for (int b = 0; b < 2; ++b) {
       if ((rc = dalloc(ctx, &ctx->ep_keys[b], (size_t)t))) return rc;
       if ((rc = dalloc(ctx, &ctx->ep_vals[b], (size_t)t))) return rc;
-      if ((rc = dalloc(ctx, &ctx->ep_pid[b], (size_t)ecap * 2))) return rc;
+      if ((rc = dalloc(ctx, &ctx->ep_pair[b], (size_t)ecap * 2))) return rc;
       if ((rc = dalloc(ctx, &ctx->ep_slot[b], (size_t)ecap))) return rc;
       HIPCHK(hipMemset(ctx->ep_keys[b], 0, (size_t)t * sizeof(unsigned long long)));
     }
@@ -3496,9 +3491,10 @@ static int launch_contacts_filter(gp_ctx* ctx, bool list_lm, bool clear_bins = f
     et.nkeys = ctx->ep_keys[b];
     et.nvals = ctx->ep_vals[b];
     et.mask = ctx->ep_tab - 1;
-    et.pid = ctx->ep_pid[b];
+    et.inv = ctx->inv;
+    et.pair = ctx->ep_pair[b];
     et.slot = ctx->ep_slot[b];
-    et.npid = c + 13 + b;
+    et.npair = c + 13 + b;
     et.merge = ctx->ep_merge;
     et.nmerge = c + 12;
     et.cap = ctx->ep_cap;
@@ -3611,17 +3607,9 @@ static int launch_early(gp_ctx* ctx) {
   auto ps = pstate<VT>(ctx, ctx->cur);
   HIPCHK(hipEventRecord(ctx->ev_efork, ctx->st));
   HIPCHK(hipStreamWaitEvent(ctx->st3, ctx->ev_efork, 0));
-  // the advance that made this state wrote its map (else built here)
-  int* pos = ctx->ep_pos[ctx->cur];
-  if (!ctx->ep_pos_valid[ctx->cur]) {
-    hipLaunchKernelGGL(k_pos_of_id<VT>, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st3, ps, pos,
-                       ctx->np);
-    ctx->ep_pos_valid[ctx->cur] = true;
-  }
   EarlyIn ein;
-  ein.pid = ctx->ep_pid[b];
+  ein.pair = ctx->ep_pair[b];
   ein.n = ctx->d_cnt + 13 + b;
-  ein.pos = pos;
   ein.res = ctx->ep_res;
   unsigned long long* c = ctx->d_cnt;
   hipLaunchKernelGGL(k_contact<VT>, dim3(ctx->lm_waves), dim3(LM_BLOCK), 0, ctx->st3, ps,
@@ -3801,7 +3789,7 @@ static int graph_steps(gp_ctx* ctx, int nsteps) {
     const unsigned long long need = 2ull * (unsigned long long)ctx->last_nclose + 4096ull;
     if ((rc = ensure_lists(ctx, need))) return rc;
     // the solve's launch width is fixed from here on: from this step's count
-    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
+    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, NCNT_HOST * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     choose_lm_waves(ctx, ctx->h_cnt[7]);
@@ -3833,6 +3821,7 @@ static int graph_steps(gp_ctx* ctx, int nsteps) {
 static void accumulate_stats(gp_ctx* ctx) {
   // called after a stream sync
   unsigned long long* h = ctx->h_cnt;
+  ctx->stats.nahead = (int64_t)std::min(h[12], ctx->ep_cap);
   ctx->stats.nwall = (int64_t)h[0];
   ctx->stats.nprefilter = (int64_t)h[1];
   ctx->stats.nlm = (int64_t)h[7];
@@ -3866,7 +3855,7 @@ int gp_adv_particle_ellp(gp_ctx* ctx, int ncycle) {
                      1ll);
   rc = ctx->mixed ? launch_advance<float>(ctx, true) : launch_advance<double>(ctx, true);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, NCNT_HOST * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, ctx->st));
   rc = check_err(ctx);
   ctx->stats.nwall = (int64_t)ctx->h_cnt[0];
@@ -4046,7 +4035,7 @@ int gp_collision(gp_ctx* ctx, int ncycle, double time) {
                        (long long)ncycle, 1ll);
   rc = ctx->mixed ? launch_collide<float>(ctx) : launch_collide<double>(ctx);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, NCNT_HOST * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, ctx->st));
   rc = check_err(ctx);
   accumulate_stats(ctx);
@@ -4068,7 +4057,7 @@ int gp_collision_advance(gp_ctx* ctx, int ncycle, double time, int ncycle_next) 
     if (rc) return rc;
     const unsigned long long need = 2ull * (unsigned long long)ctx->last_nclose + 4096ull;
     if ((rc = ensure_lists(ctx, need))) return rc;
-    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
+    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, NCNT_HOST * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     choose_lm_waves(ctx, ctx->h_cnt[7]);
@@ -4079,7 +4068,7 @@ int gp_collision_advance(gp_ctx* ctx, int ncycle, double time, int ncycle_next) 
   }
   rc = ctx->mixed ? launch_collide_advance<float>(ctx) : launch_collide_advance<double>(ctx);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, NCNT_HOST * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, ctx->st));
   return GP_OK;
 }
@@ -4165,7 +4154,7 @@ int gp_step(gp_ctx* ctx, int nsteps) {
   if (ctx->prm.graph) {
     rc = ctx->mixed ? graph_steps<float>(ctx, nsteps) : graph_steps<double>(ctx, nsteps);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
+    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, NCNT_HOST * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, ctx->st));
     return GP_OK;
   }
@@ -4177,7 +4166,7 @@ int gp_step(gp_ctx* ctx, int nsteps) {
     rc = ctx->mixed ? launch_collide<float>(ctx) : launch_collide<double>(ctx);
     if (rc) return rc;
   }
-  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, 12 * sizeof(unsigned long long),
+  HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, NCNT_HOST * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, ctx->st));
   return GP_OK;
 }

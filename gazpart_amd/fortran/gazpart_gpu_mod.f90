@@ -23,6 +23,7 @@ module gazpart_gpu
     real(c_double) :: t_advance_ms, t_sort_ms, t_contact_ms, t_greedy_ms
     integer(c_int64_t) :: ntimed, nlm, seed_fallbacks, lm_waves
     real(c_double) :: t_solve_ms
+    integer(c_int64_t) :: nahead
   end type gp_stats
 
   type, bind(C) :: gp_seed_report

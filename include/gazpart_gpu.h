@@ -173,6 +173,8 @@ typedef struct {
                            with; gp_params.lm_waves = 0 chooses it) */
   double t_solve_ms;    /* with timing: the contact-solve kernel alone (and
                            the network's seeds), part of t_contact_ms */
+  int64_t nahead;       /* of nlm, the pairs of the last collision pass that
+                           were solved ahead (gp_params.overlap = 1) */
 } gp_stats;
 
 /* lifetime ----------------------------------------------------------- */

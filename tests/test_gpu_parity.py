@@ -351,15 +351,22 @@ def test_graph_replays_equal_eager_steps(c1, prec, mode, overlap, tile):
         eager = G.Context.from_workload(wl, precision=prec, tile=tile)
         graph = G.Context.from_workload(wl, precision=prec, graph=mode, overlap=overlap,
                                         tile=tile)
+        ahead = 0
         for n in counts:
             eager.step(n)
             graph.step(n)
             a, b = eager.get_particles(), graph.get_particles()
             np.testing.assert_array_equal(a, b)
-            assert eager.stats()["ncollision"] == graph.stats()["ncollision"]
+            se, sg = eager.stats(), graph.stats()
+            assert se["ncollision"] == sg["ncollision"]
+            # every pair reached the solve, in the step or ahead of it
+            assert se["nlm"] == sg["nlm"] and 0 <= sg["nahead"] <= sg["nlm"]
+            ahead += sg["nahead"]
             # eager calls on the graph context between replays
             graph.collision_detection_connectivity()
             graph.sync()
+        # the pairs of one pass are nearly all the pairs of the pass before
+        assert (ahead > 0) == (overlap == 1)
         graph.adv_particle_ellp(1)
         graph.collision(1)
         eager.adv_particle_ellp(1)
