@@ -701,6 +701,9 @@ def main():
         "speedup_mixed_over_fp64": round(m["updates_per_s"] / f["updates_per_s"], 3),
         "stage_ms": stage(m["stats"]),
         "per_step": m["stats"]["per_step"],
+        # the last timed step's contact solves and, of them, the pairs solved
+        # ahead beside its sort and filters (pipelined steps, overlap = 1)
+        "solved_ahead": {"pairs": m["stats"]["nahead"], "of_solves": m["stats"]["nlm"]},
         # not the headline: particles uploaded from and downloaded to host
         # memory around every step (PCIe-inclusive), one step per call
         "host_handover": None if m["stats"]["host_handover_ms_per_step"] is None else {
