@@ -928,6 +928,7 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
                                                 int* __restrict__ sorted,
                                                 float4* __restrict__ pos4,
                                                 int* __restrict__ csort,
+                                                int* __restrict__ gsort,
                                                 int* __restrict__ inv, int np, int xcd,
                                                 SideClear sc) {
   side_clear(sc);
@@ -953,16 +954,19 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
   // members are contiguous here
   pos4[o + r] = make_float4((float)px[i], (float)py[i], (float)pz[i], rm[i]);
   csort[o + r] = c;
+  // and of the ensemble: the member ranges of one ensemble are then found
+  // without a dependent load per member
+  if (grp) gsort[o + r] = grp[i];
 }
 
-// members of cell range [o, o+m) (sorted by ensemble) in ensemble g
-__device__ __forceinline__ void group_range(const int* __restrict__ sorted,
-                                            const int* __restrict__ grp, int o, int m, int g,
+// members of cell range [o, o+m) (sorted by ensemble) in ensemble g, from
+// the ensemble of every sorted position
+__device__ __forceinline__ void group_range(const int* __restrict__ gsort, int o, int m, int g,
                                             int* lo, int* hi) {
   int a = o, b = o + m;
-  while (a < b && grp[sorted[a]] < g) ++a;
+  while (a < b && gsort[a] < g) ++a;
   int e = a;
-  while (e < b && grp[sorted[e]] == g) ++e;
+  while (e < b && gsort[e] == g) ++e;
   *lo = a;
   *hi = e;
 }
@@ -972,7 +976,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair_count(CellGrid g, const int* __r
                                                       const int* __restrict__ cell,
                                                       const int* __restrict__ off,
                                                       const int* __restrict__ cnt,
-                                                      const int* __restrict__ grp,
+                                                      const int* __restrict__ gsort,
                                                       int* __restrict__ pc, int nsorted) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nsorted) return;
@@ -981,17 +985,17 @@ __global__ __launch_bounds__(BLOCK) void k_pair_count(CellGrid g, const int* __r
   int nb[13];
   half_shell(g, c, nb);
   int k;
-  if (!grp) {
+  if (!gsort) {
     k = cnt[c] - 1 - (p - off[c]);
 #pragma unroll
     for (int v = 0; v < 13; ++v) k += cnt[nb[v]];
   } else {
-    const int gr = grp[hi];
+    const int gr = gsort[p];
     int lo, up;
-    group_range(sorted, grp, off[c], cnt[c], gr, &lo, &up);
+    group_range(gsort, off[c], cnt[c], gr, &lo, &up);
     k = up - 1 - p;
     for (int v = 0; v < 13; ++v) {
-      group_range(sorted, grp, off[nb[v]], cnt[nb[v]], gr, &lo, &up);
+      group_range(gsort, off[nb[v]], cnt[nb[v]], gr, &lo, &up);
       k += up - lo;
     }
   }
@@ -1005,7 +1009,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair_emit(CellGrid g, const int* __re
                                                      const int* __restrict__ cnt,
                                                      const long long* __restrict__ poff,
                                                      const int* __restrict__ id,
-                                                     const int* __restrict__ grp,
+                                                     const int* __restrict__ gsort,
                                                      int* __restrict__ home, int* __restrict__ nr,
                                                      int nsorted) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1015,15 +1019,15 @@ __global__ __launch_bounds__(BLOCK) void k_pair_emit(CellGrid g, const int* __re
   const int h = id[hi] + 1;
   long long k = poff[p];
   int lo = off[c], up = off[c] + cnt[c];
-  const int gr = grp ? grp[hi] : 0;
-  if (grp) group_range(sorted, grp, off[c], cnt[c], gr, &lo, &up);
+  const int gr = gsort ? gsort[p] : 0;
+  if (gsort) group_range(gsort, off[c], cnt[c], gr, &lo, &up);
   for (int t = p + 1; t < up; ++t) { home[k] = h; nr[k] = id[sorted[t]] + 1; ++k; }
   int nb[13];
   half_shell(g, c, nb);
   for (int v = 0; v < 13; ++v) {
     lo = off[nb[v]];
     up = lo + cnt[nb[v]];
-    if (grp) group_range(sorted, grp, off[nb[v]], cnt[nb[v]], gr, &lo, &up);
+    if (gsort) group_range(gsort, off[nb[v]], cnt[nb[v]], gr, &lo, &up);
     for (int t = lo; t < up; ++t) { home[k] = h; nr[k] = id[sorted[t]] + 1; ++k; }
   }
 }
@@ -1058,10 +1062,13 @@ __device__ __forceinline__ void min_image(const PState<VT>& p, int i, int j, dou
 // LDS overflow, in dense suspensions, appends directly.
 constexpr int PF_STAGE = 1024;
 
+// GROUPED: several ensembles (gsort given), a variant of its own so that
+// the plain test keeps its registers
+template <bool GROUPED>
 __global__ __launch_bounds__(BLOCK) void k_prefilter(
     CellGrid g, const float4* __restrict__ pos4, const int* __restrict__ sorted,
     const int* __restrict__ csort, const int* __restrict__ off, const int* __restrict__ cnt,
-    const int* __restrict__ grp, double L, float tol, int use_filter, WorkRec* __restrict__ work,
+    const int* __restrict__ gsort, double L, float tol, int use_filter, WorkRec* __restrict__ work,
     unsigned long long* nwork, unsigned long long cap, unsigned long long* nclose, unsigned* err,
     int nsorted, int xcd) {
   __shared__ WorkRec stage[PF_STAGE];
@@ -1101,32 +1108,44 @@ __global__ __launch_bounds__(BLOCK) void k_prefilter(
     // (cells hold ~0-2 members, so this puts nearly every load of the
     // thread in flight at once instead of one range after another)
     int slo[14], sup[14];
-    const int gr = grp ? grp[hi] : 0;
     // a cell's range is [off[c], off[c + 1]): two words of one line
-    {
-      int lo = off[c], up = off[c + 1];
-      if (grp) group_range(sorted, grp, lo, up - lo, gr, &lo, &up);
-      slo[0] = q + 1;
-      sup[0] = up;
-    }
+    slo[0] = q + 1;
+    sup[0] = off[c + 1];
     int nb[13];
     half_shell(g, c, nb);
 #pragma unroll
     for (int v = 0; v < 13; ++v) {
-      int lo = off[nb[v]], up = off[nb[v] + 1];
-      if (grp) group_range(sorted, grp, lo, up - lo, gr, &lo, &up);
-      slo[v + 1] = lo;
-      sup[v + 1] = up;
+      slo[v + 1] = off[nb[v]];
+      sup[v + 1] = off[nb[v] + 1];
     }
     float4 first[14];
 #pragma unroll
     for (int r = 0; r < 14; ++r)
       if (slo[r] < sup[r]) first[r] = pos4[slo[r]];
+    if (!GROUPED) {
 #pragma unroll
-    for (int r = 0; r < 14; ++r) {
-      if (slo[r] < sup[r]) {
-        test(first[r], slo[r]);
-        for (int t = slo[r] + 1; t < sup[r]; ++t) test(pos4[t], t);
+      for (int r = 0; r < 14; ++r) {
+        if (slo[r] < sup[r]) {
+          test(first[r], slo[r]);
+          for (int t = slo[r] + 1; t < sup[r]; ++t) test(pos4[t], t);
+        }
+      }
+    } else {
+      // several ensembles: a cell's members are sorted by ensemble, so the
+      // home's ensemble's members are those of its ensemble, in order; the
+      // ensemble of each range's first member is fetched with its position
+      const int gr = gsort[q];
+      int gfirst[14];
+#pragma unroll
+      for (int r = 0; r < 14; ++r)
+        if (slo[r] < sup[r]) gfirst[r] = gsort[slo[r]];
+#pragma unroll
+      for (int r = 0; r < 14; ++r) {
+        if (slo[r] < sup[r]) {
+          if (gfirst[r] == gr) test(first[r], slo[r]);
+          for (int t = slo[r] + 1; t < sup[r]; ++t)
+            if (gsort[t] == gr) test(pos4[t], t);
+        }
       }
     }
   }
@@ -2008,6 +2027,7 @@ struct gp_ctx {
   int* trav = nullptr;
   float4* pos4 = nullptr;       // (x, y, z, rmax) as float in cell (sorted) order
   int* csort = nullptr;         // cell of each sorted position
+  int* gsort = nullptr;         // ensemble of each sorted position (several ensembles)
   // pipelined steps (gp_params.overlap)
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -2671,7 +2691,7 @@ int gp_destroy(gp_ctx* ctx) {
                   ctx->slot, ctx->cnt, ctx->off, ctx->tmp, ctx->sorted, ctx->pcount,
                   ctx->poff, ctx->scan_scr, ctx->mark, ctx->used, ctx->pos_of_id,
                   ctx->work, ctx->lmwork, ctx->brick, ctx->bslot, ctx->bcnt, ctx->boff, ctx->trav,
-                  ctx->pos4, ctx->csort, ctx->inlm, ctx->plist, ctx->inv, ctx->ccount, ctx->mlp_th, ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, ctx->mlp_wb,
+                  ctx->pos4, ctx->csort, ctx->gsort, ctx->inlm, ctx->plist, ctx->inv, ctx->ccount, ctx->mlp_th, ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, ctx->mlp_wb,
                   ctx->mlp_bias, ctx->mlp_part, ctx->mlp_lpart, ctx->mlp_loss, ctx->seeds,
                   ctx->act, ctx->act_status, ctx->d_err, ctx->d_cnt, ctx->flout, ctx->flpre,
                   ctx->ep_keys[0], ctx->ep_keys[1], ctx->ep_vals[0], ctx->ep_vals[1],
@@ -2892,6 +2912,7 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   if ((rc = dalloc(ctx, &ctx->trav, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->pos4, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->csort, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->gsort, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->inlm, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->plist, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->inv, (size_t)np))) return rc;
@@ -3322,11 +3343,12 @@ static int launch_sort(gp_ctx* ctx, bool offsets = true) {
   const int* grp = ctx->ngroups > 1 ? ps.grp : nullptr;
   hipLaunchKernelGGL(k_rank, dim3(xcd_grid(ctx, np)), dim3(BLOCK), 0, ctx->st, ctx->cell,
                      ctx->off, ctx->cnt, ctx->tmp, ps.id, grp, ps.x, ps.y, ps.z, ps.rm,
-                     ctx->sorted, ctx->pos4, ctx->csort, ctx->use_bricks ? nullptr : ctx->inv, np,
-                     ctx->prm.xcd_remap, sc_cnt);
+                     ctx->sorted, ctx->pos4, ctx->csort, ctx->gsort,
+                     ctx->use_bricks ? nullptr : ctx->inv, np, ctx->prm.xcd_remap, sc_cnt);
   if (offsets) {
     hipLaunchKernelGGL(k_pair_count, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, g, ctx->sorted,
-                       ctx->cell, ctx->off, ctx->cnt, grp, ctx->pcount, np);
+                       ctx->cell, ctx->off, ctx->cnt, grp ? ctx->gsort : nullptr, ctx->pcount,
+                       np);
     scan_exclusive<int, long long>(ctx->pcount, ctx->poff, np, (long long*)ctx->scan_scr, ctx->st);
   }
   if (ctx->use_bricks) {
@@ -3464,11 +3486,18 @@ static int launch_contacts_filter(gp_ctx* ctx, bool list_lm, bool clear_bins = f
   const int np = ctx->np;
   unsigned long long* c = ctx->d_cnt;
   auto ps = pstate<VT>(ctx, ctx->cur);
-  hipLaunchKernelGGL(k_prefilter, dim3(xcd_grid(ctx, np)), dim3(BLOCK), 0, ctx->st, ctx->g,
-                     ctx->pos4, ctx->sorted, ctx->csort, ctx->off, ctx->cnt,
-                     ctx->ngroups > 1 ? ps.grp : nullptr, ctx->prm.L, prefilter_tol(ctx),
-                     ctx->prm.prefilter, ctx->work, c + 1, ctx->work_cap, c + 11, ctx->d_err, np,
-                     ctx->prm.xcd_remap);
+  if (ctx->ngroups > 1)
+    hipLaunchKernelGGL(k_prefilter<true>, dim3(xcd_grid(ctx, np)), dim3(BLOCK), 0, ctx->st,
+                       ctx->g, ctx->pos4, ctx->sorted, ctx->csort, ctx->off, ctx->cnt,
+                       ctx->gsort, ctx->prm.L, prefilter_tol(ctx), ctx->prm.prefilter,
+                       ctx->work, c + 1, ctx->work_cap, c + 11, ctx->d_err, np,
+                       ctx->prm.xcd_remap);
+  else
+    hipLaunchKernelGGL(k_prefilter<false>, dim3(xcd_grid(ctx, np)), dim3(BLOCK), 0, ctx->st,
+                       ctx->g, ctx->pos4, ctx->sorted, ctx->csort, ctx->off, ctx->cnt,
+                       (const int*)nullptr, ctx->prm.L, prefilter_tol(ctx), ctx->prm.prefilter,
+                       ctx->work, c + 1, ctx->work_cap, c + 11, ctx->d_err, np,
+                       ctx->prm.xcd_remap);
   // the survivor count stays on the device: a fixed grid, grid-stride inside
   const unsigned fgrid = (unsigned)std::min<unsigned long long>((ctx->work_cap + BLOCK - 1) / BLOCK,
                                                                  2048ull);
@@ -3881,11 +3910,9 @@ int gp_collision_detection(gp_ctx* ctx, int64_t nclosemax, int* home, int* near_
   HIPCHK(hipMalloc((void**)&dh, (size_t)total * sizeof(int)));
   HIPCHK(hipMalloc((void**)&dn, (size_t)total * sizeof(int)));
   const int* id = ctx->mixed ? pstate<float>(ctx, ctx->cur).id : pstate<double>(ctx, ctx->cur).id;
-  const int* gp =
-      ctx->mixed ? pstate<float>(ctx, ctx->cur).grp : pstate<double>(ctx, ctx->cur).grp;
   hipLaunchKernelGGL(k_pair_emit, dim3(nblk(ctx->np)), dim3(BLOCK), 0, ctx->st, ctx->g,
                      ctx->sorted, ctx->cell, ctx->off, ctx->cnt, ctx->poff, id,
-                     ctx->ngroups > 1 ? gp : nullptr, dh, dn, ctx->np);
+                     ctx->ngroups > 1 ? ctx->gsort : nullptr, dh, dn, ctx->np);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(home, dh, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipMemcpyAsync(near_, dn, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, ctx->st));

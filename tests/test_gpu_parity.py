@@ -378,6 +378,42 @@ def test_graph_replays_equal_eager_steps(c1, prec, mode, overlap, tile):
         graph.close()
 
 
+def ensemble_alone(wl, a, b):
+    """The particles [a, b) of an ensemble workload as a workload of their
+    own (same field, species and collision grid)."""
+    return W.Workload(name="e", n=wl.n, L=wl.L, k0=wl.k0, seed=0, np_=b - a, scheme=wl.scheme,
+                      steps=0, visc=wl.visc, rhof=wl.rhof, dtime=wl.dtime, field=wl.field,
+                      umax=wl.umax, species=wl.species, species_id=wl.species_id[a:b],
+                      state=wl.state[:, a:b], dpmax=wl.dpmax)
+
+
+def test_ensemble_pair_lists_are_each_ensembles_own():
+    """HOME/NEAR of three ensembles sharing one collision grid (a cell's
+    members of other ensembles skipped): no pair crosses ensembles, and each
+    ensemble's pairs, in order, are the list that a context of that
+    ensemble alone gives (collision_detection_connectivity.f90:145-207 per
+    ensemble), after steps have mixed the ensembles within the cells."""
+    wl = W.build_ensembles("E3", 32, [700, 1500, 400], k0=3.0, seed=5, req=0.05)
+    grp = np.asarray(wl.meta["group"])
+    both = G.Context.from_workload(wl, precision=G.PREC_FP64)
+    both.set_groups(grp)
+    both.step(3)
+    state = both.get_particles()
+    home, near = both.collision_detection_connectivity()
+    assert len(home) > 100
+    assert np.all(grp[home - 1] == grp[near - 1])
+    for a, b in wl.meta["ranges"]:
+        sub = ensemble_alone(wl, a, b)
+        one = G.Context.from_workload(sub, precision=G.PREC_FP64)
+        one.set_particles(state[:, a:b], wl.species_id[a:b])
+        h1, n1 = one.collision_detection_connectivity()
+        sel = (home - 1 >= a) & (home - 1 < b)
+        np.testing.assert_array_equal(home[sel], h1 + a)
+        np.testing.assert_array_equal(near[sel], n1 + a)
+        one.close()
+    both.close()
+
+
 @pytest.mark.parametrize("prec", [G.PREC_FP64, G.PREC_MIXED])
 def test_ensembles_share_a_field_and_never_collide(prec):
     """Three ensembles (different aspect ratios, sizes and counts) in one
@@ -391,10 +427,7 @@ def test_ensembles_share_a_field_and_never_collide(prec):
     got = both.get_particles()
     ncoll = 0
     for e, (a, b) in enumerate(wl.meta["ranges"]):
-        sub = W.Workload(name="e", n=wl.n, L=wl.L, k0=wl.k0, seed=0, np_=b - a, scheme=wl.scheme,
-                         steps=0, visc=wl.visc, rhof=wl.rhof, dtime=wl.dtime, field=wl.field,
-                         umax=wl.umax, species=wl.species, species_id=wl.species_id[a:b],
-                         state=wl.state[:, a:b], dpmax=wl.dpmax)
+        sub = ensemble_alone(wl, a, b)
         one = G.Context.from_workload(sub, precision=prec)
         one.step(8)
         ncoll += one.stats()["ncollision"]
