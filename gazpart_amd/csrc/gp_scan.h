@@ -101,10 +101,13 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_sums(const int* __restrict_
   }
 }
 
+// tpre: the tiles' exclusive prefixes, when scanned beforehand (else each
+// tile sums the totals of the tiles before it)
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_from_sums(const int* __restrict__ in,
                                                                int* __restrict__ out,
                                                                const int* __restrict__ sums,
-                                                               int n, int ntiles) {
+                                                               int n, int ntiles,
+                                                               const int* __restrict__ tpre) {
   __shared__ int wsum[SCAN_BLOCK / 64];
   __shared__ int s_excl;
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -120,15 +123,19 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_from_sums(const int* __rest
     tot += v[k];
   }
   // exclusive prefix of this tile: the totals of tiles [0, blockIdx.x)
-  int pre = 0;
-  for (int t = threadIdx.x; t < (int)blockIdx.x; t += SCAN_BLOCK) pre += sums[t];
-  for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o);
-  if (lane == 0) wsum[wid] = pre;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int e = 0;
-    for (int w = 0; w < SCAN_BLOCK / 64; ++w) e += wsum[w];
-    s_excl = e;
+  if (tpre) {
+    if (threadIdx.x == 0) s_excl = tpre[blockIdx.x];
+  } else {
+    int pre = 0;
+    for (int t = threadIdx.x; t < (int)blockIdx.x; t += SCAN_BLOCK) pre += sums[t];
+    for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o);
+    if (lane == 0) wsum[wid] = pre;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int e = 0;
+      for (int w = 0; w < SCAN_BLOCK / 64; ++w) e += wsum[w];
+      s_excl = e;
+    }
   }
   __syncthreads();
   const int incl = wave_incl_scan(tot);
@@ -200,9 +207,10 @@ __global__ __launch_bounds__(SCAN1_BLOCK) void k_scan_onewg(const int* __restric
   if (threadIdx.x == 0) out[n] = carry;
 }
 
-// out[i] = sum_{k<i} in[k] for i < n, out[n] = total; sums holds one int
-// per tile of SCAN_TILE inputs
+// out[i] = sum_{k<i} in[k] for i < n, out[n] = total; sums holds two ints
+// per tile of SCAN_TILE inputs, plus one
 inline int64_t scan2_sums(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+constexpr int SCAN3_MIN_TILES = 2048;
 
 inline void scan_exclusive_int(const int* in, int* out, int n, int* sums, hipStream_t st) {
   const int ntiles = (int)scan2_sums(n);
@@ -215,8 +223,17 @@ inline void scan_exclusive_int(const int* in, int* out, int n, int* sums, hipStr
     return;
   }
   hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, st, in, sums, n);
+  // many tiles: their totals scanned in one workgroup first, so each tile
+  // reads its prefix instead of summing every total before it (a walk that
+  // grows with the square of the tile count)
+  const int* tpre = nullptr;
+  if (ntiles >= SCAN3_MIN_TILES && ntiles <= SCAN1_MAX) {
+    hipLaunchKernelGGL(k_scan_onewg, dim3(1), dim3(SCAN1_BLOCK), 0, st, sums, sums + ntiles,
+                       ntiles);
+    tpre = sums + ntiles;
+  }
   hipLaunchKernelGGL(k_scan_from_sums, dim3((unsigned)ntiles), dim3(SCAN_BLOCK), 0, st, in, out,
-                     sums, n, ntiles);
+                     sums, n, ntiles, tpre);
 }
 
 // scratch elements needed by scan_exclusive for n inputs

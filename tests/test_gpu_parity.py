@@ -513,17 +513,21 @@ def test_error_out_of_grid_and_interp_range(c1):
     assert e.value.code == 10
 
 
-@pytest.mark.parametrize("dpmax", [0.9, 0.06])
+@pytest.mark.parametrize("dpmax", [0.9, 0.06, 0.018])
 def test_cell_list_bitexact_short_and_long_scans(dpmax):
     """The cell-count scan runs in one workgroup for grids of at most 32768
-    cells and as two launches above: both give the reference's HOME/NEAR
-    lists (C oracle) bit for bit (dpmax 0.9: 27 cells; 0.06: 140608)."""
-    wl = W.c1(np_=2000)
+    cells, as two launches above, and with the tile totals scanned in a
+    launch of their own from 2048 tiles of 2048 cells: each gives the
+    reference's HOME/NEAR lists (C oracle) bit for bit (dpmax 0.9: 27
+    cells; 0.06: 140608; 0.018: 5.6M, cells narrower than C1's particles,
+    so the cell rule goes unchecked on both sides)."""
+    wl = W.c1(np_=2000 if dpmax > 0.05 else 20000)
     wl.dpmax = dpmax
     p = O.params(wl)
     g = O.collision_grid(p, dpmax)
     assert (g.nbox <= 32768) == (dpmax > 0.5)
-    ctx = G.Context.from_workload(wl)
+    assert (g.nbox >= 2048 * 2048) == (dpmax < 0.05)
+    ctx = G.Context.from_workload(wl, cell_rule=2 if dpmax < 0.05 else 0)
     home, near = ctx.collision_detection_connectivity()
     con, ncon = O.connectivity(g)
     st = wl.state
