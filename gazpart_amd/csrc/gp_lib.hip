@@ -1118,11 +1118,11 @@ __global__ __launch_bounds__(BLOCK) void k_prefilter(
       slo[v + 1] = off[nb[v]];
       sup[v + 1] = off[nb[v] + 1];
     }
-    float4 first[14];
-#pragma unroll
-    for (int r = 0; r < 14; ++r)
-      if (slo[r] < sup[r]) first[r] = pos4[slo[r]];
     if (!GROUPED) {
+      float4 first[14];
+#pragma unroll
+      for (int r = 0; r < 14; ++r)
+        if (slo[r] < sup[r]) first[r] = pos4[slo[r]];
 #pragma unroll
       for (int r = 0; r < 14; ++r) {
         if (slo[r] < sup[r]) {
@@ -1132,21 +1132,35 @@ __global__ __launch_bounds__(BLOCK) void k_prefilter(
       }
     } else {
       // several ensembles: a cell's members are sorted by ensemble, so the
-      // home's ensemble's members are those of its ensemble, in order; the
-      // ensemble of each range's first member is fetched with its position
+      // home's ensemble's members form one run [glo, ghi) in each range,
+      // found member by member over the 14 ranges at once (the loads of one
+      // depth in flight together); positions are read for those runs only
       const int gr = gsort[q];
-      int gfirst[14];
-#pragma unroll
-      for (int r = 0; r < 14; ++r)
-        if (slo[r] < sup[r]) gfirst[r] = gsort[slo[r]];
+      int glo[14], ghi[14], len = 0;
 #pragma unroll
       for (int r = 0; r < 14; ++r) {
-        if (slo[r] < sup[r]) {
-          if (gfirst[r] == gr) test(first[r], slo[r]);
-          for (int t = slo[r] + 1; t < sup[r]; ++t)
-            if (gsort[t] == gr) test(pos4[t], t);
+        glo[r] = sup[r];
+        ghi[r] = sup[r];
+        len = max(len, sup[r] - slo[r]);
+      }
+      unsigned open = (1u << 14) - 1u;   // ranges whose run has not ended
+      for (int d = 0; d < len && open; ++d) {
+#pragma unroll
+        for (int r = 0; r < 14; ++r) {
+          const int t = slo[r] + d;
+          if (((open >> r) & 1u) && t < sup[r]) {
+            const int gt = gsort[t];
+            if (gt >= gr && glo[r] == sup[r]) glo[r] = t;
+            if (gt > gr) {
+              ghi[r] = t;
+              open &= ~(1u << r);
+            }
+          }
         }
       }
+#pragma unroll
+      for (int r = 0; r < 14; ++r)
+        for (int t = glo[r]; t < ghi[r]; ++t) test(pos4[t], t);
     }
   }
   unsigned long long nc = (unsigned long long)(k - ((long long)q << 32));
