@@ -139,7 +139,7 @@ def run_gpu(wl, precision, steps, warmup, G, groups=None, theta=None, graph=1, *
 
 def host_threads():
     """The host cores this process may use: its CPU affinity, capped by
-    OMP_NUM_THREADS where set (a GPU box's share of the host for one GPU)."""
+    OMP_NUM_THREADS where set (the host's share for one GPU)."""
     n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     cap = os.environ.get("OMP_NUM_THREADS", "")
     return min(n, int(cap)) if cap.isdigit() and int(cap) > 0 else n
@@ -584,7 +584,7 @@ def main():
         print(json.dumps({"calibrate_copy_bytes": 4 * n, "launches": 3}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is design-only)"}))
         sys.exit(2)
 
