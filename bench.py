@@ -22,7 +22,7 @@ roofline     = the advance (the HBM-bound gather; with field tiles the
                rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy
                calibration, the median over the dispatches of a profiled
                run (steady state: sorted particles), when a summary for
-               this configuration and kernel is committed
+               this configuration, kernel and visiting order is committed
 cpu_baseline = the FP64 CPU oracle (a restatement of the reference path,
                OpenMP over particles) on the host cores, one full step of
                the same workload (C2), or the compiled reference Fortran on
@@ -32,6 +32,11 @@ Other configurations (--config): C1 (64^3, 10k prolate spheroids), C2L4
 (C2 with lag4), C3 (512^3 field, 8M tri-axial), C4 (64 ensembles on one
 field, autotuned launch shapes), C5 (2M tri-axial at 5 % volume fraction,
 network-seeded contact solve trained on the GPU).
+
+Launch shape (results do not depend on it): C4 takes the fastest of the
+autotuner's table (visiting orders, solve widths, launch modes); the other
+configurations tune the launch mode and the visiting order (cell order or
+bricks of 4 cells), the FP64 form's order apart from the mixed form's.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C2]
 """
@@ -234,12 +239,17 @@ def advance_kernel(launch):
     return "k_gather_tiled+k_advance_fl" if launch.get("tile", 0) > 0 else "k_advance"
 
 
-def roofline(wl, res, scheme, cfg, launch):
-    kernel = advance_kernel(launch)
+def visiting_brick(launch):
+    """The brick size of a launch shape's visiting order (0: cell order)."""
+    return 0 if launch.get("tile", 0) > 0 else max(launch.get("brick", 0), 0)
+
+
+def roofline(wl, res, scheme, cfg, launch, launch_fp64):
     roof = {}
     for name, fb, srw in (("fp64", 8, 208), ("mixed", 4, 160)):
         if name not in res:
             continue
+        kernel = advance_kernel(launch_fp64 if name == "fp64" else launch)
         st = res[name]["stats"]
         t_adv = st["t_advance_ms"] / max(st["ntimed"], 1) * 1e-3
         byt = advance_bytes(wl.np_, wl.n, fb, 64 if scheme == 3 else 125, srw)
@@ -252,8 +262,11 @@ def roofline(wl, res, scheme, cfg, launch):
     if os.path.exists(tr_file):
         tr = json.load(open(tr_file))
         for name in roof:
-            # a committed measurement of this configuration's kernel(s)
-            if name in tr and tr.get("kernel", "k_advance") == kernel:
+            # a committed measurement of this configuration's kernel(s) in
+            # the same visiting order
+            lw = launch_fp64 if name == "fp64" else launch
+            if (name in tr and tr.get("kernel", "k_advance") == roof[name]["kernel"]
+                    and tr.get("brick", 0) == visiting_brick(lw)):
                 roof[name]["traffic"] = tr[name]
                 roof[name]["traffic_source"] = tr.get("source", os.path.relpath(tr_file, ROOT))
     return roof
@@ -544,7 +557,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--brick", type=int, default=0,
                     help="advance visiting order: bricks of BRICK^3 collision cells "
-                         "(0: the library default)")
+                         "(0: cell order, or bricks of 4 where the autotuner finds them "
+                         "faster, for each form separately)")
     ap.add_argument("--tile", type=int, default=0,
                     help="advance visiting order: field tiles of TILE^3 cells staged in LDS "
                          "(0: off; overrides --brick)")
@@ -606,6 +620,7 @@ def main():
 
     kw = {"brick": args.brick, "tile": args.tile, "lm_waves": args.lm_waves,
           "graph": args.graph}
+    kw_fp64 = None   # the FP64 form's launch shape when tuned apart
     extra = {}
     theta = None
     from gazpart_amd.autotune import tune
@@ -615,13 +630,23 @@ def main():
               "lm_waves": tr["best"]["lm_waves"], "graph": tr["best"]["graph"]}
         extra["autotune"] = tr
     elif args.graph == 0 and not args.no_tune:
-        # launch mode only (captured graph replays or direct launches), with
-        # the given visiting order and solve width
-        tr = tune(wl, G.PREC_MIXED, bricks=(args.brick,) if args.tile <= 0 else (),
+        # launch mode (captured graph replays or direct launches) and, unless
+        # given, visiting order (cell order or bricks of 4 cells), with the
+        # given solve width
+        bricks = ((args.brick, 4) if args.brick <= 0 else (args.brick,)) if args.tile <= 0 else ()
+        tr = tune(wl, G.PREC_MIXED, bricks=bricks,
                   tiles=(args.tile,) if args.tile > 0 else (), lm_waves=(args.lm_waves,),
                   steps=16, log=log)
         kw["graph"] = tr["best"]["graph"]
+        kw["brick"] = tr["best"]["brick"]
         extra["autotune"] = tr
+        if len(bricks) > 1:
+            # the FP64 form's advance moves twice the bytes and can set its
+            # step: its own visiting order, in the mixed form's launch mode
+            trf = tune(wl, G.PREC_FP64, bricks=bricks, tiles=(), lm_waves=(args.lm_waves,),
+                       graphs=(kw["graph"],), steps=8, log=log)
+            kw_fp64 = dict(kw, brick=trf["best"]["brick"])
+            extra["autotune_fp64"] = trf
     if cfg == "C5":
         nctx = G.Context(wl.n, wl.L, wl.visc, wl.rhof, wl.dtime)
         fit = G.train_seed_net(nctx, npairs=args.pairs, seed=2024, epochs=args.epochs)
@@ -639,14 +664,15 @@ def main():
         runs = [("fp64", G.PREC_FP64, {}), ("mixed_centre_seed", G.PREC_MIXED, {}),
                 ("mixed", G.PREC_MIXED, {"lm_seed": 1, "theta": theta})]
     for name, prec, ek in runs:
-        dt, st, state = run_gpu(wl, prec, args.steps, args.warmup, G, groups, **kw, **ek)
+        k = kw_fp64 if (prec == G.PREC_FP64 and kw_fp64) else kw
+        dt, st, state = run_gpu(wl, prec, args.steps, args.warmup, G, groups, **k, **ek)
         ms = 1e3 * dt / args.steps
         res[name] = dict(seconds=dt, ms_per_step=ms, stats=st, state=state,
                          updates_per_s=wl.np_ * args.steps / dt)
         shown = {k: v for k, v in st.items() if k != "collision_counts"}
         log(f"{name}: {ms:.3f} ms/step, {wl.np_ * args.steps / dt:.3e} updates/s, stats {shown}")
 
-    roof = roofline(wl, res, scheme, cfg, kw)
+    roof = roofline(wl, res, scheme, cfg, kw, kw_fp64 or kw)
     cpu, cpu_ref = None, None
     if not args.no_cpu:
         if cfg == "C1":
@@ -693,7 +719,8 @@ def main():
                    "interp": f"lag{scheme}", "launch": kw,
                    "solve_waves": m["stats"]["lm_waves"]},
         "fp64": {"value": round(f["updates_per_s"], 1), "ms_per_step": round(f["ms_per_step"], 4),
-                 "dtype": "f64", "roofline": roof["fp64"], "stage_ms": stage(f["stats"]),
+                 "dtype": "f64", "launch": kw_fp64 or kw, "roofline": roof["fp64"],
+                 "stage_ms": stage(f["stats"]),
                  "per_step": f["stats"]["per_step"]},
         "accuracy_gate": (accuracy_gate(acc["mixed_vs_fp64"], acc["fp64_vs_fp64_perturbed_1e-12"])
                           if acc else None),
