@@ -902,15 +902,25 @@ __global__ __launch_bounds__(BLOCK) void k_brick_scatter(const int* __restrict__
   inv[i] = pos;
 }
 
+// the order key of every particle, (ensemble, reference index), at its
+// cell's slot: the ranking then compares a cell's keys from consecutive
+// words instead of two dependent loads per member
+__device__ __forceinline__ unsigned long long order_key(int g, int id) {
+  return ((unsigned long long)(unsigned)g << 32) | (unsigned long long)(unsigned)id;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_scatter(const int* __restrict__ cell,
                                                    const int* __restrict__ slot,
                                                    const int* __restrict__ off,
-                                                   int* __restrict__ tmp, int np, SideClear sc) {
+                                                   const int* __restrict__ id,
+                                                   const int* __restrict__ grp,
+                                                   unsigned long long* __restrict__ tkey, int np,
+                                                   SideClear sc) {
   side_clear(sc);
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   int c = cell[i];
-  if (c >= 0) tmp[off[c] + slot[i]] = i;
+  if (c >= 0) tkey[off[c] + slot[i]] = order_key(grp ? grp[i] : 0, id[i]);
 }
 
 // stable order inside each cell: by ensemble, then reference index (the
@@ -918,7 +928,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(const int* __restrict__ cell,
 __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
                                                 const int* __restrict__ off,
                                                 const int* __restrict__ cnt,
-                                                const int* __restrict__ tmp,
+                                                const unsigned long long* __restrict__ tkey,
                                                 const int* __restrict__ id,
                                                 const int* __restrict__ grp,
                                                 const double* __restrict__ px,
@@ -935,18 +945,13 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int* __restrict__ cell,
   int i = block_index(xcd) * BLOCK + threadIdx.x;
   if (i >= np) return;
   int c = cell[i];
-  const int o = off[c], m = cnt[c], me = id[i];
+  const int o = off[c], m = cnt[c];
   int r = 0;
-  if (m == 1) {
-    // alone in its cell (the common case): rank 0, no member reads
-  } else if (grp) {
-    const int g = grp[i];
-    for (int t = 0; t < m; ++t) {
-      const int j = tmp[o + t], gj = grp[j];
-      r += (gj < g || (gj == g && id[j] < me)) ? 1 : 0;
-    }
-  } else {
-    for (int t = 0; t < m; ++t) r += (id[tmp[o + t]] < me) ? 1 : 0;
+  if (m > 1) {
+    // its rank among the cell's keys (alone in its cell, the common case at
+    // low density, it has rank 0 and reads none)
+    const unsigned long long me = order_key(grp ? grp[i] : 0, id[i]);
+    for (int t = 0; t < m; ++t) r += (tkey[o + t] < me) ? 1 : 0;
   }
   sorted[o + r] = i;
   if (inv) inv[i] = o + r;
@@ -2092,7 +2097,7 @@ struct gp_ctx {
   int* slot = nullptr;
   int* cnt = nullptr;
   int* off = nullptr;
-  int* tmp = nullptr;
+  unsigned long long* tkey = nullptr;   // order key per cell slot (sort)
   int* sorted = nullptr;
   int* pcount = nullptr;
   long long* poff = nullptr;
@@ -2702,7 +2707,7 @@ int gp_destroy(gp_ctx* ctx) {
   hipStreamSynchronize(ctx->st2);
   hipStreamSynchronize(ctx->st3);
   void* bufs[] = {ctx->field, ctx->spec, ctx->pbuf[0], ctx->pbuf[1], ctx->perm, ctx->cell,
-                  ctx->slot, ctx->cnt, ctx->off, ctx->tmp, ctx->sorted, ctx->pcount,
+                  ctx->slot, ctx->cnt, ctx->off, ctx->tkey, ctx->sorted, ctx->pcount,
                   ctx->poff, ctx->scan_scr, ctx->mark, ctx->used, ctx->pos_of_id,
                   ctx->work, ctx->lmwork, ctx->brick, ctx->bslot, ctx->bcnt, ctx->boff, ctx->trav,
                   ctx->pos4, ctx->csort, ctx->gsort, ctx->inlm, ctx->plist, ctx->inv, ctx->ccount, ctx->mlp_th, ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, ctx->mlp_wb,
@@ -2914,7 +2919,7 @@ int gp_set_particles(gp_ctx* ctx, int np, const double* state, const int* specie
   if ((rc = dalloc(ctx, &ctx->perm, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->cell, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->slot, (size_t)np))) return rc;
-  if ((rc = dalloc(ctx, &ctx->tmp, (size_t)np))) return rc;
+  if ((rc = dalloc(ctx, &ctx->tkey, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->sorted, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->pcount, (size_t)np))) return rc;
   if ((rc = dalloc(ctx, &ctx->poff, (size_t)np + 1))) return rc;
@@ -3352,11 +3357,11 @@ static int launch_sort(gp_ctx* ctx, bool offsets = true) {
     sc_cnt.zc2 = ctx->d_cnt + 13 + b;
     sc_cnt.nzc2 = 1;
   }
-  hipLaunchKernelGGL(k_scatter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->slot,
-                     ctx->off, ctx->tmp, np, sc_flags);
   const int* grp = ctx->ngroups > 1 ? ps.grp : nullptr;
+  hipLaunchKernelGGL(k_scatter, dim3(nblk(np)), dim3(BLOCK), 0, ctx->st, ctx->cell, ctx->slot,
+                     ctx->off, ps.id, grp, ctx->tkey, np, sc_flags);
   hipLaunchKernelGGL(k_rank, dim3(xcd_grid(ctx, np)), dim3(BLOCK), 0, ctx->st, ctx->cell,
-                     ctx->off, ctx->cnt, ctx->tmp, ps.id, grp, ps.x, ps.y, ps.z, ps.rm,
+                     ctx->off, ctx->cnt, ctx->tkey, ps.id, grp, ps.x, ps.y, ps.z, ps.rm,
                      ctx->sorted, ctx->pos4, ctx->csort, ctx->gsort,
                      ctx->use_bricks ? nullptr : ctx->inv, np, ctx->prm.xcd_remap, sc_cnt);
   if (offsets) {
