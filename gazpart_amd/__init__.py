@@ -206,7 +206,7 @@ class Context:
     the GPU, plus the reference-shaped entry points of the step."""
 
     def __init__(self, n, L, visc, rhof, dtime, *, scheme=3, precision=PREC_FP64, ecw=1.0,
-                 muw=0.0, dpmax=0.0, prefilter=2, xcd_remap=1, renorm_quat=0,
+                 muw=0.0, dpmax=0.0, prefilter=2, xcd_remap=0, renorm_quat=0,
                  keep_fluid=False, timing=False, graph=False, brick=0, lm_seed=0,
                  lm_waves=0, overlap=1, cell_rule=0, tile=0):
         self.lib = load_library()

@@ -102,7 +102,7 @@ contains
       p%muw = MUW
       p%dpmax = 2.0*maxval(EMAJ_PART(:,:))        ! collision_initiation.f90:50
       p%prefilter = 2
-      p%xcd_remap = 1
+      p%xcd_remap = 0
       p%keep_fluid = 0
       p%timing = 0
       p%graph = 0
@@ -374,7 +374,7 @@ contains
       p%muw = MUW
       p%dpmax = 0.0
       p%prefilter = 2
-      p%xcd_remap = 1
+      p%xcd_remap = 0
       p%keep_fluid = 0
       p%timing = 0
       p%graph = 0

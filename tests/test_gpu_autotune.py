@@ -38,6 +38,21 @@ def test_contact_launch_width_changes_no_result(dense, prec):
         np.testing.assert_array_equal(out[0], o)
 
 
+@pytest.mark.parametrize("graph", [0, 2])
+def test_xcd_block_order_changes_no_result(dense, graph):
+    """Workgroups placed round-robin over the XCDs (the default) or in one
+    contiguous range per XCD (xcd_remap = 1): the same bits, eager and in
+    pipelined steps with the pairs solved ahead."""
+    out = []
+    for x in (0, 1):
+        ctx = G.Context.from_workload(dense, precision=G.PREC_MIXED, graph=graph, overlap=1,
+                                      xcd_remap=x)
+        ctx.step(7)
+        out.append(ctx.get_particles())
+        ctx.close()
+    np.testing.assert_array_equal(out[0], out[1])
+
+
 def test_autotune_picks_the_fastest_of_its_table(dense):
     r = tune(dense, G.PREC_MIXED, steps=2, warmup=1, bricks=(-1, 4), tiles=(8,),
              lm_waves=(0, 256))
